@@ -1,0 +1,284 @@
+/*
+ * kinfu_amd.h - C ABI of the MI355X (gfx950) KinectFusion core.
+ *
+ * The reference exposes its GPU stages as five host wrappers in
+ * src/parallel_processing/parallel_processing.h:19-53 (Projective_TSDF,
+ * Ray_Casting, Calculate_Vertices_And_Normals, Estimate_Sensor_Pose,
+ * Bilateral_Filtering).  Those take cv::Mat& and C++ references, allocate and
+ * free every device buffer on each call and copy the whole volume across PCIe
+ * twice per frame.  This ABI replaces them with plain C types and an opaque
+ * context that owns every device buffer for its lifetime: the volume, the
+ * depth/vertex/normal maps, the raycast model maps and the ICP reduction
+ * scratch all stay resident in HBM.
+ *
+ * Conventions (identical to the reference):
+ *   - Units are millimetres.
+ *   - K is 3x3 row-major (9 floats), T is 4x4 row-major camera-to-world
+ *     (16 floats): p_world = T * [p_cam; 1]                 (projective_tsdf.cu:43-45)
+ *   - Pixels are 1-indexed when projected: K^-1 is applied to (x+1, y+1, 1)
+ *     and a projection (u, v) reads depth[(v-1)*W + (u-1)]  (raw_vertices.cu:25-30)
+ *   - The volume is dense, x-major / z-fastest:
+ *     index = ix*(Ny*Nz) + iy*Nz + iz; voxel centre = (i+start)*d + d/2
+ *     (kinfu_pipeline.cpp:47-58).  Unobserved voxels hold NaN TSDF.
+ *   - Images are H x W row-major.
+ *
+ * Every entry point returns KF_OK (0) or a negative KF_ERR_* code and never
+ * aborts; kf_last_error() gives the message of the last failure on the
+ * calling thread.  Work is enqueued on the context's own HIP stream; entry
+ * points that hand results to host memory synchronise that stream.  A
+ * context is used by one host thread at a time.
+ */
+#ifndef KINFU_AMD_H_
+#define KINFU_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KF_OK 0
+#define KF_ERR_INVALID (-1) /* bad argument / size / pointer            */
+#define KF_ERR_HIP (-2)     /* HIP runtime error                        */
+#define KF_ERR_OOM (-3)     /* device allocation failed                 */
+#define KF_ERR_STATE (-4)   /* call out of order (e.g. no frame loaded) */
+#define KF_ERR_IO (-5)      /* file could not be read or written        */
+
+#define KF_INTRINSIC_ELEMENTS 9  /* parallel_processing.h:16 */
+#define KF_EXTRINSIC_ELEMENTS 16 /* parallel_processing.h:17 */
+
+#define KF_TSDF_F32 0 /* float TSDF + float weight (the reference's storage)  */
+#define KF_TSDF_F16 1 /* half TSDF + uint16 weight (precision variant)        */
+
+#define KF_MEM_HOST 0   /* pointer argument is host memory                   */
+#define KF_MEM_DEVICE 1 /* pointer argument is device memory (this GPU)      */
+
+#define KF_MAX_ICP_ITERS 64
+#define KF_MAX_BILATERAL_RADIUS 8
+#define KF_ICP_SUMS 28 /* 21 unique J^TJ, 6 J^Tr, inlier count          */
+
+/* Stage indices for kf_stage_times(). */
+#define KF_STAGE_DEPTH 0     /* uint16 -> float mm + range gate                */
+#define KF_STAGE_BILATERAL 1 /* bilateral filter                              */
+#define KF_STAGE_VERTEX 2    /* vertex + normal maps                          */
+#define KF_STAGE_ICP 3       /* all ICP iterations of one frame               */
+#define KF_STAGE_INTEGRATE 4 /* projective TSDF integration, per-voxel kernel  */
+#define KF_STAGE_RAYCAST 5   /* voxel-traversal raycast                       */
+#define KF_STAGE_INTEG_PREP 6 /* integration per-pixel table                  */
+#define KF_STAGE_COUNT 7
+
+/* Parameter presets for kf_preset(). */
+#define KF_PRESET_REFERENCE 100 /* the reference's compiled-in defaults          */
+#define KF_PRESET_CONFIG0 0     /* 640x480, 360x360x300 @10mm, FP32 (CPU case)    */
+#define KF_PRESET_CONFIG1 1     /* same geometry, GPU stream                      */
+#define KF_PRESET_CONFIG2 2     /* 720x720x600 @5mm, FP16+u16                     */
+#define KF_PRESET_CONFIG3 3     /* 128^3 @10mm, 320x240 differentiable render     */
+#define KF_PRESET_CONFIG4 4     /* 1024x1024x768 @4mm, FP16+u16, 1280x720         */
+
+typedef struct kf_context kf_context;
+
+/* Dense volume.  Replaces the _voxel_* constants of kinfu_pipeline.h:42-49. */
+typedef struct kf_volume_desc {
+  int32_t dims[3];   /* voxel_length (x), voxel_width (y), voxel_height (z) */
+  int32_t start[3];  /* voxel_grid_{x,y,z}_start, in voxels                 */
+  float voxel_mm;    /* voxel_distance                                      */
+  int32_t precision; /* KF_TSDF_F32 or KF_TSDF_F16                          */
+} kf_volume_desc;
+
+/* Per-context parameters.  In the reference every one of these is a
+ * compile-time constant (kinfu_pipeline.h:52-62, system_interface.cpp:10-11,
+ * sensor_pose_estimation.cu:48,69,339). */
+typedef struct kf_params {
+  float truncation_mm;      /* _truncated_distance                       */
+  float sdf_min_mm;         /* _sdf_minimum                              */
+  float sdf_max_mm;         /* _sdf_maximum                              */
+  float depth_min_mm;       /* min_depth (inclusive)                     */
+  float depth_max_mm;       /* max_depth (inclusive)                     */
+  int32_t bilateral;        /* 1: filter each frame before vertex maps   */
+  int32_t bilateral_radius; /* _bw_radius, <= KF_MAX_BILATERAL_RADIUS    */
+  float sigma_s_px;         /* _sigma_d                                  */
+  float sigma_r_mm;         /* _sigma_r                                  */
+  int32_t icp_iterations;   /* <= KF_MAX_ICP_ITERS                       */
+  float icp_dist_mm;        /* point distance gate                       */
+  float icp_angle_deg;      /* normal angle gate                         */
+  int32_t icp_compensated;  /* 1: Kahan/TwoSum compensated FP32 sums     */
+  int32_t record_icp;       /* 1: keep per-iteration inlier masks        */
+  int32_t stage_timing;     /* 1: hipEvent timing of every stage         */
+  int32_t raycast_fix_bounds; /* 0: the reference's zero-crossing bound
+                                 checks (voxel_traversal.cu:233-281, which
+                                 test ZX/ZY against voxel_height); 1: test
+                                 every index against its own axis        */
+} kf_params;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Fills vol/prm/K/height/width with a preset (KF_PRESET_*).  Any output
+ * pointer may be NULL. */
+int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9],
+              int32_t* height, int32_t* width);
+
+/* Creates a context on the current HIP device, allocates every buffer for an
+ * H x W image and the volume, and fills the volume with "never observed"
+ * (NaN TSDF; weight NaN for F32 / 0 for F16) as kinfu_pipeline.cpp:61-64. */
+int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width,
+              const kf_params* prm, kf_context** out);
+int kf_destroy(kf_context* ctx);
+const char* kf_last_error(void);
+int kf_set_params(kf_context* ctx, const kf_params* prm);
+int kf_get_params(const kf_context* ctx, kf_params* prm);
+/* Re-fills the volume with "never observed" and the pose with identity. */
+int kf_reset(kf_context* ctx);
+int kf_synchronize(kf_context* ctx);
+/* Bytes of device memory the context holds. */
+int kf_device_bytes(const kf_context* ctx, size_t* bytes);
+
+/* ---- A1: depth input (system_utility.cpp:58-68, 77-86) -------------------
+ * Converts uint16 mm to float mm and zeroes values outside
+ * [depth_min_mm, depth_max_mm] (inclusive both ends, cv::inRange). */
+int kf_upload_depth_u16(kf_context* ctx, const uint16_t* depth, int32_t mem_kind);
+/* Loads an already-gated float depth map unchanged (stage tests, callers
+ * holding float depth as the reference's cv::Mat CV_32F does). */
+int kf_upload_depth_f32(kf_context* ctx, const float* depth_mm, int32_t mem_kind);
+/* Reads a raw 640x480-style uint16 little-endian .dat frame
+ * (system_utility.cpp:58-68 reads "../data/dep%04d.dat") into host memory. */
+int kf_read_depth_file(const char* path, int32_t height, int32_t width, uint16_t* out);
+int kf_download_depth(kf_context* ctx, float* depth_mm);
+
+/* ---- A4: bilateral filter (bilateral_filtering.cu:3-108) -----------------
+ * Replaces the working depth by its bilateral filtering with the context's
+ * radius/sigmas.  Pixels outside [r, H-r] x [r, W-r] become 0, as the
+ * reference's zero-initialised output copied over the depth does. */
+int kf_bilateral(kf_context* ctx);
+/* The (2r+1)^2 spatial weights exp(-(dx^2+dy^2)/(2 sigma^2)) the filter uses
+ * (system_utility.cpp:88-113), row-major by (dy+r, dx+r). */
+int kf_spatial_kernel(int32_t radius, float sigma_s_px, float* weights);
+
+/* ---- A3: vertex and normal maps (raw_vertices.cu:3-126) ------------------ */
+int kf_vertex_normals(kf_context* ctx, const float K[9]);
+/* vz is the working depth.  Any pointer may be NULL. */
+int kf_download_vertex_maps(kf_context* ctx, float* vx, float* vy, float* vz,
+                            float* nx, float* ny, float* nz, uint8_t* mask);
+/* Loads vertex/normal/mask planes (ICP stage tests); vz becomes the depth. */
+int kf_upload_vertex_maps(kf_context* ctx, const float* vx, const float* vy,
+                          const float* vz, const float* nx, const float* ny,
+                          const float* nz, const uint8_t* mask);
+
+/* ---- A5-A7: point-to-plane ICP (sensor_pose_estimation.cu:7-470) ---------
+ * T_inout holds the previous pose on entry (the pose the model maps were
+ * raycast at) and the refined pose on return. */
+int kf_estimate_pose(kf_context* ctx, const float K[9], float T_inout[16]);
+/* Per-iteration record of the last kf_estimate_pose / pipeline frame:
+ * pose after each iteration [n][16], FP64 sums [n][KF_ICP_SUMS] (order:
+ * upper triangle of J^TJ row-major, J^Tr, count), solver status [n]
+ * (0 = LDLT solved, 1 = not positive definite: increment = right-hand side),
+ * inlier masks [n][H*W] when record_icp is set.  Any pointer may be NULL. */
+int kf_icp_history(kf_context* ctx, int32_t* n_iters, float* T_iters,
+                   double* sums, int32_t* status, uint8_t* masks);
+
+/* ---- A8: projective TSDF integration (projective_tsdf.cu:5-214) ---------- */
+int kf_integrate(kf_context* ctx, const float K[9], const float T[16],
+                 int32_t first_frame);
+/* Volume in the reference's layout: two float arrays (F32 contexts copy, F16
+ * contexts widen; F16 weight 0 is returned as NaN, "never observed"). */
+int kf_download_volume(kf_context* ctx, float* tsdf, float* weight);
+int kf_upload_volume(kf_context* ctx, const float* tsdf, const float* weight);
+/* Native storage: F32 = interleaved {tsdf, weight} float pairs,
+ * F16 = uint32 {half tsdf (bits 0-15), uint16 weight (bits 16-31)}. */
+int kf_download_volume_raw(kf_context* ctx, void* out, size_t bytes);
+int kf_upload_volume_raw(kf_context* ctx, const void* in, size_t bytes);
+/* Device address of the native volume (for zero-copy consumers). */
+int kf_volume_device_ptr(kf_context* ctx, void** ptr, size_t* bytes);
+
+/* ---- A9-A10: voxel-traversal raycast (voxel_traversal.cu:4-544) ---------
+ * Skips along each pixel ray to (depth - truncation), then walks the
+ * 1-indexed Amanatides-Woo traversal to the first +/- zero crossing. */
+int kf_raycast(kf_context* ctx, const float K[9], const float T[16]);
+/* Model maps in world mm; zero where no surface was found.  hit_index is the
+ * 0-based linear index of the voxel where the crossing was detected, -1 for
+ * no hit.  Any pointer may be NULL. */
+int kf_download_model_maps(kf_context* ctx, float* x, float* y, float* z,
+                           float* nx, float* ny, float* nz, int32_t* hit_index);
+int kf_upload_model_maps(kf_context* ctx, const float* x, const float* y,
+                         const float* z, const float* nx, const float* ny,
+                         const float* nz);
+
+/* ---- frame sequence (KinfuPipeline, kinfu_pipeline.cpp:91-193) ----------- */
+/* Keeps n frames of uint16 depth resident in HBM for the pipeline calls. */
+int kf_stream_upload_u16(kf_context* ctx, const uint16_t* frames, int32_t n_frames,
+                         int32_t mem_kind);
+/* InitialProcessing: depth(frame) -> [bilateral] -> integrate(first) ->
+ * raycast, at the context pose. */
+int kf_pipeline_initial(kf_context* ctx, const float K[9], int32_t frame);
+/* IncomingFrameProcessing: depth(frame) -> [bilateral] -> vertex/normal ->
+ * ICP (updates the pose on the device) -> integrate -> raycast.  Enqueued
+ * only; nothing is copied to the host. */
+int kf_pipeline_incoming(kf_context* ctx, const float K[9], int32_t frame);
+/* initial(first) then incoming(first+1 .. first+count-1); enqueued only. */
+int kf_pipeline_run(kf_context* ctx, const float K[9], int32_t first, int32_t count);
+int kf_get_pose(kf_context* ctx, float T[16]);
+int kf_set_pose(kf_context* ctx, const float T[16]);
+/* Pose after each processed stream frame, [n][16] (frame order). */
+int kf_download_poses(kf_context* ctx, float* poses, int32_t n_frames);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Accumulated milliseconds and launch counts per KF_STAGE_*, from hipEvents
+ * recorded on the context stream while stage_timing is set. */
+int kf_stage_times(kf_context* ctx, double* ms, int64_t* counts);
+int kf_reset_stage_times(kf_context* ctx);
+
+/* ---- conventions exposed for tests ---------------------------------------
+ * The angle gates are evaluated as cosine thresholds that are exactly
+ * equivalent to the reference's acos tests for a correctly rounded acos:
+ *   ICP  (sensor_pose_estimation.cu:63-69): acosf(c)*(180/pi_f) <= gate
+ *        <=>  icp_lo <= c <= 1
+ *   flip (voxel_traversal.cu:361-364): acos((double)c)*(180/pi_f) > 90
+ *        <=>  -1 <= c < flip_hi                                          */
+int kf_gate_thresholds(float icp_angle_deg, float* icp_lo, float* flip_hi);
+/* The exp used by the bilateral filter and its spatial kernel. */
+float kf_expf_abi(float x);
+/* FP64 helpers rounded to FP32, as the kernels use them. */
+int kf_mat4_inverse(const float T[16], float Tinv[16]);
+int kf_mat3_inverse(const float K[9], float Kinv[9]);
+int kf_ldlt6(const double A[36], const double b[6], double x[6]);
+
+/* ---- synthetic RGB-D stream ----------------------------------------------
+ * Box room with spheres and axis-aligned boxes, analytic raycast, Kinect
+ * axial noise sigma(z) = 0.0012 + 0.0019 (z - 0.4)^2 m, random holes,
+ * depth zeroed beyond max_depth. Replaces the reference's data/ frames. */
+#define KF_SYNTH_MAX_SPHERES 16
+#define KF_SYNTH_MAX_BOXES 16
+typedef struct kf_synth_scene {
+  float room_min[3], room_max[3];               /* mm, seen from inside */
+  int32_t n_spheres;
+  float spheres[KF_SYNTH_MAX_SPHERES][4];       /* cx, cy, cz, r (mm)   */
+  int32_t n_boxes;
+  float boxes[KF_SYNTH_MAX_BOXES][6];           /* min xyz, max xyz     */
+} kf_synth_scene;
+
+int kf_synth_build_scene(int32_t n_spheres, int32_t n_boxes, uint64_t seed,
+                         const float room_min[3], const float room_max[3],
+                         kf_synth_scene* out);
+/* Ground-truth camera-to-world pose of the orbit trajectory (frame 0 =
+ * identity, <= 1 cm and 0.5 deg per frame). */
+int kf_synth_orbit_pose(int32_t frame, float T[16]);
+/* Renders one uint16 mm depth frame.  noise_scale multiplies sigma(z)
+ * (0 = noiseless); clean_mm (nullable) receives the exact float z-depth. */
+int kf_synth_render(const kf_synth_scene* scene, const float K[9], const float T[16],
+                    int32_t height, int32_t width, float noise_scale, float hole_frac,
+                    float max_depth_mm, uint64_t noise_seed, uint16_t* depth,
+                    float* clean_mm, int32_t n_threads);
+/* Renders frames [first, first+n) of the orbit: noise seed 1000+frame. */
+int kf_synth_stream(const kf_synth_scene* scene, const float K[9], int32_t height,
+                    int32_t width, int32_t first, int32_t n, float noise_scale,
+                    float hole_frac, float max_depth_mm, uint16_t* frames,
+                    float* gt_poses, int32_t n_threads);
+/* The scene each preset uses (6 spheres + 4 boxes for configs 0-2, 12 + 8
+ * for config 4; seed 0; room inset from the volume by truncation + 2 voxels). */
+int kf_synth_preset_scene(int32_t which, kf_synth_scene* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KINFU_AMD_H_ */

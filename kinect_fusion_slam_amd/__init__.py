@@ -1,0 +1,17 @@
+"""MI355X (gfx950) KinectFusion core: the per-frame GPU stages of
+PoChang007/Kinect_Fusion_SLAM (bilateral filter, vertex/normal maps,
+point-to-plane ICP, projective TSDF integration, voxel-traversal raycast)
+as hand-written HIP kernels behind the C ABI in include/kinfu_amd.h.
+
+This package is the Python view of that ABI (ctypes).  Every stage runs in
+libkinfu_amd.so; importing without the built library fails.
+"""
+from ._abi import (KF_MEM_DEVICE, KF_MEM_HOST, KF_TSDF_F16, KF_TSDF_F32, PRESET_CONFIG,  # noqa: F401
+                   PRESET_REFERENCE, STAGES, KinfuContext, KinfuError, Params, SynthScene,
+                   VolumeDesc, exported_symbols, gate_thresholds, ldlt6, lib, mat3_inverse,
+                   mat4_inverse, orbit_pose, preset, preset_scene, render, spatial_kernel,
+                   stream)
+
+__all__ = ["KinfuContext", "KinfuError", "VolumeDesc", "Params", "SynthScene", "preset",
+           "preset_scene", "orbit_pose", "render", "stream", "gate_thresholds", "mat4_inverse",
+           "mat3_inverse", "ldlt6", "spatial_kernel", "lib", "exported_symbols"]

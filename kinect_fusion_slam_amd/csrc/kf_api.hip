@@ -1,0 +1,963 @@
+// kf_api.hip - the C ABI: context lifetime, transfers, stage entry points and
+// the KinfuPipeline frame sequence.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "kf_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_last_error;
+
+void kf_set_error(const std::string& msg) { g_last_error = msg; }
+
+int kf_fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define KF_REQUIRE(cond, code, msg) \
+  do {                              \
+    if (!(cond)) return kf_fail((code), (msg)); \
+  } while (0)
+
+#define KF_LAUNCH(expr)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return kf_fail(KF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// small kernels owned by this file
+
+namespace {
+
+struct kf_pose_arg {
+  float T[16];
+  float Tinv[16];
+};
+
+__global__ void k_set_pose(kf_dev_state* st, kf_pose_arg p) {
+  const int i = threadIdx.x;
+  if (i < 16) {
+    st->T[i] = p.T[i];
+    st->Tinv[i] = p.Tinv[i];
+  }
+}
+
+// Magic multiplier for floor(n / D), n < 2^31: q = (umulhi(n, m) + n) >> s.
+void magic_for(uint32_t D, uint32_t* m, int32_t* s) {
+  int l = 0;
+  while ((1ull << l) < D) ++l;
+  const uint64_t mm = (((1ull << 32) * ((1ull << l) - D)) / D) + 1;
+  *m = (uint32_t)mm;
+  *s = l;
+}
+
+// Order-preserving map of floats to integers (for bisection over floats).
+int64_t f2ord(float f) {
+  int32_t b;
+  memcpy(&b, &f, 4);
+  return b >= 0 ? (int64_t)b : -(int64_t)(b & 0x7fffffff);
+}
+float ord2f(int64_t o) {
+  int32_t b = o >= 0 ? (int32_t)o : (int32_t)((-o) | 0x80000000ll);
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+const float kDegF = 180.0f / 3.141592654f;  // 180.0f / CUDART_PI_F
+
+// Smallest float c in [-1, 1] with (float)acos(c) * kDegF <= gate, i.e. the
+// reference's ICP normal gate (sensor_pose_estimation.cu:63-69) for a
+// correctly rounded acosf.  acos is monotone, so the gate holds exactly on
+// [c, 1].
+float icp_cos_threshold(float gate_deg) {
+  auto pass = [&](float c) {
+    const float ang = (float)acos((double)c) * kDegF;
+    return ang <= gate_deg && ang >= 0.0f;
+  };
+  if (pass(-1.0f)) return -1.0f;
+  if (!pass(1.0f)) return 2.0f;  // nothing passes
+  int64_t lo = f2ord(-1.0f), hi = f2ord(1.0f);  // pass(lo) false, pass(hi) true
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (pass(ord2f(mid)))
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return ord2f(hi);
+}
+
+// Smallest float c with !(acos((double)c) * kDegF > 90): the normal is flipped
+// (voxel_traversal.cu:361-369) exactly for -1 <= c < this value.
+float flip_cos_threshold() {
+  auto flip = [](float c) { return acos((double)c) * (double)kDegF > 90; };
+  int64_t lo = f2ord(-1.0f), hi = f2ord(1.0f);  // flip(lo) true, flip(hi) false
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (flip(ord2f(mid)))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return ord2f(hi);
+}
+
+bool params_valid(const kf_params* p, std::string* why) {
+  if (p->bilateral_radius < 1 || p->bilateral_radius > KF_MAX_BILATERAL_RADIUS) {
+    *why = "bilateral_radius must be in [1, 8]";
+    return false;
+  }
+  if (p->icp_iterations < 0 || p->icp_iterations > KF_MAX_ICP_ITERS) {
+    *why = "icp_iterations must be in [0, 64]";
+    return false;
+  }
+  if (!(p->truncation_mm > 0) || !(p->sigma_s_px > 0) || !(p->sigma_r_mm > 0)) {
+    *why = "truncation and sigmas must be positive";
+    return false;
+  }
+  return true;
+}
+
+void host_spatial_kernel(int r, float sigma, float* w) {
+  const int L = 2 * r + 1;
+  for (int i = 0; i < L; ++i)
+    for (int j = 0; j < L; ++j) {
+      const float X = (float)(j - r), Y = (float)(i - r);
+      const float t = -(X * X + Y * Y) / (2 * sigma * sigma);
+      w[i * L + j] = kf_expf(t);
+    }
+}
+
+}  // namespace
+
+void kf_make_cam(const float* K, kf_cam* cam) {
+  for (int i = 0; i < 9; ++i) cam->K[i] = K[i];
+  if (kf_m3_inverse(K, cam->Kinv) != 0)
+    for (int i = 0; i < 9; ++i) cam->Kinv[i] = NAN;
+}
+
+// ---------------------------------------------------------------------------
+// stage timing: a pair of events per stage invocation, drained on demand
+
+namespace {
+struct PendingEv {
+  int stage;
+  hipEvent_t a, b;
+};
+}  // namespace
+
+struct kf_timing {
+  std::vector<PendingEv> pending;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t open[KF_STAGE_COUNT] = {};
+};
+
+static kf_timing* timing_of(kf_context* c);
+
+void kf_stage_begin(kf_context* c, int stage) {
+  if (!c->prm.stage_timing) return;
+  kf_timing* t = timing_of(c);
+  hipEvent_t e;
+  if (!t->pool.empty()) {
+    e = t->pool.back();
+    t->pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  (void)hipEventRecord(e, c->stream);
+  t->open[stage] = e;
+}
+
+void kf_stage_end(kf_context* c, int stage) {
+  if (!c->prm.stage_timing) return;
+  kf_timing* t = timing_of(c);
+  if (!t->open[stage]) return;
+  hipEvent_t e;
+  if (!t->pool.empty()) {
+    e = t->pool.back();
+    t->pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  (void)hipEventRecord(e, c->stream);
+  t->pending.push_back({stage, t->open[stage], e});
+  t->open[stage] = nullptr;
+}
+
+// The timing record hangs off the context through a side table so that
+// kf_context stays a plain struct shared with the kernels' header.
+#include <map>
+#include <mutex>
+static std::mutex g_timing_mu;
+static std::map<const kf_context*, kf_timing*> g_timing;
+static kf_timing* timing_of(kf_context* c) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  auto it = g_timing.find(c);
+  if (it != g_timing.end()) return it->second;
+  kf_timing* t = new kf_timing();
+  g_timing[c] = t;
+  return t;
+}
+static void timing_drop(kf_context* c) {
+  kf_timing* t = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    auto it = g_timing.find(c);
+    if (it == g_timing.end()) return;
+    t = it->second;
+    g_timing.erase(it);
+  }
+  for (auto& p : t->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : t->pool) (void)hipEventDestroy(e);
+  for (auto e : t->open)
+    if (e) (void)hipEventDestroy(e);
+  delete t;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* kf_last_error(void) { return g_last_error.c_str(); }
+
+int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], int32_t* height,
+              int32_t* width) {
+  kf_volume_desc v{};
+  kf_params p{};
+  float k[9] = {525.0f, 0.0f, 319.5f, 0.0f, 525.0f, 239.5f, 0.0f, 0.0f, 1.0f};
+  int H = 480, W = 640;
+  // settings shared by the synthetic configs.  The SDF band keeps the
+  // reference's shape [-trunc, 10 trunc] (kinfu_pipeline.h:52-55 has trunc
+  // 300, band [-30, 300]): the traversal's interpolation taps reach back to
+  // voxels several steps in front of the surface, which must be observed.
+  v = {{360, 360, 300}, {-200, -200, -120}, 10.0f, KF_TSDF_F32};
+  p.truncation_mm = 30.0f;
+  p.sdf_min_mm = -30.0f;
+  p.sdf_max_mm = 300.0f;
+  p.depth_min_mm = 300.0f;
+  p.depth_max_mm = 4000.0f;
+  p.bilateral = 1;
+  p.bilateral_radius = 2;
+  p.sigma_s_px = 4.5f;
+  p.sigma_r_mm = 30.0f;
+  p.icp_iterations = 10;
+  p.icp_dist_mm = 100.0f;
+  p.icp_angle_deg = 20.0f;
+  p.icp_compensated = 0;
+  p.record_icp = 0;
+  p.stage_timing = 0;
+  p.raycast_fix_bounds = 0;
+  switch (which) {
+    case KF_PRESET_REFERENCE: {
+      // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,
+      // sensor_pose_estimation.cu:48, 69, 339; bilateral disabled (kinfu_pipeline.cpp:141-145)
+      const float kr[9] = {524.639f, 0.0f, 316.625f, 0.0f, 523.503f, 256.2318f, 0.0f, 0.0f, 1.0f};
+      memcpy(k, kr, sizeof k);
+      p.truncation_mm = 300.0f;
+      p.sdf_min_mm = -30.0f;
+      p.sdf_max_mm = 300.0f;
+      p.depth_max_mm = 2000.0f;
+      p.bilateral = 0;
+      p.bilateral_radius = 3;
+      p.icp_dist_mm = 20.0f;
+      p.icp_angle_deg = 30.0f;
+      break;
+    }
+    case KF_PRESET_CONFIG0:
+    case KF_PRESET_CONFIG1:
+      break;
+    case KF_PRESET_CONFIG2:
+      v = {{720, 720, 600}, {-400, -400, -240}, 5.0f, KF_TSDF_F16};
+      p.truncation_mm = 15.0f;
+      p.sdf_min_mm = -15.0f;
+      p.sdf_max_mm = 150.0f;
+      p.icp_compensated = 1;
+      break;
+    case KF_PRESET_CONFIG3: {
+      v = {{128, 128, 128}, {-64, -64, 36}, 10.0f, KF_TSDF_F32};
+      const float k3[9] = {262.5f, 0.0f, 159.5f, 0.0f, 262.5f, 119.5f, 0.0f, 0.0f, 1.0f};
+      memcpy(k, k3, sizeof k);
+      H = 240;
+      W = 320;
+      p.bilateral = 0;
+      break;
+    }
+    case KF_PRESET_CONFIG4: {
+      v = {{1024, 1024, 768}, {-512, -512, -300}, 4.0f, KF_TSDF_F16};
+      const float k4[9] = {1000.0f, 0.0f, 639.5f, 0.0f, 1000.0f, 359.5f, 0.0f, 0.0f, 1.0f};
+      memcpy(k, k4, sizeof k);
+      H = 720;
+      W = 1280;
+      p.truncation_mm = 20.0f;
+      p.sdf_min_mm = -20.0f;
+      p.sdf_max_mm = 200.0f;
+      p.icp_compensated = 1;
+      break;
+    }
+    default:
+      return kf_fail(KF_ERR_INVALID, "unknown preset");
+  }
+  if (vol) *vol = v;
+  if (prm) *prm = p;
+  if (K) memcpy(K, k, sizeof k);
+  if (height) *height = H;
+  if (width) *width = W;
+  return KF_OK;
+}
+
+int kf_gate_thresholds(float icp_angle_deg, float* icp_lo, float* flip_hi) {
+  if (icp_lo) *icp_lo = icp_cos_threshold(icp_angle_deg);
+  if (flip_hi) *flip_hi = flip_cos_threshold();
+  return KF_OK;
+}
+
+float kf_expf_abi(float x) { return kf_expf(x); }
+
+int kf_mat4_inverse(const float T[16], float Tinv[16]) {
+  KF_REQUIRE(T && Tinv, KF_ERR_INVALID, "null pointer");
+  return kf_m4_inverse((const float*)T, (float*)Tinv) ? kf_fail(KF_ERR_INVALID, "singular")
+                                                         : KF_OK;
+}
+
+int kf_mat3_inverse(const float K[9], float Kinv[9]) {
+  KF_REQUIRE(K && Kinv, KF_ERR_INVALID, "null pointer");
+  return kf_m3_inverse((const float*)K, (float*)Kinv) ? kf_fail(KF_ERR_INVALID, "singular")
+                                                         : KF_OK;
+}
+
+int kf_ldlt6(const double A[36], const double b[6], double x[6]) {
+  KF_REQUIRE(A && b && x, KF_ERR_INVALID, "null pointer");
+  return kf_ldlt6_solve(A, b, x);
+}
+
+int kf_spatial_kernel(int32_t radius, float sigma_s_px, float* weights) {
+  KF_REQUIRE(weights && radius >= 1 && radius <= KF_MAX_BILATERAL_RADIUS, KF_ERR_INVALID,
+             "bad spatial kernel arguments");
+  host_spatial_kernel(radius, sigma_s_px, weights);
+  return KF_OK;
+}
+
+int kf_destroy(kf_context* c) {
+  if (!c) return KF_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  timing_drop(c);
+  void* bufs[] = {c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
+                  c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
+                  c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
+                  c->frames,  c->poses};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return KF_OK;
+}
+
+static int apply_params(kf_context* c, const kf_params* prm) {
+  std::string why;
+  KF_REQUIRE(params_valid(prm, &why), KF_ERR_INVALID, why);
+  c->prm = *prm;
+  float w[(2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)];
+  host_spatial_kernel(prm->bilateral_radius, prm->sigma_s_px, w);
+  const int L = 2 * prm->bilateral_radius + 1;
+  KF_HIP_CHECK(hipMemcpy(c->spatial, w, sizeof(float) * L * L, hipMemcpyHostToDevice));
+  c->icp_cos_lo = icp_cos_threshold(prm->icp_angle_deg);
+  c->flip_cos_hi = flip_cos_threshold();
+  if (prm->record_icp && !c->icp_masks) {
+    const size_t bytes = (size_t)KF_MAX_ICP_ITERS * c->H * c->W;
+    if (hipMalloc(&c->icp_masks, bytes) != hipSuccess) {
+      c->icp_masks = nullptr;
+      return kf_fail(KF_ERR_OOM, "cannot allocate ICP mask record");
+    }
+    KF_HIP_CHECK(hipMemset(c->icp_masks, 0, bytes));
+  }
+  return KF_OK;
+}
+
+int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf_params* prm,
+              kf_context** out) {
+  KF_REQUIRE(vol && prm && out, KF_ERR_INVALID, "null argument");
+  *out = nullptr;
+  KF_REQUIRE(height > 1 && width > 1 && (int64_t)height * width < (1ll << 28), KF_ERR_INVALID,
+             "bad image size");
+  for (int i = 0; i < 3; ++i)
+    KF_REQUIRE(vol->dims[i] >= 3 && vol->dims[i] <= 65536, KF_ERR_INVALID, "bad volume dims");
+  const int64_t nv = (int64_t)vol->dims[0] * vol->dims[1] * vol->dims[2];
+  KF_REQUIRE(nv < (1ll << 31) - 1024, KF_ERR_INVALID, "volume must hold fewer than 2^31 voxels");
+  KF_REQUIRE(vol->voxel_mm > 0, KF_ERR_INVALID, "voxel size must be positive");
+  KF_REQUIRE(vol->precision == KF_TSDF_F32 || vol->precision == KF_TSDF_F16, KF_ERR_INVALID,
+             "unknown precision");
+  std::string why;
+  KF_REQUIRE(params_valid(prm, &why), KF_ERR_INVALID, why);
+
+  kf_context* c = new kf_context();
+  c->H = height;
+  c->W = width;
+  c->vol = *vol;
+  c->n_voxels = nv;
+  c->vg.nx = vol->dims[0];
+  c->vg.ny = vol->dims[1];
+  c->vg.nz = vol->dims[2];
+  c->vg.sx = vol->start[0];
+  c->vg.sy = vol->start[1];
+  c->vg.sz = vol->start[2];
+  c->vg.d = vol->voxel_mm;
+  magic_for((uint32_t)vol->dims[2], &c->vg.mz, &c->vg.shz);
+  magic_for((uint32_t)vol->dims[1], &c->vg.my, &c->vg.shy);
+  const size_t n = (size_t)height * width;
+  c->icp_blocks = (int)((n + KF_ICP_BLOCK * KF_ICP_PIX_PER_THREAD - 1) /
+                        (KF_ICP_BLOCK * KF_ICP_PIX_PER_THREAD));
+  c->volume_bytes = (size_t)nv * (vol->precision == KF_TSDF_F32 ? 8 : 4);
+
+  auto fail = [&](int code, const std::string& m) {
+    kf_destroy(c);
+    return kf_fail(code, m);
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return fail(KF_ERR_HIP, "no HIP device");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(KF_ERR_HIP, "hipStreamCreate failed");
+  struct Alloc {
+    void** p;
+    size_t bytes;
+  } allocs[] = {
+      {(void**)&c->depth, n * 4},       {(void**)&c->depth_tmp, n * 4},
+      {(void**)&c->depth_u16, n * 2},   {(void**)&c->vmap, n * 16},
+      {(void**)&c->nmap, n * 16},       {(void**)&c->mask, n},
+      {(void**)&c->model_v, n * 16},    {(void**)&c->model_n, n * 16},
+      {(void**)&c->hit, n * 4},         {(void**)&c->integ_tab, n * 8},
+      {(void**)&c->spatial, 4 * (2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)},
+      {(void**)&c->state, sizeof(kf_dev_state)},
+      {(void**)&c->icp_partials, (size_t)c->icp_blocks * KF_ICP_SUMS * 2 * 4},
+      {&c->volume, c->volume_bytes},
+  };
+  for (auto& a : allocs) {
+    if (hipMalloc(a.p, a.bytes) != hipSuccess) {
+      *a.p = nullptr;
+      return fail(KF_ERR_OOM, "device allocation failed");
+    }
+  }
+  int rc = apply_params(c, prm);
+  if (rc != KF_OK) {
+    const std::string m = g_last_error;
+    return fail(rc, m);
+  }
+  void* zero_bufs[] = {c->depth, c->depth_tmp, c->vmap, c->nmap, c->mask, c->model_v, c->model_n};
+  size_t zero_sizes[] = {n * 4, n * 4, n * 16, n * 16, n, n * 16, n * 16};
+  for (int i = 0; i < 7; ++i)
+    if (hipMemsetAsync(zero_bufs[i], 0, zero_sizes[i], c->stream) != hipSuccess)
+      return fail(KF_ERR_HIP, "hipMemsetAsync failed");
+  if (hipMemsetAsync(c->hit, 0xff, n * 4, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->state, 0, sizeof(kf_dev_state), c->stream) != hipSuccess)
+    return fail(KF_ERR_HIP, "hipMemsetAsync failed");
+  *out = c;
+  rc = kf_reset(c);
+  if (rc != KF_OK) {
+    *out = nullptr;
+    const std::string m = g_last_error;
+    return fail(rc, m);
+  }
+  return KF_OK;
+}
+
+int kf_set_params(kf_context* c, const kf_params* prm) {
+  KF_REQUIRE(c && prm, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return apply_params(c, prm);
+}
+
+int kf_get_params(const kf_context* c, kf_params* prm) {
+  KF_REQUIRE(c && prm, KF_ERR_INVALID, "null argument");
+  *prm = c->prm;
+  return KF_OK;
+}
+
+int kf_set_pose(kf_context* c, const float T[16]) {
+  KF_REQUIRE(c && T, KF_ERR_INVALID, "null argument");
+  kf_pose_arg a;
+  memcpy(a.T, T, sizeof a.T);
+  KF_REQUIRE(kf_m4_inverse(a.T, a.Tinv) == 0, KF_ERR_INVALID, "pose is singular");
+  k_set_pose<<<1, 64, 0, c->stream>>>(c->state, a);
+  KF_HIP_CHECK(hipGetLastError());
+  return KF_OK;
+}
+
+int kf_reset(kf_context* c) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  KF_LAUNCH(kf_launch_volume_fill(c));
+  const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  return kf_set_pose(c, I);
+}
+
+int kf_synchronize(kf_context* c) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+int kf_device_bytes(const kf_context* c, size_t* bytes) {
+  KF_REQUIRE(c && bytes, KF_ERR_INVALID, "null argument");
+  const size_t n = (size_t)c->H * c->W;
+  size_t b = c->volume_bytes + n * (4 + 4 + 2 + 16 + 16 + 1 + 16 + 16 + 4 + 8) + sizeof(kf_dev_state) +
+             (size_t)c->icp_blocks * KF_ICP_SUMS * 8;
+  if (c->icp_masks) b += (size_t)KF_MAX_ICP_ITERS * n;
+  if (c->frames) b += (size_t)c->n_frames * n * 2 + (size_t)c->n_frames * 64;
+  *bytes = b;
+  return KF_OK;
+}
+
+// ---- A1 -------------------------------------------------------------------
+
+int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) {
+  KF_REQUIRE(c && depth, KF_ERR_INVALID, "null argument");
+  kf_stage_begin(c, KF_STAGE_DEPTH);
+  const uint16_t* src = depth;
+  if (mem_kind == KF_MEM_HOST) {
+    KF_HIP_CHECK(hipMemcpyAsync(c->depth_u16, depth, (size_t)c->H * c->W * 2,
+                                hipMemcpyHostToDevice, c->stream));
+    src = c->depth_u16;
+  }
+  KF_LAUNCH(kf_launch_depth_u16(c, src));
+  kf_stage_end(c, KF_STAGE_DEPTH);
+  c->have_depth = true;
+  return KF_OK;
+}
+
+int kf_upload_depth_f32(kf_context* c, const float* depth_mm, int32_t mem_kind) {
+  KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipMemcpyAsync(c->depth, depth_mm, (size_t)c->H * c->W * 4,
+                              mem_kind == KF_MEM_HOST ? hipMemcpyHostToDevice
+                                                      : hipMemcpyDeviceToDevice,
+                              c->stream));
+  c->have_depth = true;
+  return KF_OK;
+}
+
+int kf_read_depth_file(const char* path, int32_t height, int32_t width, uint16_t* out) {
+  KF_REQUIRE(path && out && height > 0 && width > 0, KF_ERR_INVALID, "bad argument");
+  FILE* fp = fopen(path, "rb");
+  KF_REQUIRE(fp, KF_ERR_IO, std::string("cannot open ") + path);
+  const size_t n = (size_t)height * width;
+  std::vector<unsigned char> raw(n * 2);
+  const size_t got = fread(raw.data(), 1, n * 2, fp);
+  fclose(fp);
+  KF_REQUIRE(got == n * 2, KF_ERR_IO, std::string("short read from ") + path);
+  for (size_t i = 0; i < n; ++i) out[i] = (uint16_t)(raw[2 * i] | (raw[2 * i + 1] << 8));
+  return KF_OK;
+}
+
+int kf_download_depth(kf_context* c, float* depth_mm) {
+  KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipMemcpyAsync(depth_mm, c->depth, (size_t)c->H * c->W * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+// ---- A4 -------------------------------------------------------------------
+
+int kf_bilateral(kf_context* c) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  kf_stage_begin(c, KF_STAGE_BILATERAL);
+  KF_LAUNCH(kf_launch_bilateral(c));
+  kf_stage_end(c, KF_STAGE_BILATERAL);
+  return KF_OK;
+}
+
+// ---- A3 -------------------------------------------------------------------
+
+int kf_vertex_normals(kf_context* c, const float K[9]) {
+  KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  kf_stage_begin(c, KF_STAGE_VERTEX);
+  KF_LAUNCH(kf_launch_vertex(c, cam));
+  kf_stage_end(c, KF_STAGE_VERTEX);
+  return KF_OK;
+}
+
+int kf_download_vertex_maps(kf_context* c, float* vx, float* vy, float* vz, float* nx, float* ny,
+                            float* nz, uint8_t* mask) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  const size_t n = (size_t)c->H * c->W;
+  std::vector<float4> V(n), N(n);
+  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->vmap, n * 16, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->nmap, n * 16, hipMemcpyDeviceToHost, c->stream));
+  if (mask)
+    KF_HIP_CHECK(hipMemcpyAsync(mask, c->mask, n, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; ++i) {
+    if (vx) vx[i] = V[i].x;
+    if (vy) vy[i] = V[i].y;
+    if (vz) vz[i] = V[i].z;
+    if (nx) nx[i] = N[i].x;
+    if (ny) ny[i] = N[i].y;
+    if (nz) nz[i] = N[i].z;
+  }
+  return KF_OK;
+}
+
+int kf_upload_vertex_maps(kf_context* c, const float* vx, const float* vy, const float* vz,
+                          const float* nx, const float* ny, const float* nz,
+                          const uint8_t* mask) {
+  KF_REQUIRE(c && vx && vy && vz && nx && ny && nz && mask, KF_ERR_INVALID, "null argument");
+  const size_t n = (size_t)c->H * c->W;
+  std::vector<float4> V(n), N(n);
+  for (size_t i = 0; i < n; ++i) {
+    V[i] = make_float4(vx[i], vy[i], vz[i], mask[i] ? 1.0f : 0.0f);
+    N[i] = make_float4(nx[i], ny[i], nz[i], 0.0f);
+  }
+  KF_HIP_CHECK(hipMemcpyAsync(c->vmap, V.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->nmap, N.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->mask, mask, n, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->depth, vz, n * 4, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->have_depth = true;
+  return KF_OK;
+}
+
+// ---- A5-A7 ----------------------------------------------------------------
+
+int kf_estimate_pose(kf_context* c, const float K[9], float T_inout[16]) {
+  KF_REQUIRE(c && K && T_inout, KF_ERR_INVALID, "null argument");
+  int rc = kf_set_pose(c, T_inout);
+  if (rc != KF_OK) return rc;
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  kf_stage_begin(c, KF_STAGE_ICP);
+  KF_LAUNCH(kf_launch_icp(c, cam));
+  kf_stage_end(c, KF_STAGE_ICP);
+  KF_HIP_CHECK(hipMemcpyAsync(T_inout, c->state->T, 64, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+int kf_icp_history(kf_context* c, int32_t* n_iters, float* T_iters, double* sums,
+                   int32_t* status, uint8_t* masks) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  kf_dev_state* h = new kf_dev_state;
+  hipError_t e = hipMemcpyAsync(h, c->state, sizeof(kf_dev_state), hipMemcpyDeviceToHost,
+                                c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    delete h;
+    return kf_fail(KF_ERR_HIP, hipGetErrorString(e));
+  }
+  const int n = h->icp_iters_done;
+  if (n_iters) *n_iters = n;
+  if (T_iters) memcpy(T_iters, h->iter_T, sizeof(float) * 16 * n);
+  if (sums) memcpy(sums, h->iter_sums, sizeof(double) * KF_ICP_SUMS * n);
+  if (status) memcpy(status, h->iter_status, sizeof(int32_t) * n);
+  delete h;
+  if (masks) {
+    KF_REQUIRE(c->icp_masks && c->prm.record_icp, KF_ERR_STATE, "record_icp is not enabled");
+    KF_HIP_CHECK(hipMemcpy(masks, c->icp_masks, (size_t)n * c->H * c->W, hipMemcpyDeviceToHost));
+  }
+  return KF_OK;
+}
+
+// ---- A8 -------------------------------------------------------------------
+
+static int integrate_at_current_pose(kf_context* c, const kf_cam& cam, int first) {
+  kf_stage_begin(c, KF_STAGE_INTEG_PREP);
+  KF_LAUNCH(kf_launch_integ_table(c, cam));
+  kf_stage_end(c, KF_STAGE_INTEG_PREP);
+  kf_stage_begin(c, KF_STAGE_INTEGRATE);
+  KF_LAUNCH(kf_launch_integrate(c, cam, first));
+  kf_stage_end(c, KF_STAGE_INTEGRATE);
+  return KF_OK;
+}
+
+int kf_integrate(kf_context* c, const float K[9], const float T[16], int32_t first_frame) {
+  KF_REQUIRE(c && K && T, KF_ERR_INVALID, "null argument");
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  int rc = kf_set_pose(c, T);
+  if (rc != KF_OK) return rc;
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  return integrate_at_current_pose(c, cam, first_frame ? 1 : 0);
+}
+
+int kf_download_volume(kf_context* c, float* tsdf, float* weight) {
+  KF_REQUIRE(c && (tsdf || weight), KF_ERR_INVALID, "null argument");
+  const size_t nv = (size_t)c->n_voxels;
+  const size_t chunk = (size_t)1 << 24;
+  if (c->vol.precision == KF_TSDF_F32) {
+    std::vector<float2> buf(std::min(nv, chunk));
+    for (size_t o = 0; o < nv; o += chunk) {
+      const size_t m = std::min(chunk, nv - o);
+      KF_HIP_CHECK(hipMemcpy(buf.data(), (float2*)c->volume + o, m * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < m; ++i) {
+        if (tsdf) tsdf[o + i] = buf[i].x;
+        if (weight) weight[o + i] = buf[i].y;
+      }
+    }
+  } else {
+    std::vector<uint32_t> buf(std::min(nv, chunk));
+    for (size_t o = 0; o < nv; o += chunk) {
+      const size_t m = std::min(chunk, nv - o);
+      KF_HIP_CHECK(hipMemcpy(buf.data(), (uint32_t*)c->volume + o, m * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < m; ++i) {
+        const uint32_t p = buf[i];
+        if (tsdf) tsdf[o + i] = kf_f16_bits_to_f32((uint16_t)(p & 0xffffu));
+        if (weight) weight[o + i] = (p >> 16) ? (float)(p >> 16) : NAN;
+      }
+    }
+  }
+  return KF_OK;
+}
+
+int kf_upload_volume(kf_context* c, const float* tsdf, const float* weight) {
+  KF_REQUIRE(c && tsdf && weight, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  const size_t nv = (size_t)c->n_voxels;
+  const size_t chunk = (size_t)1 << 24;
+  if (c->vol.precision == KF_TSDF_F32) {
+    std::vector<float2> buf(std::min(nv, chunk));
+    for (size_t o = 0; o < nv; o += chunk) {
+      const size_t m = std::min(chunk, nv - o);
+      for (size_t i = 0; i < m; ++i) buf[i] = make_float2(tsdf[o + i], weight[o + i]);
+      KF_HIP_CHECK(hipMemcpy((float2*)c->volume + o, buf.data(), m * 8, hipMemcpyHostToDevice));
+    }
+  } else {
+    std::vector<uint32_t> buf(std::min(nv, chunk));
+    for (size_t o = 0; o < nv; o += chunk) {
+      const size_t m = std::min(chunk, nv - o);
+      for (size_t i = 0; i < m; ++i) {
+        const float w = weight[o + i];
+        uint32_t wi = 0;
+        if (w == w && w > 0) wi = w >= 65535.0f ? 65535u : (uint32_t)w;
+        buf[i] = (wi << 16) | kf_f32_to_f16_bits(tsdf[o + i]);
+      }
+      KF_HIP_CHECK(hipMemcpy((uint32_t*)c->volume + o, buf.data(), m * 4, hipMemcpyHostToDevice));
+    }
+  }
+  return KF_OK;
+}
+
+int kf_download_volume_raw(kf_context* c, void* out, size_t bytes) {
+  KF_REQUIRE(c && out && bytes == c->volume_bytes, KF_ERR_INVALID, "size mismatch");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  KF_HIP_CHECK(hipMemcpy(out, c->volume, bytes, hipMemcpyDeviceToHost));
+  return KF_OK;
+}
+
+int kf_upload_volume_raw(kf_context* c, const void* in, size_t bytes) {
+  KF_REQUIRE(c && in && bytes == c->volume_bytes, KF_ERR_INVALID, "size mismatch");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  KF_HIP_CHECK(hipMemcpy(c->volume, in, bytes, hipMemcpyHostToDevice));
+  return KF_OK;
+}
+
+int kf_volume_device_ptr(kf_context* c, void** ptr, size_t* bytes) {
+  KF_REQUIRE(c && ptr, KF_ERR_INVALID, "null argument");
+  *ptr = c->volume;
+  if (bytes) *bytes = c->volume_bytes;
+  return KF_OK;
+}
+
+// ---- A9-A10 ---------------------------------------------------------------
+
+static int raycast_at_current_pose(kf_context* c, const kf_cam& cam) {
+  kf_stage_begin(c, KF_STAGE_RAYCAST);
+  KF_LAUNCH(kf_launch_raycast(c, cam));
+  kf_stage_end(c, KF_STAGE_RAYCAST);
+  return KF_OK;
+}
+
+int kf_raycast(kf_context* c, const float K[9], const float T[16]) {
+  KF_REQUIRE(c && K && T, KF_ERR_INVALID, "null argument");
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  int rc = kf_set_pose(c, T);
+  if (rc != KF_OK) return rc;
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  return raycast_at_current_pose(c, cam);
+}
+
+int kf_download_model_maps(kf_context* c, float* x, float* y, float* z, float* nx, float* ny,
+                           float* nz, int32_t* hit_index) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  const size_t n = (size_t)c->H * c->W;
+  std::vector<float4> V(n), N(n);
+  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->model_v, n * 16, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->model_n, n * 16, hipMemcpyDeviceToHost, c->stream));
+  if (hit_index)
+    KF_HIP_CHECK(hipMemcpyAsync(hit_index, c->hit, n * 4, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; ++i) {
+    if (x) x[i] = V[i].x;
+    if (y) y[i] = V[i].y;
+    if (z) z[i] = V[i].z;
+    if (nx) nx[i] = N[i].x;
+    if (ny) ny[i] = N[i].y;
+    if (nz) nz[i] = N[i].z;
+  }
+  return KF_OK;
+}
+
+int kf_upload_model_maps(kf_context* c, const float* x, const float* y, const float* z,
+                         const float* nx, const float* ny, const float* nz) {
+  KF_REQUIRE(c && x && y && z && nx && ny && nz, KF_ERR_INVALID, "null argument");
+  const size_t n = (size_t)c->H * c->W;
+  std::vector<float4> V(n), N(n);
+  for (size_t i = 0; i < n; ++i) {
+    V[i] = make_float4(x[i], y[i], z[i], 0.0f);
+    N[i] = make_float4(nx[i], ny[i], nz[i], 0.0f);
+  }
+  KF_HIP_CHECK(hipMemcpyAsync(c->model_v, V.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->model_n, N.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+// ---- frame sequence ---------------------------------------------------------
+
+int kf_stream_upload_u16(kf_context* c, const uint16_t* frames, int32_t n_frames,
+                         int32_t mem_kind) {
+  KF_REQUIRE(c && frames && n_frames > 0, KF_ERR_INVALID, "bad argument");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (c->frames) (void)hipFree(c->frames);
+  if (c->poses) (void)hipFree(c->poses);
+  c->frames = nullptr;
+  c->poses = nullptr;
+  c->n_frames = 0;
+  const size_t bytes = (size_t)n_frames * c->H * c->W * 2;
+  if (hipMalloc(&c->frames, bytes) != hipSuccess) {
+    c->frames = nullptr;
+    return kf_fail(KF_ERR_OOM, "cannot allocate the frame stream");
+  }
+  if (hipMalloc(&c->poses, (size_t)n_frames * 64) != hipSuccess) {
+    c->poses = nullptr;
+    return kf_fail(KF_ERR_OOM, "cannot allocate the pose record");
+  }
+  KF_HIP_CHECK(hipMemcpy(c->frames, frames, bytes,
+                         mem_kind == KF_MEM_HOST ? hipMemcpyHostToDevice
+                                                 : hipMemcpyDeviceToDevice));
+  KF_HIP_CHECK(hipMemset(c->poses, 0, (size_t)n_frames * 64));
+  c->n_frames = n_frames;
+  return KF_OK;
+}
+
+static int load_stream_frame(kf_context* c, int32_t frame) {
+  KF_REQUIRE(c->frames && frame >= 0 && frame < c->n_frames, KF_ERR_INVALID,
+             "frame outside the uploaded stream");
+  kf_stage_begin(c, KF_STAGE_DEPTH);
+  KF_LAUNCH(kf_launch_depth_u16(c, c->frames + (size_t)frame * c->H * c->W));
+  kf_stage_end(c, KF_STAGE_DEPTH);
+  c->have_depth = true;
+  if (c->prm.bilateral) {
+    kf_stage_begin(c, KF_STAGE_BILATERAL);
+    KF_LAUNCH(kf_launch_bilateral(c));
+    kf_stage_end(c, KF_STAGE_BILATERAL);
+  }
+  return KF_OK;
+}
+
+static int record_pose(kf_context* c, int32_t frame) {
+  KF_HIP_CHECK(hipMemcpyAsync(c->poses + (size_t)frame * 16, c->state->T, 64,
+                              hipMemcpyDeviceToDevice, c->stream));
+  return KF_OK;
+}
+
+int kf_pipeline_initial(kf_context* c, const float K[9], int32_t frame) {
+  KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
+  int rc = load_stream_frame(c, frame);
+  if (rc != KF_OK) return rc;
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  if ((rc = integrate_at_current_pose(c, cam, 1)) != KF_OK) return rc;
+  if ((rc = raycast_at_current_pose(c, cam)) != KF_OK) return rc;
+  return record_pose(c, frame);
+}
+
+int kf_pipeline_incoming(kf_context* c, const float K[9], int32_t frame) {
+  KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
+  int rc = load_stream_frame(c, frame);
+  if (rc != KF_OK) return rc;
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  kf_stage_begin(c, KF_STAGE_VERTEX);
+  KF_LAUNCH(kf_launch_vertex(c, cam));
+  kf_stage_end(c, KF_STAGE_VERTEX);
+  kf_stage_begin(c, KF_STAGE_ICP);
+  KF_LAUNCH(kf_launch_icp(c, cam));
+  kf_stage_end(c, KF_STAGE_ICP);
+  if ((rc = integrate_at_current_pose(c, cam, 0)) != KF_OK) return rc;
+  if ((rc = raycast_at_current_pose(c, cam)) != KF_OK) return rc;
+  return record_pose(c, frame);
+}
+
+int kf_pipeline_run(kf_context* c, const float K[9], int32_t first, int32_t count) {
+  KF_REQUIRE(c && K && count > 0, KF_ERR_INVALID, "bad argument");
+  int rc = kf_pipeline_initial(c, K, first);
+  for (int32_t f = first + 1; rc == KF_OK && f < first + count; ++f)
+    rc = kf_pipeline_incoming(c, K, f);
+  return rc;
+}
+
+int kf_get_pose(kf_context* c, float T[16]) {
+  KF_REQUIRE(c && T, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipMemcpyAsync(T, c->state->T, 64, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+int kf_download_poses(kf_context* c, float* poses, int32_t n_frames) {
+  KF_REQUIRE(c && poses && c->poses && n_frames > 0 && n_frames <= c->n_frames, KF_ERR_INVALID,
+             "bad argument");
+  KF_HIP_CHECK(hipMemcpyAsync(poses, c->poses, (size_t)n_frames * 64, hipMemcpyDeviceToHost,
+                              c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+// ---- measurement ----------------------------------------------------------
+
+int kf_stage_times(kf_context* c, double* ms, int64_t* counts) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  kf_timing* t = timing_of(c);
+  for (auto& p : t->pending) {
+    float e = 0.f;
+    if (hipEventElapsedTime(&e, p.a, p.b) == hipSuccess) {
+      c->stage_ms[p.stage] += e;
+      c->stage_count[p.stage] += 1;
+    }
+    t->pool.push_back(p.a);
+    t->pool.push_back(p.b);
+  }
+  t->pending.clear();
+  for (int i = 0; i < KF_STAGE_COUNT; ++i) {
+    if (ms) ms[i] = c->stage_ms[i];
+    if (counts) counts[i] = c->stage_count[i];
+  }
+  return KF_OK;
+}
+
+int kf_reset_stage_times(kf_context* c) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  int rc = kf_stage_times(c, nullptr, nullptr);
+  if (rc != KF_OK) return rc;
+  for (int i = 0; i < KF_STAGE_COUNT; ++i) {
+    c->stage_ms[i] = 0.0;
+    c->stage_count[i] = 0;
+  }
+  return KF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+"""ctypes view of libkf_oracle.so, the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product (kinect_fusion_slam_amd) never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkf_oracle.so")
+
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_U8 = C.POINTER(C.c_uint8)
+_U16 = C.POINTER(C.c_uint16)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built (make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "oracle_range_gate": (None, [_U16, _F, C.c_int64, C.c_float, C.c_float]),
+        "oracle_spatial_kernel": (None, [C.c_int, C.c_float, _F]),
+        "oracle_bilateral": (None, [_F, _F, C.c_int, C.c_int, _F, C.c_int, C.c_float]),
+        "oracle_vertex_normals": (None, [_F, _F, C.c_int, C.c_int, _F, _F, _F, _F, _F, _U8]),
+        "oracle_integrate_f32": (None, [_F, C.c_int, C.c_int, _F, _F, _I, _I, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_int, _F, _F]),
+        "oracle_integrate_f64": (None, [_F, C.c_int, C.c_int, _F, _F, _I, _I, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_int, _D, _D]),
+        "oracle_integrate_f16": (None, [_F, C.c_int, C.c_int, _F, _F, _I, _I, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_int, _U16, _U16]),
+        "oracle_raycast_f32": (None, [_F, C.c_int, C.c_int, _F, _F, _F, _I, _I, C.c_float,
+                                      C.c_float, C.c_int, _F, _F, _F, _F, _F, _F, _I32, _I64]),
+        "oracle_raycast_f16": (None, [_F, C.c_int, C.c_int, _F, _F, _U16, _I, _I, C.c_float,
+                                      C.c_float, C.c_int, _F, _F, _F, _F, _F, _F, _I32, _I64]),
+        "oracle_raycast_f64": (None, [_F, C.c_int, C.c_int, _F, _F, _D, _I, _I, C.c_float,
+                                      C.c_float, C.c_int, _D, _D, _D, _D, _D, _D, _I32, _I64]),
+        "oracle_icp_angle_gate": (C.c_int, [C.c_float, C.c_float]),
+        "oracle_normal_flip": (C.c_int, [C.c_float]),
+        "oracle_icp": (None, [_F, _F, _F, _F, _F, _F, _U8, _F, _F, _F, _F, _F, _F, C.c_int,
+                              C.c_int, _F, _F, _F, C.c_int, C.c_float, C.c_float, C.c_int, _F,
+                              _D, _I32, _U8]),
+        "oracle_pipeline": (None, [_U16, C.c_int, C.c_int, C.c_int, _F, _I, _I, C.c_float,
+                                   C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
+                                   C.c_float, C.c_int, C.c_int, _F, _F, _F, _D]),
+        "oracle_f32_to_f16": (None, [_F, _U16, C.c_int64]),
+        "oracle_f16_to_f32": (None, [_U16, _F, C.c_int64]),
+        "oracle_max_threads": (C.c_int, []),
+        "oracle_set_threads": (None, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _f(a):
+    return a.ctypes.data_as(_F)
+
+
+def _c32(a, n=None):
+    a = np.ascontiguousarray(a, np.float32)
+    if n is not None:
+        assert a.size == n, (a.size, n)
+    return a
+
+
+def _ivec(v):
+    return (C.c_int * 3)(*[int(x) for x in v])
+
+
+def set_threads(n):
+    lib.oracle_set_threads(int(n))
+
+
+def max_threads():
+    return lib.oracle_max_threads()
+
+
+def range_gate(depth_u16, dmin, dmax):
+    d = np.ascontiguousarray(depth_u16, np.uint16)
+    out = np.zeros(d.shape, np.float32)
+    lib.oracle_range_gate(d.ctypes.data_as(_U16), _f(out), d.size, dmin, dmax)
+    return out
+
+
+def spatial_kernel(r, sigma):
+    w = np.zeros((2 * r + 1, 2 * r + 1), np.float32)
+    lib.oracle_spatial_kernel(r, sigma, _f(w))
+    return w
+
+
+def bilateral(depth, r, sigma_s, sigma_r):
+    d = _c32(depth)
+    H, W = d.shape
+    ws = spatial_kernel(r, sigma_s)
+    out = np.zeros_like(d)
+    lib.oracle_bilateral(_f(d), _f(out), H, W, _f(ws), r, sigma_r)
+    return out
+
+
+def vertex_normals(depth, Kinv):
+    d = _c32(depth)
+    H, W = d.shape
+    Ki = _c32(Kinv, 9)
+    outs = [np.zeros_like(d) for _ in range(5)]
+    m = np.zeros(d.shape, np.uint8)
+    lib.oracle_vertex_normals(_f(d), _f(Ki), H, W, *(_f(o) for o in outs), m.ctypes.data_as(_U8))
+    return dict(zip(("vx", "vy", "nx", "ny", "nz"), outs), mask=m)
+
+
+def new_volume(dims, kind="f32"):
+    n = int(np.prod(dims))
+    if kind == "f32":
+        return np.full(n, np.nan, np.float32), np.full(n, np.nan, np.float32)
+    if kind == "f64":
+        return np.full(n, np.nan, np.float64), np.full(n, np.nan, np.float64)
+    if kind == "f16":
+        return np.full(n, 0x7e00, np.uint16), np.zeros(n, np.uint16)
+    raise ValueError(kind)
+
+
+def integrate(depth, K, T, vol, prm, first, tsdf, weight):
+    """In-place update of (tsdf, weight); dtype selects the f32/f64/f16 variant."""
+    d = _c32(depth)
+    H, W = d.shape
+    args = (_f(d), H, W, _f(_c32(K, 9)), _f(_c32(T, 16)), _ivec(vol.dims), _ivec(vol.start),
+            vol.voxel_mm, prm.truncation_mm, prm.sdf_min_mm, prm.sdf_max_mm, int(first))
+    if tsdf.dtype == np.float32:
+        lib.oracle_integrate_f32(*args, _f(tsdf), _f(weight))
+    elif tsdf.dtype == np.float64:
+        lib.oracle_integrate_f64(*args, tsdf.ctypes.data_as(_D), weight.ctypes.data_as(_D))
+    elif tsdf.dtype == np.uint16:
+        lib.oracle_integrate_f16(*args, tsdf.ctypes.data_as(_U16), weight.ctypes.data_as(_U16))
+    else:
+        raise TypeError(tsdf.dtype)
+
+
+def raycast(depth, K, T, tsdf, vol, trunc, fix_bounds=0):
+    d = _c32(depth)
+    H, W = d.shape
+    stats = np.zeros(2, np.int64)
+    hit = np.zeros((H, W), np.int32)
+    dt = np.float64 if tsdf.dtype == np.float64 else np.float32
+    outs = [np.zeros((H, W), dt) for _ in range(6)]
+    args = (_f(d), H, W, _f(_c32(K, 9)), _f(_c32(T, 16)))
+    tail = (_ivec(vol.dims), _ivec(vol.start), vol.voxel_mm, trunc, int(fix_bounds))
+    if tsdf.dtype == np.float32:
+        lib.oracle_raycast_f32(*args, _f(tsdf), *tail, *(_f(o) for o in outs),
+                               hit.ctypes.data_as(_I32), stats.ctypes.data_as(_I64))
+    elif tsdf.dtype == np.uint16:
+        lib.oracle_raycast_f16(*args, tsdf.ctypes.data_as(_U16), *tail, *(_f(o) for o in outs),
+                               hit.ctypes.data_as(_I32), stats.ctypes.data_as(_I64))
+    else:
+        lib.oracle_raycast_f64(*args, tsdf.ctypes.data_as(_D), *tail,
+                               *(o.ctypes.data_as(_D) for o in outs), hit.ctypes.data_as(_I32),
+                               stats.ctypes.data_as(_I64))
+    res = dict(zip(("x", "y", "z", "nx", "ny", "nz"), outs))
+    res["hit"] = hit
+    res["steps"], res["hits"] = int(stats[0]), int(stats[1])
+    return res
+
+
+def icp(frame, model, K, T_prev, iters, dist_gate, angle_gate, reduce_mode=0, masks=True,
+        T_start=None):
+    """frame: dict vx, vy, vz(depth), nx, ny, nz, mask; model: dict x, y, z, nx, ny, nz."""
+    H, W = frame["vz"].shape
+    fr = [_c32(frame[k]) for k in ("vx", "vy", "vz", "nx", "ny", "nz")]
+    m = np.ascontiguousarray(frame["mask"], np.uint8)
+    mo = [_c32(model[k]) for k in ("x", "y", "z", "nx", "ny", "nz")]
+    T = _c32(T_prev, 16).copy()
+    Ts = np.zeros((max(iters, 1), 16), np.float32)
+    sums = np.zeros((max(iters, 1), 28), np.float64)
+    st = np.zeros(max(iters, 1), np.int32)
+    mk = np.zeros((max(iters, 1), H, W), np.uint8)
+    lib.oracle_icp(*(_f(a) for a in fr), m.ctypes.data_as(_U8), *(_f(a) for a in mo), H, W,
+                   _f(_c32(K, 9)), _f(T), _f(_c32(T_start, 16)) if T_start is not None else None,
+                   iters, dist_gate, angle_gate, reduce_mode, _f(Ts),
+                   sums.ctypes.data_as(_D), st.ctypes.data_as(_I32),
+                   mk.ctypes.data_as(_U8) if masks else None)
+    return {"T": T.reshape(4, 4), "T_iters": Ts[:iters].reshape(-1, 4, 4), "sums": sums[:iters],
+            "status": st[:iters], "masks": mk[:iters] if masks else None}
+
+
+def f32_to_f16_bits(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(x.shape, np.uint16)
+    lib.oracle_f32_to_f16(_f(x), out.ctypes.data_as(_U16), x.size)
+    return out
+
+
+def f16_bits_to_f32(h):
+    h = np.ascontiguousarray(h, np.uint16)
+    out = np.zeros(h.shape, np.float32)
+    lib.oracle_f16_to_f32(h.ctypes.data_as(_U16), _f(out), h.size)
+    return out
+
+
+def icp_angle_gate(c, gate):
+    return lib.oracle_icp_angle_gate(float(c), float(gate))
+
+
+def normal_flip(c):
+    return lib.oracle_normal_flip(float(c))
+
+
+def pipeline(frames, K, vol, prm, reduce_mode=0, stage_ms=None):
+    f = np.ascontiguousarray(frames, np.uint16)
+    n, H, W = f.shape
+    tsdf, weight = new_volume(vol.dims, "f32")
+    poses = np.zeros((n, 16), np.float32)
+    ms = np.zeros(6, np.float64)
+    r = prm.bilateral_radius if prm.bilateral else 0
+    lib.oracle_pipeline(f.ctypes.data_as(_U16), n, H, W, _f(_c32(K, 9)), _ivec(vol.dims),
+                        _ivec(vol.start), vol.voxel_mm, prm.truncation_mm, prm.sdf_min_mm,
+                        prm.sdf_max_mm, prm.depth_min_mm, prm.depth_max_mm, r, prm.sigma_s_px,
+                        prm.sigma_r_mm, prm.icp_iterations, prm.icp_dist_mm, prm.icp_angle_deg,
+                        reduce_mode, prm.raycast_fix_bounds, _f(tsdf), _f(weight), _f(poses),
+                        ms.ctypes.data_as(_D))
+    if stage_ms is not None:
+        stage_ms[:] = ms
+    return poses.reshape(n, 4, 4), tsdf, weight

@@ -1,0 +1,99 @@
+"""The C ABI library: loads on a CPU host, exports every declared entry point,
+validates arguments before touching the GPU, and its presets reproduce the
+reference's compiled-in constants."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def declared_functions():
+    hdr = open(os.path.join(ROOT, "include", "kinfu_amd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|float|const char\s*\*)\s*(kf_\w+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol(kf):
+    names = declared_functions()
+    assert len(names) >= 45
+    lib = ctypes.CDLL(kf._abi.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    unbound = sorted(set(names) - set(kf.exported_symbols()))
+    assert not unbound, f"declared but not bound in the Python view: {unbound}"
+
+
+def test_reference_preset_matches_reference_constants(kf):
+    vol, prm, K, H, W = kf.preset(kf.PRESET_REFERENCE)
+    # kinfu_pipeline.cpp:14-24
+    np.testing.assert_array_equal(K, np.array([524.639, 0, 316.625, 0, 523.503, 256.2318, 0, 0, 1],
+                                              np.float32))
+    # kinfu_pipeline.h:42-62, system_interface.cpp:6-11, sensor_pose_estimation.cu:48,69,339
+    assert (H, W) == (480, 640)
+    assert list(vol.dims) == [360, 360, 300] and list(vol.start) == [-200, -200, -120]
+    assert vol.voxel_mm == 10.0 and vol.precision == kf.KF_TSDF_F32
+    assert (prm.truncation_mm, prm.sdf_min_mm, prm.sdf_max_mm) == (300.0, -30.0, 300.0)
+    assert (prm.depth_min_mm, prm.depth_max_mm) == (300.0, 2000.0)
+    assert (prm.icp_iterations, prm.icp_dist_mm, prm.icp_angle_deg) == (10, 20.0, 30.0)
+    assert prm.bilateral == 0 and prm.bilateral_radius == 3
+    assert (prm.sigma_s_px, prm.sigma_r_mm) == (4.5, 30.0)
+
+
+def test_config_presets(kf):
+    vol, prm, K, H, W = kf.preset(1)
+    assert np.prod(list(vol.dims)) == 38_880_000 and (H, W) == (480, 640)
+    assert K[0] == K[4] == 525.0 and K[2] == 319.5 and K[5] == 239.5
+    assert prm.truncation_mm == 30.0 and prm.bilateral_radius == 2
+    assert (prm.icp_dist_mm, prm.icp_angle_deg) == (100.0, 20.0)
+    vol2, prm2, *_ = kf.preset(2)
+    assert np.prod(list(vol2.dims)) == 311_040_000 and vol2.precision == kf.KF_TSDF_F16
+    vol4, prm4, K4, H4, W4 = kf.preset(4)
+    assert np.prod(list(vol4.dims)) == 1024 * 1024 * 768 and (H4, W4) == (720, 1280)
+
+
+def test_create_rejects_bad_arguments_without_a_gpu(kf):
+    vol, prm, K, H, W = kf.preset(1)
+    h = ctypes.c_void_p()
+    vol.dims[0] = 1
+    rc = kf.lib.kf_create(ctypes.byref(vol), H, W, ctypes.byref(prm), ctypes.byref(h))
+    assert rc == -1 and b"dims" in kf.lib.kf_last_error()
+    vol.dims[:] = [2048, 2048, 1024]  # 2^32 voxels: above the int32 voxel index range
+    rc = kf.lib.kf_create(ctypes.byref(vol), H, W, ctypes.byref(prm), ctypes.byref(h))
+    assert rc == -1 and b"2^31" in kf.lib.kf_last_error()
+    vol.dims[:] = [360, 360, 300]
+    prm.bilateral_radius = 9
+    rc = kf.lib.kf_create(ctypes.byref(vol), H, W, ctypes.byref(prm), ctypes.byref(h))
+    assert rc == -1 and b"bilateral_radius" in kf.lib.kf_last_error()
+    assert not h.value
+
+
+def test_read_depth_file_roundtrip_and_missing(kf, tmp_path):
+    # the reference reads raw little-endian uint16 with no header
+    # (system_utility.cpp:58-68)
+    H, W = 6, 8
+    d = (np.arange(H * W, dtype=np.uint16) * 1237) % 65535
+    p = tmp_path / "dep0001.dat"
+    p.write_bytes(d.astype("<u2").tobytes())
+    out = np.zeros(H * W, np.uint16)
+    rc = kf.lib.kf_read_depth_file(str(p).encode(), H, W,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+    assert rc == 0
+    np.testing.assert_array_equal(out, d)
+    rc = kf.lib.kf_read_depth_file(str(tmp_path / "none.dat").encode(), H, W,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+    assert rc == -5
+    p.write_bytes(d[:10].astype("<u2").tobytes())
+    rc = kf.lib.kf_read_depth_file(str(p).encode(), H, W,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+    assert rc == -5
+
+
+def test_spatial_kernel_same_as_oracle_and_numpy(kf, oracle):
+    for r, s in ((2, 4.5), (3, 4.5), (5, 2.0)):
+        w = kf.spatial_kernel(r, s)
+        np.testing.assert_array_equal(w, oracle.spatial_kernel(r, s))
+        yy, xx = np.mgrid[-r:r + 1, -r:r + 1]
+        np.testing.assert_allclose(w, np.exp(-(xx ** 2 + yy ** 2) / (2.0 * s * s)), rtol=3e-7)
