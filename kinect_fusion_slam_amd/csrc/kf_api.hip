@@ -51,15 +51,6 @@ __global__ void k_set_pose(kf_dev_state* st, kf_pose_arg p) {
   }
 }
 
-// Magic multiplier for floor(n / D), n < 2^31: q = (umulhi(n, m) + n) >> s.
-void magic_for(uint32_t D, uint32_t* m, int32_t* s) {
-  int l = 0;
-  while ((1ull << l) < D) ++l;
-  const uint64_t mm = (((1ull << 32) * ((1ull << l) - D)) / D) + 1;
-  *m = (uint32_t)mm;
-  *s = l;
-}
-
 // Order-preserving map of floats to integers (for bisection over floats).
 int64_t f2ord(float f) {
   int32_t b;
@@ -355,7 +346,7 @@ int kf_destroy(kf_context* c) {
   if (!c) return KF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   timing_drop(c);
-  void* bufs[] = {c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
+  void* bufs[] = {c->ray_queue, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses};
@@ -396,7 +387,12 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   for (int i = 0; i < 3; ++i)
     KF_REQUIRE(vol->dims[i] >= 3 && vol->dims[i] <= 65536, KF_ERR_INVALID, "bad volume dims");
   const int64_t nv = (int64_t)vol->dims[0] * vol->dims[1] * vol->dims[2];
-  KF_REQUIRE(nv < (1ll << 31) - 1024, KF_ERR_INVALID, "volume must hold fewer than 2^31 voxels");
+  {
+    const int64_t bricks = (int64_t)((vol->dims[0] + 7) / 8) * ((vol->dims[1] + 7) / 8) *
+                           ((vol->dims[2] + 7) / 8);
+    KF_REQUIRE(bricks * 512 < (1ll << 31), KF_ERR_INVALID,
+               "volume must hold fewer than 2^31 voxels (in 8^3 bricks)");
+  }
   KF_REQUIRE(vol->voxel_mm > 0, KF_ERR_INVALID, "voxel size must be positive");
   KF_REQUIRE(vol->precision == KF_TSDF_F32 || vol->precision == KF_TSDF_F16, KF_ERR_INVALID,
              "unknown precision");
@@ -415,12 +411,23 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   c->vg.sy = vol->start[1];
   c->vg.sz = vol->start[2];
   c->vg.d = vol->voxel_mm;
-  magic_for((uint32_t)vol->dims[2], &c->vg.mz, &c->vg.shz);
-  magic_for((uint32_t)vol->dims[1], &c->vg.my, &c->vg.shy);
+  c->vg.bx = (vol->dims[0] + KF_BRICK - 1) / KF_BRICK;
+  c->vg.by = (vol->dims[1] + KF_BRICK - 1) / KF_BRICK;
+  c->vg.bz = (vol->dims[2] + KF_BRICK - 1) / KF_BRICK;
+  c->n_bricks = c->vg.bx * c->vg.by * c->vg.bz;
+  c->n_store = (int64_t)c->n_bricks * 512;
+
   const size_t n = (size_t)height * width;
-  c->icp_blocks = (int)((n + KF_ICP_BLOCK * KF_ICP_PIX_PER_THREAD - 1) /
-                        (KF_ICP_BLOCK * KF_ICP_PIX_PER_THREAD));
-  c->volume_bytes = (size_t)nv * (vol->precision == KF_TSDF_F32 ? 8 : 4);
+  for (c->icp_px = 1; c->icp_px < 4; c->icp_px *= 2)
+    if ((n + (size_t)KF_ICP_BLOCK * c->icp_px - 1) / ((size_t)KF_ICP_BLOCK * c->icp_px) <=
+        KF_ICP_MAX_BLOCKS)
+      break;
+  c->icp_blocks = (int)((n + (size_t)KF_ICP_BLOCK * c->icp_px - 1) / ((size_t)KF_ICP_BLOCK * c->icp_px));
+  c->volume_bytes = (size_t)c->n_store * (vol->precision == KF_TSDF_F32 ? 8 : 4);
+  if (c->icp_blocks > KF_ICP_MAX_BLOCKS) {
+    delete c;
+    return kf_fail(KF_ERR_INVALID, "image too large for the ICP reduction (max 2^21 pixels)");
+  }
 
   auto fail = [&](int code, const std::string& m) {
     kf_destroy(c);
@@ -438,9 +445,10 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->nmap, n * 16},       {(void**)&c->mask, n},
       {(void**)&c->model_v, n * 16},    {(void**)&c->model_n, n * 16},
       {(void**)&c->hit, n * 4},         {(void**)&c->integ_tab, n * 8},
+      {&c->ray_queue, n * kf_ray_state_bytes()},
       {(void**)&c->spatial, 4 * (2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)},
       {(void**)&c->state, sizeof(kf_dev_state)},
-      {(void**)&c->icp_partials, (size_t)c->icp_blocks * KF_ICP_SUMS * 2 * 4},
+      {(void**)&c->icp_partials, (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 2 * 4},
       {&c->volume, c->volume_bytes},
   };
   for (auto& a : allocs) {
@@ -449,6 +457,8 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       return fail(KF_ERR_OOM, "device allocation failed");
     }
   }
+  c->tsdf = c->volume;
+  c->weight = (char*)c->volume + c->volume_bytes / 2;
   int rc = apply_params(c, prm);
   if (rc != KF_OK) {
     const std::string m = g_last_error;
@@ -511,7 +521,7 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
   KF_REQUIRE(c && bytes, KF_ERR_INVALID, "null argument");
   const size_t n = (size_t)c->H * c->W;
   size_t b = c->volume_bytes + n * (4 + 4 + 2 + 16 + 16 + 1 + 16 + 16 + 4 + 8) + sizeof(kf_dev_state) +
-             (size_t)c->icp_blocks * KF_ICP_SUMS * 8;
+             (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 8;
   if (c->icp_masks) b += (size_t)KF_MAX_ICP_ITERS * n;
   if (c->frames) b += (size_t)c->n_frames * n * 2 + (size_t)c->n_frames * 64;
   *bytes = b;
@@ -692,31 +702,41 @@ int kf_integrate(kf_context* c, const float K[9], const float T[16], int32_t fir
   return integrate_at_current_pose(c, cam, first_frame ? 1 : 0);
 }
 
+}  // extern "C"
+
+// Conversions between the brick-tiled device arrays and the reference's
+// linear layout (index ix*Ny*Nz + iy*Nz + iz, z fastest).
+template <class Fn>
+static void for_each_voxel(const kf_context* c, Fn fn) {
+  const int Nx = c->vg.nx, Ny = c->vg.ny, Nz = c->vg.nz;
+  int64_t lin = 0;
+  for (int x = 0; x < Nx; ++x)
+    for (int y = 0; y < Ny; ++y)
+      for (int z = 0; z < Nz; ++z, ++lin) fn(lin, (int64_t)kf_store_index(c->vg, x, y, z));
+}
+
+extern "C" {
+
 int kf_download_volume(kf_context* c, float* tsdf, float* weight) {
   KF_REQUIRE(c && (tsdf || weight), KF_ERR_INVALID, "null argument");
-  const size_t nv = (size_t)c->n_voxels;
-  const size_t chunk = (size_t)1 << 24;
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  const size_t ns = (size_t)c->n_store;
   if (c->vol.precision == KF_TSDF_F32) {
-    std::vector<float2> buf(std::min(nv, chunk));
-    for (size_t o = 0; o < nv; o += chunk) {
-      const size_t m = std::min(chunk, nv - o);
-      KF_HIP_CHECK(hipMemcpy(buf.data(), (float2*)c->volume + o, m * 8, hipMemcpyDeviceToHost));
-      for (size_t i = 0; i < m; ++i) {
-        if (tsdf) tsdf[o + i] = buf[i].x;
-        if (weight) weight[o + i] = buf[i].y;
-      }
-    }
+    std::vector<float> T(tsdf ? ns : 0), Wt(weight ? ns : 0);
+    if (tsdf) KF_HIP_CHECK(hipMemcpy(T.data(), c->tsdf, ns * 4, hipMemcpyDeviceToHost));
+    if (weight) KF_HIP_CHECK(hipMemcpy(Wt.data(), c->weight, ns * 4, hipMemcpyDeviceToHost));
+    for_each_voxel(c, [&](int64_t lin, int64_t s) {
+      if (tsdf) tsdf[lin] = T[s];
+      if (weight) weight[lin] = Wt[s];
+    });
   } else {
-    std::vector<uint32_t> buf(std::min(nv, chunk));
-    for (size_t o = 0; o < nv; o += chunk) {
-      const size_t m = std::min(chunk, nv - o);
-      KF_HIP_CHECK(hipMemcpy(buf.data(), (uint32_t*)c->volume + o, m * 4, hipMemcpyDeviceToHost));
-      for (size_t i = 0; i < m; ++i) {
-        const uint32_t p = buf[i];
-        if (tsdf) tsdf[o + i] = kf_f16_bits_to_f32((uint16_t)(p & 0xffffu));
-        if (weight) weight[o + i] = (p >> 16) ? (float)(p >> 16) : NAN;
-      }
-    }
+    std::vector<uint16_t> T(tsdf ? ns : 0), Wt(weight ? ns : 0);
+    if (tsdf) KF_HIP_CHECK(hipMemcpy(T.data(), c->tsdf, ns * 2, hipMemcpyDeviceToHost));
+    if (weight) KF_HIP_CHECK(hipMemcpy(Wt.data(), c->weight, ns * 2, hipMemcpyDeviceToHost));
+    for_each_voxel(c, [&](int64_t lin, int64_t s) {
+      if (tsdf) tsdf[lin] = kf_f16_bits_to_f32(T[s]);
+      if (weight) weight[lin] = Wt[s] ? (float)Wt[s] : NAN;
+    });
   }
   return KF_OK;
 }
@@ -724,27 +744,24 @@ int kf_download_volume(kf_context* c, float* tsdf, float* weight) {
 int kf_upload_volume(kf_context* c, const float* tsdf, const float* weight) {
   KF_REQUIRE(c && tsdf && weight, KF_ERR_INVALID, "null argument");
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
-  const size_t nv = (size_t)c->n_voxels;
-  const size_t chunk = (size_t)1 << 24;
+  const size_t ns = (size_t)c->n_store;
   if (c->vol.precision == KF_TSDF_F32) {
-    std::vector<float2> buf(std::min(nv, chunk));
-    for (size_t o = 0; o < nv; o += chunk) {
-      const size_t m = std::min(chunk, nv - o);
-      for (size_t i = 0; i < m; ++i) buf[i] = make_float2(tsdf[o + i], weight[o + i]);
-      KF_HIP_CHECK(hipMemcpy((float2*)c->volume + o, buf.data(), m * 8, hipMemcpyHostToDevice));
-    }
+    std::vector<float> T(ns, NAN), Wt(ns, NAN);
+    for_each_voxel(c, [&](int64_t lin, int64_t s) {
+      T[s] = tsdf[lin];
+      Wt[s] = weight[lin];
+    });
+    KF_HIP_CHECK(hipMemcpy(c->tsdf, T.data(), ns * 4, hipMemcpyHostToDevice));
+    KF_HIP_CHECK(hipMemcpy(c->weight, Wt.data(), ns * 4, hipMemcpyHostToDevice));
   } else {
-    std::vector<uint32_t> buf(std::min(nv, chunk));
-    for (size_t o = 0; o < nv; o += chunk) {
-      const size_t m = std::min(chunk, nv - o);
-      for (size_t i = 0; i < m; ++i) {
-        const float w = weight[o + i];
-        uint32_t wi = 0;
-        if (w == w && w > 0) wi = w >= 65535.0f ? 65535u : (uint32_t)w;
-        buf[i] = (wi << 16) | kf_f32_to_f16_bits(tsdf[o + i]);
-      }
-      KF_HIP_CHECK(hipMemcpy((uint32_t*)c->volume + o, buf.data(), m * 4, hipMemcpyHostToDevice));
-    }
+    std::vector<uint16_t> T(ns, 0x7e00u), Wt(ns, 0);
+    for_each_voxel(c, [&](int64_t lin, int64_t s) {
+      T[s] = kf_f32_to_f16_bits(tsdf[lin]);
+      const float w = weight[lin];
+      Wt[s] = (w == w && w > 0) ? (w >= 65535.0f ? 65535u : (uint16_t)w) : 0;
+    });
+    KF_HIP_CHECK(hipMemcpy(c->tsdf, T.data(), ns * 2, hipMemcpyHostToDevice));
+    KF_HIP_CHECK(hipMemcpy(c->weight, Wt.data(), ns * 2, hipMemcpyHostToDevice));
   }
   return KF_OK;
 }

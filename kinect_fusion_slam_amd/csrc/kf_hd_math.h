@@ -85,17 +85,21 @@ KF_HD float kf_expf(float x) {
 // whose last-bit behaviour is not pinned by anything in the reference; these
 // FP64 versions are the convention on both sides of the comparison.
 
+// Element (i, j) of A * B for 4x4 row-major float matrices, FP64 products
+// summed left to right and rounded once.
+KF_HD float kf_mat4_mul_elem(const float* A, const float* B, int i, int j) {
+  double s = (double)A[i * 4 + 0] * (double)B[0 * 4 + j];
+  s = s + (double)A[i * 4 + 1] * (double)B[1 * 4 + j];
+  s = s + (double)A[i * 4 + 2] * (double)B[2 * 4 + j];
+  s = s + (double)A[i * 4 + 3] * (double)B[3 * 4 + j];
+  return (float)s;
+}
+
 // C = A * B, 4x4 row-major; C may alias A or B.
 KF_HD void kf_mat4_mul(const float* A, const float* B, float* C) {
   float out[16];
   for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      double s = (double)A[i * 4 + 0] * (double)B[0 * 4 + j];
-      s = s + (double)A[i * 4 + 1] * (double)B[1 * 4 + j];
-      s = s + (double)A[i * 4 + 2] * (double)B[2 * 4 + j];
-      s = s + (double)A[i * 4 + 3] * (double)B[3 * 4 + j];
-      out[i * 4 + j] = (float)s;
-    }
+    for (int j = 0; j < 4; ++j) out[i * 4 + j] = kf_mat4_mul_elem(A, B, i, j);
   for (int i = 0; i < 16; ++i) C[i] = out[i];
 }
 
@@ -179,17 +183,18 @@ KF_HD int kf_m3_inverse(const float* in, float* out) {
 
 KF_HD int kf_ldlt6_solve(const double* A, const double* b, double* x) {
   double L[6][6];
-  double d[6];
+  double d[6], rd[6];  // pivots and their reciprocals (6 divisions in total)
   for (int i = 0; i < 6; ++i) {
     for (int j = 0; j < i; ++j) {
       double s = A[i * 6 + j];
       for (int k = 0; k < j; ++k) s = s - L[i][k] * L[j][k] * d[k];
-      L[i][j] = s / d[j];
+      L[i][j] = s * rd[j];
     }
     double s = A[i * 6 + i];
     for (int k = 0; k < i; ++k) s = s - L[i][k] * L[i][k] * d[k];
     if (!(s >= KF_LDLT_EPS)) return 1;
     d[i] = s;
+    rd[i] = 1.0 / s;
   }
   double y[6];
   for (int i = 0; i < 6; ++i) {
@@ -197,7 +202,7 @@ KF_HD int kf_ldlt6_solve(const double* A, const double* b, double* x) {
     for (int k = 0; k < i; ++k) s = s - L[i][k] * y[k];
     y[i] = s;
   }
-  for (int i = 0; i < 6; ++i) y[i] = y[i] / d[i];
+  for (int i = 0; i < 6; ++i) y[i] = y[i] * rd[i];
   for (int i = 5; i >= 0; --i) {
     double s = y[i];
     for (int k = i + 1; k < 6; ++k) s = s - L[k][i] * x[k];
@@ -212,13 +217,12 @@ KF_HD int kf_sym_index(int i, int j) {  // i <= j
   return i * 6 - (i * (i - 1)) / 2 + (j - i);
 }
 
-// One ICP update from FP64 sums: builds the symmetric system, solves it, and
-// composes the small-angle increment into T (sensor_pose_estimation.cu:399-
-// 429): x = (beta, gamma, alpha, tx, ty, tz) rounded to float,
-//   inc = [[1, x2, -x1, x3], [-x2, 1, x0, x4], [x1, -x0, 1, x5], [0, 0, 0, 1]]
-//   T <- inc * T.
+// The ICP increment from FP64 sums (sensor_pose_estimation.cu:399-425):
+// solves the symmetric system and returns
+// x = (beta, gamma, alpha, tx, ty, tz) rounded to float as the small-angle
+//   inc = [[1, x2, -x1, x3], [-x2, 1, x0, x4], [x1, -x0, 1, x5], [0, 0, 0, 1]].
 // Returns the solver status (0 solved, 1 fallback to the right-hand side).
-KF_HD int kf_icp_update(const double* sums, float* T, float* x_out) {
+KF_HD int kf_icp_increment(const double* sums, float* inc) {
   double A[36], b[6], xd[6];
   for (int i = 0; i < 6; ++i)
     for (int j = i; j < 6; ++j) {
@@ -230,13 +234,20 @@ KF_HD int kf_icp_update(const double* sums, float* T, float* x_out) {
   const int status = kf_ldlt6_solve(A, b, xd);
   float x[6];
   for (int i = 0; i < 6; ++i) x[i] = (float)(status ? b[i] : xd[i]);
-  float inc[16] = {1.0f, x[2], -x[1], x[3],  //
-                   -x[2], 1.0f, x[0], x[4],  //
-                   x[1], -x[0], 1.0f, x[5],  //
-                   0.0f, 0.0f, 0.0f, 1.0f};
+  const float m[16] = {1.0f, x[2], -x[1], x[3],  //
+                       -x[2], 1.0f, x[0], x[4],  //
+                       x[1], -x[0], 1.0f, x[5],  //
+                       0.0f, 0.0f, 0.0f, 1.0f};
+  for (int i = 0; i < 16; ++i) inc[i] = m[i];
+  return status;
+}
+
+// One ICP update: T <- inc * T (sensor_pose_estimation.cu:428).  Returns the
+// solver status.
+KF_HD int kf_icp_update(const double* sums, float* T) {
+  float inc[16];
+  const int status = kf_icp_increment(sums, inc);
   kf_mat4_mul(inc, T, T);
-  if (x_out)
-    for (int i = 0; i < 6; ++i) x_out[i] = x[i];
   return status;
 }
 

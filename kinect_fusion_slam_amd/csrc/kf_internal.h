@@ -20,7 +20,7 @@ struct kf_dev_state {
   float F[16];          // Tprev_inv * T (frame_to_frame, sensor_pose_estimation.cu:343)
   float center_dir[4];  // integration front-test direction (projective_tsdf.cu:57-71)
   int32_t icp_iters_done;
-  uint32_t ticket;      // last-block-done counter of the ICP reduction
+  uint32_t ray_queue;   // rays queued by the first raycast pass
   int32_t pad[2];
   float iter_T[KF_MAX_ICP_ITERS][16];
   double iter_sums[KF_MAX_ICP_ITERS][KF_ICP_SUMS];
@@ -34,19 +34,36 @@ struct kf_cam {
 };
 
 // Volume geometry passed by value.
+//
+// Storage is brick-tiled: the volume is cut into 8x8x8 bricks, bricks are
+// laid out x-major (z fastest), and inside a brick the voxel (lx, ly, lz)
+// sits at lx*64 + ly*8 + lz.  TSDF and weight are separate arrays of
+// bx*by*bz*512 elements (float, or half/uint16 for the FP16 volume); voxels
+// of partial bricks beyond the volume are padding that nothing reads.  A
+// wave's rays then touch a few cache lines per step instead of one line per
+// ray (x neighbours of the reference's linear layout are Ny*Nz elements
+// apart), and the raycast reads a dense TSDF-only array.
 struct kf_vol {
   int32_t nx, ny, nz;
   int32_t sx, sy, sz;  // start indices
   float d;             // voxel edge (mm)
-  // floor division by nz and ny for linear voxel ids < 2^31:
-  // q = (umulhi(n, m) + n) >> s
-  uint32_t mz, my;
-  int32_t shz, shy;
+  int32_t bx, by, bz;  // bricks per axis
 };
 
+#define KF_BRICK 8
+
+__host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int z) {
+  return ((((x >> 3) * v.by + (y >> 3)) * v.bz + (z >> 3)) << 9) | ((x & 7) << 6) |
+         ((y & 7) << 3) | (z & 7);
+}
+
 // ICP reduction geometry.
-#define KF_ICP_BLOCK 256
-#define KF_ICP_PIX_PER_THREAD 4
+#define KF_ICP_BLOCK 1024
+// Block partials one lane reads per row when an iteration is finished: the
+// pixels per thread (1, 2 or 4) are chosen so that at most 64 * 8 blocks
+// cover the image (up to 2^21 pixels).
+#define KF_ICP_MAX_PART_PER_LANE 8
+#define KF_ICP_MAX_BLOCKS (64 * KF_ICP_MAX_PART_PER_LANE)
 
 struct kf_context {
   int device = 0;
@@ -66,18 +83,24 @@ struct kf_context {
   float4* model_v = nullptr;    // raycast vertices {x, y, z, 0}
   float4* model_n = nullptr;    // raycast normals
   int32_t* hit = nullptr;       // raycast hit voxel index
+  void* ray_queue = nullptr;    // H*W ray states between the two raycast passes
   float2* integ_tab = nullptr;  // per-pixel {predicted depth, 1/|R lambda|}
   float* spatial = nullptr;     // bilateral spatial weights
 
-  // volume
+  // volume (one allocation: TSDF array, then weight array, brick-tiled)
   void* volume = nullptr;
+  void* tsdf = nullptr;
+  void* weight = nullptr;
   size_t volume_bytes = 0;
-  int64_t n_voxels = 0;
+  int64_t n_voxels = 0;   // Nx*Ny*Nz
+  int64_t n_store = 0;    // bricks * 512
+  int n_bricks = 0;
 
   // ICP
   kf_dev_state* state = nullptr;
-  float* icp_partials = nullptr;   // [blocks][KF_ICP_SUMS][2]
+  float* icp_partials = nullptr;   // 2 x [2 * KF_ICP_SUMS][blocks], by iteration parity
   int icp_blocks = 0;
+  int icp_px = 4;                  // pixels per thread
   uint8_t* icp_masks = nullptr;    // [KF_MAX_ICP_ITERS][H*W] when record_icp
   float icp_cos_lo = 0.f;          // ICP normal gate as a cosine threshold
   float flip_cos_hi = 0.f;         // raycast normal flip threshold
@@ -115,6 +138,8 @@ hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_volume_fill(kf_context* c);
 hipError_t kf_launch_set_pose(kf_context* c, const float* T_host);
 hipError_t kf_launch_record_pose(kf_context* c, int frame);
+
+size_t kf_ray_state_bytes();
 
 // shared host helpers
 void kf_make_cam(const float* K, kf_cam* cam);

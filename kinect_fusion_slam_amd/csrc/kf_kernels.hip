@@ -31,12 +31,6 @@ __device__ __forceinline__ void load_uniform(const float* __restrict__ src, floa
   for (int i = 0; i < N; ++i) dst[i] = src[i];
 }
 
-__device__ __forceinline__ uint32_t umulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
-
-// floor(n / D) for n < 2^31 with the precomputed magic (m, s).
-__device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t m, int s) {
-  return (umulhi32(n, m) + n) >> s;
-}
 
 // cuda_functions.cuh:2-10
 __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
@@ -93,23 +87,21 @@ __device__ __forceinline__ void wave_sum_comp(float& s, float& c) {
 }
 
 // ---------------------------------------------------------------------------
-// volume accessors: F32 = float2 {tsdf, weight}; F16 = uint32 {half, u16}
-
-__device__ __forceinline__ float f16_lo_to_f32(uint32_t p) {
-  return __half2float(__ushort_as_half((unsigned short)(p & 0xffffu)));
-}
+// TSDF readers over the brick-tiled arrays (index = kf_store_index)
 
 template <int PREC>
 struct VolRead;
 template <>
 struct VolRead<KF_TSDF_F32> {
-  const float2* __restrict__ v;
-  __device__ __forceinline__ float operator()(int i) const { return v[i].x; }
+  const float* __restrict__ v;
+  __device__ __forceinline__ float operator()(int i) const { return v[i]; }
 };
 template <>
 struct VolRead<KF_TSDF_F16> {
-  const uint32_t* __restrict__ v;
-  __device__ __forceinline__ float operator()(int i) const { return f16_lo_to_f32(v[i]); }
+  const uint16_t* __restrict__ v;
+  __device__ __forceinline__ float operator()(int i) const {
+    return __half2float(__ushort_as_half(v[i]));
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -242,45 +234,148 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_
 // ---------------------------------------------------------------------------
 // A5-A7  ICP.
 
-// Frame start: T_prev := T, T_prev^-1, F = T_prev^-1 T (sensor_pose_estimation.cu:
-// 255-259, 343 at i = 0).
-__global__ void k_icp_begin(kf_dev_state* st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  float T[16], Ti[16], F[16];
-  for (int i = 0; i < 16; ++i) T[i] = st->T[i];
-  if (kf_m4_inverse(T, Ti) != 0)
-    for (int i = 0; i < 16; ++i) Ti[i] = __builtin_nanf("");
-  kf_mat4_mul(Ti, T, F);
-  for (int i = 0; i < 16; ++i) {
-    st->Tprev[i] = T[i];
-    st->Tprev_inv[i] = Ti[i];
-    st->Tinv[i] = Ti[i];
-    st->F[i] = F[i];
+// One kernel per iteration, and no cross-block synchronisation inside a
+// kernel.  The blocks of iteration i first finish iteration i-1: each block
+// reads the block partials iteration i-1 wrote (ROWS rows of nb floats, rows
+// 2k and 2k+1 = sum and compensation of quantity k in compensated mode, row k
+// in plain mode), adds them in FP64 in a fixed order, solves the 6x6 system
+// and composes the pose.  Every block computes bit-identical values; block 0
+// records them.  The blocks of iteration 0 start the frame instead:
+// T_prev := T, T_prev^-1 and F = T_prev^-1 T (sensor_pose_estimation.cu:
+// 255-259, 343).  A one-block kernel finishes the last iteration.  Partials
+// are double-buffered by iteration parity, and the pose an iteration starts
+// from is read from the per-iteration record, so nothing a kernel reads is
+// written by that kernel.
+
+template <int ROWS>
+struct IcpShared {
+  double lane_part[ROWS][64];
+  double quarter[ROWS][4];
+  double sums[KF_ICP_SUMS];
+  float inc[16], T0[16], Tp[16], Tn[16], F[16];
+  int status;
+};
+
+// Sums the ROWS x nb partials into sh.sums (all threads take part).  Each
+// wave takes rows w, w + NW, ...; every lane issues all of its loads for
+// those rows at once (one memory round trip), adds its up to
+// KF_ICP_MAX_PART_PER_LANE entries of each row in FP64, and the 64 lane sums
+// of a row are then added in lane order by 4 threads of 16 lanes each.
+template <int ROWS>
+__device__ __forceinline__ void icp_reduce_partials(const float* __restrict__ P, int nb,
+                                                    IcpShared<ROWS>& sh) {
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  constexpr int NW = KF_ICP_BLOCK / 64;
+  constexpr int RPW = (ROWS + NW - 1) / NW;  // rows per wave
+  float v[RPW][KF_ICP_MAX_PART_PER_LANE];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int row = wave + r * NW;
+#pragma unroll
+    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j) {
+      const int idx = ln + 64 * j;
+      v[r][j] = (row < ROWS && idx < nb) ? P[(size_t)row * nb + idx] : 0.0f;
+    }
   }
-  st->ticket = 0u;
-  st->icp_iters_done = 0;
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int row = wave + r * NW;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j) s += (double)v[r][j];
+    if (row < ROWS) sh.lane_part[row][ln] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 * ROWS) {
+    const int row = threadIdx.x >> 2, q = threadIdx.x & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) s += sh.lane_part[row][q * 16 + l];
+    sh.quarter[row][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < KF_ICP_SUMS) {
+    const int k = threadIdx.x;
+    auto row_total = [&](int row) {
+      return ((sh.quarter[row][0] + sh.quarter[row][1]) + sh.quarter[row][2]) +
+             sh.quarter[row][3];
+    };
+    sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? row_total(2 * k) + row_total(2 * k + 1) : row_total(k);
+  }
+  __syncthreads();
 }
 
-// One ICP iteration: association and gates of every pixel
-// (sensor_pose_estimation.cu:16-136), reduction of the 27 unique products
-// plus the inlier count, and - in the last block to finish - the FP64 solve
-// and pose update (:389-429).
-template <bool KAHAN>
+// Finishes an iteration: T_new = inc(sums) * T0 and F = Tp * T_new, into sh.
+template <int ROWS>
+__device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh) {
+  if (threadIdx.x == 0) sh.status = kf_icp_increment(sh.sums, sh.inc);
+  __syncthreads();
+  if (threadIdx.x < 16)
+    sh.Tn[threadIdx.x] = kf_mat4_mul_elem(sh.inc, sh.T0, threadIdx.x >> 2, threadIdx.x & 3);
+  __syncthreads();
+  if (threadIdx.x < 16)
+    sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
+  __syncthreads();
+}
+
+// Records iteration `it`'s result (block 0 / the finishing kernel only).
+template <int ROWS>
+__device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpShared<ROWS>& sh) {
+  if (threadIdx.x < 16) st->iter_T[it][threadIdx.x] = sh.Tn[threadIdx.x];
+  if (threadIdx.x < KF_ICP_SUMS) st->iter_sums[it][threadIdx.x] = sh.sums[threadIdx.x];
+  if (threadIdx.x == 0) {
+    st->iter_status[it] = sh.status;
+    st->icp_iters_done = it + 1;
+  }
+}
+
+// Per-pixel association and gates (sensor_pose_estimation.cu:16-136) plus the
+// block reduction, for iteration `iter`.
+template <bool KAHAN, int PX>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const float4* __restrict__ vmap, const float4* __restrict__ nmap,
     const float4* __restrict__ mv, const float4* __restrict__ mn, kf_cam cam, int H, int W,
     float dist_gate, float cos_lo, kf_dev_state* __restrict__ st,
-    float* __restrict__ partials, uint8_t* __restrict__ mask_out, int iter, int last) {
+    const float* __restrict__ part_in, float* __restrict__ part_out,
+    uint8_t* __restrict__ mask_out, int iter) {
   constexpr int NS = KF_ICP_SUMS;
+  constexpr int ROWS = KAHAN ? 2 * NS : NS;
+  __shared__ IcpShared<ROWS> sh;
   __shared__ float red_s[KF_ICP_BLOCK / 64][NS];
   __shared__ float red_c[KF_ICP_BLOCK / 64][NS];
-  __shared__ double fin[NS][8];
-  __shared__ double sums_d[NS];
-  __shared__ int is_last;
+  const int nb = gridDim.x;
 
+  if (iter == 0) {
+    // frame start
+    if (threadIdx.x == 0) {
+      float T[16], Ti[16];
+      for (int i = 0; i < 16; ++i) T[i] = st->T[i];
+      if (kf_m4_inverse(T, Ti) != 0)
+        for (int i = 0; i < 16; ++i) Ti[i] = __builtin_nanf("");
+      for (int i = 0; i < 16; ++i) sh.Tp[i] = Ti[i], sh.Tn[i] = T[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16)
+      sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < 16) {
+      st->Tprev[threadIdx.x] = sh.Tn[threadIdx.x];
+      st->Tprev_inv[threadIdx.x] = sh.Tp[threadIdx.x];
+    }
+  } else {
+    // finish iteration iter-1, which started from the pose recorded for
+    // iter-2 (or from T_prev for iter-1 == 0)
+    if (threadIdx.x < 16) {
+      sh.T0[threadIdx.x] = iter == 1 ? st->Tprev[threadIdx.x] : st->iter_T[iter - 2][threadIdx.x];
+      sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
+    }
+    icp_reduce_partials<ROWS>(part_in, nb, sh);
+    icp_update_pose<ROWS>(sh);
+    if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
+  }
   float F[12], Te[12];
-  load_uniform<12>(st->F, F);
-  load_uniform<12>(st->T, Te);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) F[i] = sh.F[i], Te[i] = sh.Tn[i];
   const float* K = cam.K;
 
   float acc[NS], comp[NS];
@@ -288,55 +383,75 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   for (int k = 0; k < NS; ++k) acc[k] = 0.f, comp[k] = 0.f;
 
   const int n = H * W;
-  const int base = blockIdx.x * (KF_ICP_BLOCK * KF_ICP_PIX_PER_THREAD) + threadIdx.x;
-#pragma unroll 1
-  for (int k = 0; k < KF_ICP_PIX_PER_THREAD; ++k) {
+  const int base = blockIdx.x * (KF_ICP_BLOCK * PX) + threadIdx.x;
+  // phase 1: this thread's vertex and normal loads, issued together
+  float4 V[PX], Nc[PX];
+#pragma unroll
+  for (int k = 0; k < PX; ++k) {
     const int x = base + k * KF_ICP_BLOCK;
-    if (x >= n) break;
-    const float4 V = vmap[x];
-    bool ok = false;
-    float a[6], b = 0.f;
-    if (V.z != 0 && V.w != 0) {
-      const float cx = F[0] * V.x + F[1] * V.y + F[2] * V.z + F[3];
-      const float cy = F[4] * V.x + F[5] * V.y + F[6] * V.z + F[7];
-      const float cz = F[8] * V.x + F[9] * V.y + F[10] * V.z + F[11];
+    V[k] = x < n ? vmap[x] : make_float4(0.f, 0.f, 0.f, 0.f);
+    Nc[k] = x < n ? nmap[x] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // phase 2: projective association, then the model gathers, issued together
+  int jx[PX];
+#pragma unroll
+  for (int k = 0; k < PX; ++k) {
+    jx[k] = -1;
+    if (V[k].z != 0 && V[k].w != 0) {
+      const float cx = F[0] * V[k].x + F[1] * V[k].y + F[2] * V[k].z + F[3];
+      const float cy = F[4] * V[k].x + F[5] * V[k].y + F[6] * V[k].z + F[7];
+      const float cz = F[8] * V[k].x + F[9] * V[k].y + F[10] * V[k].z + F[11];
       const float px = K[0] * cx + K[1] * cy + K[2] * cz;
       const float py = K[3] * cx + K[4] * cy + K[5] * cz;
       const float pz = K[6] * cx + K[7] * cy + K[8] * cz;
       const int u = kf_round_to_int(px / pz);
       const int v = kf_round_to_int(py / pz);
-      if (u > 0 && u <= W && v > 0 && v <= H) {
-        const float ux = Te[0] * V.x + Te[1] * V.y + Te[2] * V.z + Te[3];
-        const float uy = Te[4] * V.x + Te[5] * V.y + Te[6] * V.z + Te[7];
-        const float uz = Te[8] * V.x + Te[9] * V.y + Te[10] * V.z + Te[11];
-        const int j = (v - 1) * W + (u - 1);
-        const float4 S = mv[j];
-        const float ddx = ux - S.x, ddy = uy - S.y, ddz = uz - S.z;
-        const float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-        if (dist <= dist_gate && dist >= 0.0f) {
-          const float4 Nc = nmap[x];
-          float nnx = Te[0] * Nc.x + Te[1] * Nc.y + Te[2] * Nc.z;
-          float nny = Te[4] * Nc.x + Te[5] * Nc.y + Te[6] * Nc.z;
-          float nnz = Te[8] * Nc.x + Te[9] * Nc.y + Te[10] * Nc.z;
-          normalize3(nnx, nny, nnz);
-          const float4 Sn = mn[j];
-          float ax = Sn.x, ay = Sn.y, az = Sn.z;
-          normalize3(ax, ay, az);
-          const float c = (nnx * ax + nny * ay + nnz * az);
-          if (c >= cos_lo && c <= 1.0f) {
-            ok = true;
-            a[0] = uz * Sn.y + (-uy * Sn.z);
-            a[1] = -uz * Sn.x + (ux * Sn.z);
-            a[2] = uy * Sn.x + (-ux * Sn.y);
-            a[3] = Sn.x;
-            a[4] = Sn.y;
-            a[5] = Sn.z;
-            b = Sn.x * (S.x - ux) + Sn.y * (S.y - uy) + Sn.z * (S.z - uz);
-          }
+      if (u > 0 && u <= W && v > 0 && v <= H) jx[k] = (v - 1) * W + (u - 1);
+    }
+  }
+  float4 S[PX], Sn[PX];
+#pragma unroll
+  for (int k = 0; k < PX; ++k) {
+    S[k] = jx[k] >= 0 ? mv[jx[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    Sn[k] = jx[k] >= 0 ? mn[jx[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // phase 3: gates, Jacobian rows, accumulation
+#pragma unroll
+  for (int k = 0; k < PX; ++k) {
+    const int x = base + k * KF_ICP_BLOCK;
+    bool ok = false;
+    float a[6], b = 0.f;
+    if (jx[k] >= 0) {
+      const float4 Vk = V[k];
+      const float ux = Te[0] * Vk.x + Te[1] * Vk.y + Te[2] * Vk.z + Te[3];
+      const float uy = Te[4] * Vk.x + Te[5] * Vk.y + Te[6] * Vk.z + Te[7];
+      const float uz = Te[8] * Vk.x + Te[9] * Vk.y + Te[10] * Vk.z + Te[11];
+      const float4 Sk = S[k];
+      const float ddx = ux - Sk.x, ddy = uy - Sk.y, ddz = uz - Sk.z;
+      const float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+      if (dist <= dist_gate && dist >= 0.0f) {
+        const float4 Nk = Nc[k];
+        float nnx = Te[0] * Nk.x + Te[1] * Nk.y + Te[2] * Nk.z;
+        float nny = Te[4] * Nk.x + Te[5] * Nk.y + Te[6] * Nk.z;
+        float nnz = Te[8] * Nk.x + Te[9] * Nk.y + Te[10] * Nk.z;
+        normalize3(nnx, nny, nnz);
+        const float4 Snk = Sn[k];
+        float ax = Snk.x, ay = Snk.y, az = Snk.z;
+        normalize3(ax, ay, az);
+        const float c = (nnx * ax + nny * ay + nnz * az);
+        if (c >= cos_lo && c <= 1.0f) {
+          ok = true;
+          a[0] = uz * Snk.y + (-uy * Snk.z);
+          a[1] = -uz * Snk.x + (ux * Snk.z);
+          a[2] = uy * Snk.x + (-ux * Snk.y);
+          a[3] = Snk.x;
+          a[4] = Snk.y;
+          a[5] = Snk.z;
+          b = Snk.x * (Sk.x - ux) + Snk.y * (Sk.y - uy) + Snk.z * (Sk.z - uz);
         }
       }
     }
-    if (mask_out) mask_out[x] = ok ? 1 : 0;
+    if (mask_out && x < n) mask_out[x] = ok ? 1 : 0;
     if (ok) {
       int q = 0;
 #pragma unroll
@@ -372,19 +487,19 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 
   // wave reduction -> LDS.  Kahan keeps s - c as the value (c is the running
   // error with the opposite sign), so the pair (s, -c) is reduced.
-  const int wave = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6, ln = lane_id();
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     if constexpr (KAHAN) {
       float s = acc[k], c = -comp[k];
       wave_sum_comp(s, c);
-      if (lane_id() == 63) {
+      if (ln == 63) {
         red_s[wave][k] = s;
         red_c[wave][k] = c;
       }
     } else {
       const float s = wave_sum(acc[k]);
-      if (lane_id() == 63) red_s[wave][k] = s;
+      if (ln == 63) red_s[wave][k] = s;
     }
   }
   __syncthreads();
@@ -398,59 +513,38 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       else
         s = s + red_s[w][k];
     }
-    partials[((size_t)blockIdx.x * NS + k) * 2 + 0] = s;
-    partials[((size_t)blockIdx.x * NS + k) * 2 + 1] = c;
+    if constexpr (KAHAN) {
+      part_out[(size_t)(2 * k) * nb + blockIdx.x] = s;
+      part_out[(size_t)(2 * k + 1) * nb + blockIdx.x] = c;
+    } else {
+      part_out[(size_t)k * nb + blockIdx.x] = s;
+    }
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = atomicAdd(&st->ticket, 1u);
-    is_last = (t == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
+}
 
-  // Last block: FP64 sum of the block partials in block order.
-  const int nb = gridDim.x;
-  if (threadIdx.x < NS * 8) {
-    const int k = threadIdx.x >> 3, ch = threadIdx.x & 7;
-    double s = 0.0;
-    for (int bk = ch; bk < nb; bk += 8) {
-      const float* p = partials + ((size_t)bk * NS + k) * 2;
-      s += (double)p[0] + (double)p[1];
-    }
-    fin[k][ch] = s;
+// Finishes the last iteration: final pose, its inverse and the record.
+template <bool KAHAN>
+__global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_finish(kf_dev_state* __restrict__ st,
+                                                             const float* __restrict__ part_in,
+                                                             int nb, int last) {
+  constexpr int ROWS = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
+  __shared__ IcpShared<ROWS> sh;
+  if (threadIdx.x < 16) {
+    sh.T0[threadIdx.x] = last == 0 ? st->Tprev[threadIdx.x] : st->iter_T[last - 1][threadIdx.x];
+    sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
   }
-  __syncthreads();
-  if (threadIdx.x < NS) {
-    double s = 0.0;
-#pragma unroll
-    for (int ch = 0; ch < 8; ++ch) s += fin[threadIdx.x][ch];
-    sums_d[threadIdx.x] = s;
-    st->iter_sums[iter][threadIdx.x] = s;
+  icp_reduce_partials<ROWS>(part_in, nb, sh);
+  icp_update_pose<ROWS>(sh);
+  icp_record<ROWS>(st, last, sh);
+  if (threadIdx.x < 16) {
+    st->T[threadIdx.x] = sh.Tn[threadIdx.x];
+    st->F[threadIdx.x] = sh.F[threadIdx.x];
   }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    float T[16], Tp[16], Fn[16];
-    for (int i = 0; i < 16; ++i) T[i] = st->T[i], Tp[i] = st->Tprev_inv[i];
-    const int status = kf_icp_update(sums_d, T, nullptr);
-    kf_mat4_mul(Tp, T, Fn);
-    for (int i = 0; i < 16; ++i) {
-      st->T[i] = T[i];
-      st->F[i] = Fn[i];
-      st->iter_T[iter][i] = T[i];
-    }
-    st->iter_status[iter] = status;
-    st->icp_iters_done = iter + 1;
-    if (last) {
-      float Ti[16];
-      if (kf_m4_inverse(T, Ti) != 0)
-        for (int i = 0; i < 16; ++i) Ti[i] = __builtin_nanf("");
-      for (int i = 0; i < 16; ++i) st->Tinv[i] = Ti[i];
-    }
-    __threadfence();
-    atomicExch(&st->ticket, 0u);
+    float Ti[16];
+    if (kf_m4_inverse(sh.Tn, Ti) != 0)
+      for (int i = 0; i < 16; ++i) Ti[i] = __builtin_nanf("");
+    for (int i = 0; i < 16; ++i) st->Tinv[i] = Ti[i];
   }
 }
 
@@ -518,18 +612,42 @@ __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ d
   tab[i] = out;
 }
 
-// Per-voxel part.  One thread per voxel, 4 voxels per thread strided by the
-// block so that a wave touches 64 consecutive z voxels (one coalesced 512 B
-// F32 / 256 B F16 segment).  The voxel is read and written only when its
-// signed distance falls in [sdf_min, sdf_max], as in the reference.
-constexpr int kIntegVPT = 4;
+// Per-voxel part.  A 256-thread block owns 32 consecutive bricks (8x8x8
+// voxels each).
+//
+// * Brick cull.  Lane (brick, corner) evaluates, in FP64, the affine
+//   quantities the per-voxel test depends on at one of the brick's 8 corner
+//   voxel centres: the projection p = K T^-1 c and the front-test dot
+//   mag = dir . (c - t).  If one of "mag < -M", "pz > M and px < 0.45 pz - M",
+//   "px > (W + 0.55) pz + M" or the same two for py/H holds at all 8 corners,
+//   it holds on the whole brick, and the margin M (16 + 3e-5 |K| |l|, many
+//   times the rounding error of the float evaluation) guarantees that the
+//   reference's float test rejects every voxel of the brick.  Such bricks are
+//   skipped; no decision changes.  The surviving bricks of the block are
+//   shared by its 4 waves.
+// * Per brick, lane (lx, ly) walks its z column of 8 voxels.  The reference
+//   sums T^-1 c left to right, so Ti0*cx + Ti1*cy (and the x/y parts of the
+//   front test and of |c - t|^2) are the same for the column and are computed
+//   once, bit-identically.
+// * Division pre-check.  When pz > 0 and px < 0.25 pz (or > (W+1) pz, and
+//   the same for y, mirrored for pz < 0), the rounded quotient cannot land in
+//   [1, W]; the voxel is rejected before the two correctly rounded divisions.
+// The voxel is read and written only when its signed distance falls in
+// [sdf_min, sdf_max], as in the reference.
+constexpr int kBricksPerBlock = 32;
 
 template <int PREC, bool FIRST>
-__global__ __launch_bounds__(256) void k_integrate(void* __restrict__ volume,
+__global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
+                                                   void* __restrict__ weight_,
                                                    const float2* __restrict__ tab, kf_cam cam,
                                                    kf_vol vg, int H, int W, float trunc,
                                                    float smin, float smax,
-                                                   const kf_dev_state* __restrict__ st, int nvox) {
+                                                   const kf_dev_state* __restrict__ st,
+                                                   int nbricks) {
+  __shared__ int keep_list[kBricksPerBlock];
+  __shared__ uint32_t keep_mask[4];
+  __shared__ int n_keep;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float Ti[12], t[3], cd[3];
   load_uniform<12>(st->Tinv, Ti);
   t[0] = st->T[3];
@@ -539,76 +657,159 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ volume,
   const float* K = cam.K;
   const float d = vg.d;
   const float hd = d / 2.f;
-#pragma unroll
-  for (int k = 0; k < kIntegVPT; ++k) {
-    const int v = (blockIdx.x * kIntegVPT + k) * 256 + (int)threadIdx.x;
-    if (v >= nvox) return;
-    const uint32_t q1 = magic_div((uint32_t)v, vg.mz, vg.shz);
-    const int iz = v - (int)q1 * vg.nz;
-    const uint32_t q2 = magic_div(q1, vg.my, vg.shy);
-    const int iy = (int)q1 - (int)q2 * vg.ny;
-    const int ix = (int)q2;
+
+  {  // brick cull: thread = (brick of the block, corner)
+    const int brick = blockIdx.x * kBricksPerBlock + (int)(threadIdx.x >> 3);
+    const int corner = threadIdx.x & 7;
+    bool r1 = true, r2 = true, r3 = true, r4 = true, r5 = true;
+    if (brick < nbricks) {
+      const int bz = brick % vg.bz;
+      const int q = brick / vg.bz;
+      const int by = q % vg.by;
+      const int bx = q / vg.by;
+      const int x = min(bx * 8 + ((corner & 4) ? 7 : 0), vg.nx - 1);
+      const int y = min(by * 8 + ((corner & 2) ? 7 : 0), vg.ny - 1);
+      const int z = min(bz * 8 + ((corner & 1) ? 7 : 0), vg.nz - 1);
+      const double cx = (double)((float)(x + vg.sx) * d + hd);
+      const double cy = (double)((float)(y + vg.sy) * d + hd);
+      const double cz = (double)((float)(z + vg.sz) * d + hd);
+      const double lx = (double)Ti[0] * cx + (double)Ti[1] * cy + (double)Ti[2] * cz + (double)Ti[3];
+      const double ly = (double)Ti[4] * cx + (double)Ti[5] * cy + (double)Ti[6] * cz + (double)Ti[7];
+      const double lz = (double)Ti[8] * cx + (double)Ti[9] * cy + (double)Ti[10] * cz + (double)Ti[11];
+      const double px = (double)K[0] * lx + (double)K[1] * ly + (double)K[2] * lz;
+      const double py = (double)K[3] * lx + (double)K[4] * ly + (double)K[5] * lz;
+      const double pz = (double)K[6] * lx + (double)K[7] * ly + (double)K[8] * lz;
+      double kabs = 0.0;
+      for (int i = 0; i < 9; ++i) kabs = fmax(kabs, fabs((double)K[i]));
+      const double M = 16.0 + 3e-5 * kabs * (fabs(lx) + fabs(ly) + fabs(lz) + 1.0);
+      const double tx = cx - (double)t[0], ty = cy - (double)t[1], tz = cz - (double)t[2];
+      const double mag = (double)cd[0] * tx + (double)cd[1] * ty + (double)cd[2] * tz;
+      const double Mm =
+          1.0 + 1e-5 * (fabs((double)cd[0]) + fabs((double)cd[1]) + fabs((double)cd[2])) *
+                    (fabs(tx) + fabs(ty) + fabs(tz));
+      const bool front = pz > M;
+      r1 = mag < -Mm;
+      r2 = front && px < 0.45 * pz - M;
+      r3 = front && px > ((double)W + 0.55) * pz + M;
+      r4 = front && py < 0.45 * pz - M;
+      r5 = front && py > ((double)H + 0.55) * pz + M;
+    }
+    const uint64_t m1 = __ballot(r1), m2 = __ballot(r2), m3 = __ballot(r3), m4 = __ballot(r4),
+                   m5 = __ballot(r5);
+    if (lane == 0) {
+      uint32_t keep = 0;
+      for (int j = 0; j < 8; ++j) {
+        const int sh = 8 * j;
+        const bool rej = ((m1 >> sh) & 0xff) == 0xff || ((m2 >> sh) & 0xff) == 0xff ||
+                         ((m3 >> sh) & 0xff) == 0xff || ((m4 >> sh) & 0xff) == 0xff ||
+                         ((m5 >> sh) & 0xff) == 0xff;
+        if (!rej) keep |= 1u << j;
+      }
+      keep_mask[wave] = keep;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int n = 0;
+      for (int w = 0; w < 4; ++w)
+        for (int j = 0; j < 8; ++j)
+          if (keep_mask[w] & (1u << j)) keep_list[n++] = blockIdx.x * kBricksPerBlock + w * 8 + j;
+      n_keep = n;
+    }
+    __syncthreads();
+  }
+
+  const float Wp1 = (float)(W + 1), Hp1 = (float)(H + 1);
+  for (int kb = wave; kb < n_keep; kb += 4) {
+    const int brick = keep_list[kb];
+    const int bz = brick % vg.bz;
+    const int q = brick / vg.bz;
+    const int by = q % vg.by;
+    const int bx = q / vg.by;
+    const int ix = bx * 8 + (lane >> 3), iy = by * 8 + (lane & 7);
+    if (ix >= vg.nx || iy >= vg.ny) continue;
     const float cx = (float)(ix + vg.sx) * d + hd;
     const float cy = (float)(iy + vg.sy) * d + hd;
-    const float cz = (float)(iz + vg.sz) * d + hd;
-    const float lx = Ti[0] * cx + Ti[1] * cy + Ti[2] * cz + Ti[3];
-    const float ly = Ti[4] * cx + Ti[5] * cy + Ti[6] * cz + Ti[7];
-    const float lz = Ti[8] * cx + Ti[9] * cy + Ti[10] * cz + Ti[11];
-    const float px = K[0] * lx + K[1] * ly + K[2] * lz;
-    const float py = K[3] * lx + K[4] * ly + K[5] * lz;
-    const float pz = K[6] * lx + K[7] * ly + K[8] * lz;
-    const int u = kf_round_to_int(px / pz);
-    const int pv = kf_round_to_int(py / pz);
-    if (!(u > 0 && u <= W && pv > 0 && pv <= H)) continue;
-    const float2 tb = tab[(pv - 1) * W + (u - 1)];
-    if (tb.x != tb.x) continue;  // depth 0 at that pixel
-    const float tdx = cx - t[0], tdy = cy - t[1], tdz = cz - t[2];
-    const float mag = cd[0] * tdx + cd[1] * tdy + cd[2] * tdz;
-    if (!(mag > 0)) continue;
-    const float tm = sqrtf(tdx * tdx + tdy * tdy + tdz * tdz);
-    const float frk = tb.x - (tb.y * tm);
-    const bool pos = frk > 0 && frk <= smax;
-    const bool neg = frk <= 0 && frk >= smin;
-    if (!pos && !neg) continue;
-    const float f = pos ? fminf(1.f, (frk / trunc)) : fmaxf(-1.f, frk / trunc);
-    if constexpr (PREC == KF_TSDF_F32) {
-      float2* vol = reinterpret_cast<float2*>(volume);
-      if constexpr (FIRST) {
-        vol[v] = make_float2(f, 1.0f);
-      } else {
-        float2 fw = vol[v];
-        if (fw.x != fw.x) fw.x = 0.0f;
-        if (fw.y != fw.y) fw.y = 0.0f;
-        fw.x = (fw.y * fw.x + 1.0f * f) / (fw.y + 1.0f);
-        fw.y = fw.y + 1.0f;
-        vol[v] = fw;
+    const float Px = Ti[0] * cx + Ti[1] * cy;
+    const float Py = Ti[4] * cx + Ti[5] * cy;
+    const float Pz = Ti[8] * cx + Ti[9] * cy;
+    const float tdx = cx - t[0], tdy = cy - t[1];
+    const float Pm = cd[0] * tdx + cd[1] * tdy;
+    const float Pt = tdx * tdx + tdy * tdy;
+    const int base = (brick << 9) | (lane << 3);
+    const int zn = min(8, vg.nz - bz * 8);
+    for (int lz = 0; lz < zn; ++lz) {
+      const float cz = (float)(bz * 8 + lz + vg.sz) * d + hd;
+      const float lx = (Px + Ti[2] * cz) + Ti[3];
+      const float ly = (Py + Ti[6] * cz) + Ti[7];
+      const float lzz = (Pz + Ti[10] * cz) + Ti[11];
+      const float px = K[0] * lx + K[1] * ly + K[2] * lzz;
+      const float py = K[3] * lx + K[4] * ly + K[5] * lzz;
+      const float pz = K[6] * lx + K[7] * ly + K[8] * lzz;
+      if (pz > 0.f) {
+        if (px < 0.25f * pz || px > Wp1 * pz || py < 0.25f * pz || py > Hp1 * pz) continue;
+      } else if (pz < 0.f) {
+        if (px > 0.25f * pz || px < Wp1 * pz || py > 0.25f * pz || py < Hp1 * pz) continue;
       }
-    } else {
-      uint32_t* vol = reinterpret_cast<uint32_t*>(volume);
-      if constexpr (FIRST) {
-        vol[v] = (1u << 16) | (uint32_t)__half_as_ushort(__float2half_rn(f));
+      const int u = kf_round_to_int(px / pz);
+      const int pv = kf_round_to_int(py / pz);
+      if (!(u > 0 && u <= W && pv > 0 && pv <= H)) continue;
+      const float2 tb = tab[(pv - 1) * W + (u - 1)];
+      if (tb.x != tb.x) continue;  // depth 0 at that pixel
+      const float tdz = cz - t[2];
+      const float mag = Pm + cd[2] * tdz;
+      if (!(mag > 0)) continue;
+      const float tm = sqrtf(Pt + tdz * tdz);
+      const float frk = tb.x - (tb.y * tm);
+      const bool pos = frk > 0 && frk <= smax;
+      const bool neg = frk <= 0 && frk >= smin;
+      if (!pos && !neg) continue;
+      const float f = pos ? fminf(1.f, (frk / trunc)) : fmaxf(-1.f, frk / trunc);
+      const int v = base + lz;
+      if constexpr (PREC == KF_TSDF_F32) {
+        float* T = reinterpret_cast<float*>(tsdf_);
+        float* Wt = reinterpret_cast<float*>(weight_);
+        if constexpr (FIRST) {
+          T[v] = f;
+          Wt[v] = 1.0f;
+        } else {
+          float F0 = T[v], W0 = Wt[v];
+          if (F0 != F0) F0 = 0.0f;
+          if (W0 != W0) W0 = 0.0f;
+          T[v] = (W0 * F0 + 1.0f * f) / (W0 + 1.0f);
+          Wt[v] = W0 + 1.0f;
+        }
       } else {
-        const uint32_t p = vol[v];
-        float F0 = f16_lo_to_f32(p);
-        if (F0 != F0) F0 = 0.0f;
-        const uint32_t w = p >> 16;
-        const float Wt = (float)w;
-        const float Fn = (Wt * F0 + 1.0f * f) / (Wt + 1.0f);
-        const uint32_t w2 = w == 65535u ? 65535u : w + 1u;
-        vol[v] = (w2 << 16) | (uint32_t)__half_as_ushort(__float2half_rn(Fn));
+        uint16_t* T = reinterpret_cast<uint16_t*>(tsdf_);
+        uint16_t* Wt = reinterpret_cast<uint16_t*>(weight_);
+        if constexpr (FIRST) {
+          T[v] = __half_as_ushort(__float2half_rn(f));
+          Wt[v] = 1;
+        } else {
+          float F0 = __half2float(__ushort_as_half(T[v]));
+          if (F0 != F0) F0 = 0.0f;
+          const uint32_t w = Wt[v];
+          const float W0 = (float)w;
+          const float Fn = (W0 * F0 + 1.0f * f) / (W0 + 1.0f);
+          T[v] = __half_as_ushort(__float2half_rn(Fn));
+          Wt[v] = (uint16_t)(w == 65535u ? 65535u : w + 1u);
+        }
       }
     }
   }
 }
 
-// Never-observed fill: F32 {NaN, NaN}; F16 {half NaN, 0}.
-__global__ __launch_bounds__(256) void k_volume_fill(void* volume, int prec, int64_t nvox) {
+// Never-observed fill: F32 TSDF and weight NaN; F16 TSDF half NaN, weight 0.
+__global__ __launch_bounds__(256) void k_volume_fill(void* tsdf, void* weight, int prec,
+                                                     int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= nvox) return;
-  if (prec == KF_TSDF_F32)
-    reinterpret_cast<float2*>(volume)[i] = make_float2(__builtin_nanf(""), __builtin_nanf(""));
-  else
-    reinterpret_cast<uint32_t*>(volume)[i] = 0x7e00u;
+  if (i >= n) return;
+  if (prec == KF_TSDF_F32) {
+    reinterpret_cast<float*>(tsdf)[i] = __builtin_nanf("");
+    reinterpret_cast<float*>(weight)[i] = __builtin_nanf("");
+  } else {
+    reinterpret_cast<uint16_t*>(tsdf)[i] = 0x7e00u;
+    reinterpret_cast<uint16_t*>(weight)[i] = 0u;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -618,244 +819,395 @@ __global__ __launch_bounds__(256) void k_volume_fill(void* volume, int prec, int
 // rays walk neighbouring voxels.  Consecutive block ids are dealt round-robin
 // to the 8 XCDs; the id is remapped so that each XCD gets a contiguous band
 // of the image and its L2 sees the band's voxels.
+// Traversal state of voxel_traversal.cu:133-205: the 1-based voxel, the
+// indices recorded at the last step on each axis (XY, XZ: y and z at the
+// last x step, ...) and the parametric distances to the next boundaries.
+struct DdaState {
+  int ix, iy, iz, XY, XZ, YX, YZ, ZX, ZY;
+  float tmx, tmy, tmz;
+};
+struct DdaStep {
+  int sx, sy, sz;
+  float tdx, tdy, tdz;
+};
+
+// One Amanatides-Woo step with the reference's tie order (:384-417):
+//   tmx < tmy ? (tmx < tmz ? x : z) : (tmy < tmz ? y : z),
+// written with selects so that a wave steps its 64 rays without branching.
+// NaN comparisons are false in both forms, so the choice is the same.
+__device__ __forceinline__ void dda_step(DdaState& s, const DdaStep& p) {
+  const bool xy = s.tmx < s.tmy;
+  const bool xm = xy && s.tmx < s.tmz;
+  const bool ym = !xy && s.tmy < s.tmz;
+  const bool zm = !xm && !ym;
+  const int ix = s.ix, iy = s.iy, iz = s.iz;
+  const float nx = s.tmx + p.tdx, ny = s.tmy + p.tdy, nz = s.tmz + p.tdz;
+  s.ix = xm ? ix + p.sx : ix;
+  s.XY = xm ? iy : s.XY;
+  s.XZ = xm ? iz : s.XZ;
+  s.tmx = xm ? nx : s.tmx;
+  s.iy = ym ? iy + p.sy : iy;
+  s.YX = ym ? ix : s.YX;
+  s.YZ = ym ? iz : s.YZ;
+  s.tmy = ym ? ny : s.tmy;
+  s.iz = zm ? iz + p.sz : iz;
+  s.ZX = zm ? ix : s.ZX;
+  s.ZY = zm ? iy : s.ZY;
+  s.tmz = zm ? nz : s.tmz;
+}
+
+// Zero crossing found at voxel h (voxel_traversal.cu:222-377): bound checks,
+// 13-tap NaN checks, per-axis interpolation, central-difference normal and
+// the flip towards the camera.  Returns the 0-based voxel index, or -1 when a
+// check rejects the crossing.
+template <int PREC>
+__device__ __forceinline__ int crossing(const VolRead<PREC>& F, const DdaState& h, float f,
+                                        const kf_vol& vg, int fix_bounds, float camx,
+                                        float camy, float camz, float dx, float dy, float dz,
+                                        float flip_hi, float4& ov, float4& on) {
+  const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
+  const int NyNz = Ny * Nz;
+  const float d = vg.d;
+  const int bXY = fix_bounds ? Ny : Nx, bXZ = fix_bounds ? Nz : Nx;
+  const int bYX = fix_bounds ? Nx : Ny, bYZ = fix_bounds ? Nz : Ny;
+  const int bZX = fix_bounds ? Nx : Nz, bZY = fix_bounds ? Ny : Nz;
+  const int ix = h.ix, iy = h.iy, iz = h.iz;
+  const int XY = h.XY, XZ = h.XZ, YX = h.YX, YZ = h.YZ, ZX = h.ZX, ZY = h.ZY;
+  const int vi = (ix - 1) * NyNz + (iy - 1) * Nz + (iz - 1);
+  const int xn = (ix - 1) - 1, xp = (ix - 1) + 1;
+  const int yn = (iy - 1) - 1, yp = (iy - 1) + 1;
+  const int zn = (iz - 1) - 1, zp = (iz - 1) + 1;
+  if (xn >= Nx || xn < 0 || xp >= Nx || xp < 0) return -1;
+  if ((XY - 1) >= bXY || (XY - 1) < 0 || (XZ - 1) >= bXZ || (XZ - 1) < 0) return -1;
+  if (yn >= Ny || yn < 0 || yp >= Ny || yp < 0) return -1;
+  if ((YX - 1) >= bYX || (YX - 1) < 0 || (YZ - 1) >= bYZ || (YZ - 1) < 0) return -1;
+  if (zn >= Nz || zn < 0 || zp >= Nz || zp < 0) return -1;
+  if ((ZX - 1) >= bZX || (ZX - 1) < 0 || (ZY - 1) >= bZY || (ZY - 1) < 0) return -1;
+  const float f_xnp = F(kf_store_index(vg, xn, XY - 1, XZ - 1));
+  const float f_xpp = F(kf_store_index(vg, xp, XY - 1, XZ - 1));
+  const float f_ynp = F(kf_store_index(vg, YX - 1, yn, YZ - 1));
+  const float f_ypp = F(kf_store_index(vg, YX - 1, yp, YZ - 1));
+  const float f_znp = F(kf_store_index(vg, ZX - 1, ZY - 1, zn));
+  const float f_zpp = F(kf_store_index(vg, ZX - 1, ZY - 1, zp));
+  const float f_xnn = F(kf_store_index(vg, ix - 2, iy - 1, iz - 1));
+  const float f_xpn = F(kf_store_index(vg, ix, iy - 1, iz - 1));
+  const float f_ynn = F(kf_store_index(vg, ix - 1, iy - 2, iz - 1));
+  const float f_ypn = F(kf_store_index(vg, ix - 1, iy, iz - 1));
+  const float f_znn = F(kf_store_index(vg, ix - 1, iy - 1, iz - 2));
+  const float f_zpn = F(kf_store_index(vg, ix - 1, iy - 1, iz));
+  if (f_xnp != f_xnp || f_xpp != f_xpp || f_ynp != f_ynp || f_ypp != f_ypp || f_znp != f_znp ||
+      f_zpp != f_zpp || f != f)
+    return -1;
+  if (f_xnn != f_xnn || f_xpn != f_xpn || f_ynn != f_ynn || f_ypn != f_ypn || f_znn != f_znn ||
+      f_zpn != f_zpn)
+    return -1;
+  const float hd = d / 2.f;
+  float spx, spy, spz;
+  if (dx > 0) {
+    const float g = (float)(xn + vg.sx) * d + hd;
+    spx = g + d * (fabsf(f_xnp) / (fabsf(f_xnp - f)));
+  } else {
+    const float g = (float)(xp + vg.sx) * d + hd;
+    spx = g - d * (fabsf(f_xpp) / (fabsf(f_xpp - f)));
+  }
+  if (dy > 0) {
+    const float g = (float)(yn + vg.sy) * d + hd;
+    spy = g + d * (fabsf(f_ynp) / (fabsf(f_ynp - f)));
+  } else {
+    const float g = (float)(yp + vg.sy) * d + hd;
+    spy = g - d * (fabsf(f_ypp) / (fabsf(f_ypp - f)));
+  }
+  if (dz > 0) {
+    const float g = (float)(zn + vg.sz) * d + hd;
+    spz = g + d * (fabsf(f_znp) / (fabsf(f_znp - f)));
+  } else {
+    const float g = (float)(zp + vg.sz) * d + hd;
+    spz = g - d * (fabsf(f_zpp) / (fabsf(f_zpp - f)));
+  }
+  float bxn = (f_xpn - f_xnn) / 2.0f;
+  float byn = (f_ypn - f_ynn) / 2.0f;
+  float bzn = (f_zpn - f_znn) / 2.0f;
+  float ax = camx - spx, ay = camy - spy, az = camz - spz;
+  normalize3(ax, ay, az);
+  normalize3(bxn, byn, bzn);
+  const float c = ax * bxn + ay * byn + az * bzn;
+  if (c < flip_hi && c >= -1.0f) {
+    bxn = -bxn;
+    byn = -byn;
+    bzn = -bzn;
+  }
+  ov = make_float4(spx, spy, spz, 0.f);
+  on = make_float4(bxn, byn, bzn, 0.f);
+  return vi;
+}
+
+// A ray's traversal state between the two raycast passes.
+struct RayState {
+  DdaState cur;
+  float prev;
+  float dx, dy, dz;
+  DdaStep stp;
+  int pix;
+};
+
+// Ray setup (voxel_traversal.cu:90-200): ray skipping to (depth - trunc),
+// slab test, 1-based start voxel, steps and boundary distances.  Returns
+// false when the ray never enters the traversal.
+__device__ __forceinline__ bool ray_setup(int x, int y, float dval, const float* Ki,
+                                          const float* T, const kf_vol& vg, float trunc,
+                                          RayState& r) {
+  const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
+  const float d = vg.d;
+  const float X = (float)x + 1.0f, Y = (float)y + 1.0f;
+  const float lamx = Ki[0] * X + Ki[1] * Y + Ki[2] * 1.f;
+  const float lamy = Ki[3] * X + Ki[4] * Y + Ki[5] * 1.f;
+  const float lamz = Ki[6] * X + Ki[7] * Y + Ki[8] * 1.f;
+  const float lam = sqrtf(lamx * lamx + lamy * lamy + lamz * lamz);
+  const float sc = (lam * dval - trunc) / lam;
+  const float slx = sc * lamx, sly = sc * lamy, slz = sc * lamz;
+  const float sgx = T[0] * slx + T[1] * sly + T[2] * slz + T[3] * 1.f;
+  const float sgy = T[4] * slx + T[5] * sly + T[6] * slz + T[7] * 1.f;
+  const float sgz = T[8] * slx + T[9] * sly + T[10] * slz + T[11] * 1.f;
+  const float camx = T[3], camy = T[7], camz = T[11];
+  const float rgx = T[0] * lamx + T[1] * lamy + T[2] * lamz + T[3] * 1.f;
+  const float rgy = T[4] * lamx + T[5] * lamy + T[6] * lamz + T[7] * 1.f;
+  const float rgz = T[8] * lamx + T[9] * lamy + T[10] * lamz + T[11] * 1.f;
+  float dx = rgx - camx, dy = rgy - camy, dz = rgz - camz;
+  normalize3(dx, dy, dz);
+  const float mnx = (float)vg.sx * d, mny = (float)vg.sy * d, mnz = (float)vg.sz * d;
+  const float mxx = (float)(vg.sx + Nx) * d, mxy = (float)(vg.sy + Ny) * d,
+              mxz = (float)(vg.sz + Nz) * d;
+  // Ray_Box_Intersection (voxel_traversal.cu:4-73)
+  float tmin, tmax, tymin, tymax, tzmin, tzmax;
+  if (dx >= 0) {
+    tmin = (mnx - sgx) / dx;
+    tmax = (mxx - sgx) / dx;
+  } else {
+    tmin = (mxx - sgx) / dx;
+    tmax = (mnx - sgx) / dx;
+  }
+  if (dy >= 0) {
+    tymin = (mny - sgy) / dy;
+    tymax = (mxy - sgy) / dy;
+  } else {
+    tymin = (mxy - sgy) / dy;
+    tymax = (mny - sgy) / dy;
+  }
+  if ((tmin > tymax) || (tymin > tmax)) return false;
+  if (tymin > tmin) tmin = tymin;
+  if (tymax < tmax) tmax = tymax;
+  if (dz >= 0) {
+    tzmin = (mnz - sgz) / dz;
+    tzmax = (mxz - sgz) / dz;
+  } else {
+    tzmin = (mxz - sgz) / dz;
+    tzmax = (mnz - sgz) / dz;
+  }
+  if ((tmin > tzmax) || (tzmin > tmax)) return false;
+  if (tzmin > tmin) tmin = tzmin;
+  if (tmin < 0) tmin = 0;
+  const float stx = sgx + tmin * dx, sty = sgy + tmin * dy, stz = sgz + tmin * dz;
+  const float bxs = mxx - mnx, bys = mxy - mny, bzs = mxz - mnz;
+  int ix = kf_floor_to_int(((stx - mnx) / bxs) * (float)Nx) + 1;
+  int iy = kf_floor_to_int(((sty - mny) / bys) * (float)Ny) + 1;
+  int iz = kf_floor_to_int(((stz - mnz) / bzs) * (float)Nz) + 1;
+  if (ix == Nx + 1) ix = ix - 1;
+  if (iy == Ny + 1) iy = iy - 1;
+  if (iz == Nz + 1) iz = iz - 1;
+  float tvx, tvy, tvz;
+  int stx_, sty_, stz_;
+  if (dx >= 0) {
+    tvx = (float)(ix) / (float)Nx;
+    stx_ = 1;
+  } else {
+    tvx = (float)(ix - 1) / (float)Nx;
+    stx_ = -1;
+  }
+  if (dy >= 0) {
+    tvy = (float)(iy) / (float)Ny;
+    sty_ = 1;
+  } else {
+    tvy = (float)(iy - 1) / (float)Ny;
+    sty_ = -1;
+  }
+  if (dz >= 0) {
+    tvz = (float)(iz) / (float)Nz;
+    stz_ = 1;
+  } else {
+    tvz = (float)(iz - 1) / (float)Nz;
+    stz_ = -1;
+  }
+  const float vmx = mnx + tvx * bxs, vmy = mny + tvy * bys, vmz = mnz + tvz * bzs;
+  r.cur.ix = ix;
+  r.cur.iy = iy;
+  r.cur.iz = iz;
+  // XY..ZY start at the start voxel (the reference reads them before the
+  // first step on that axis, voxel_traversal.cu:136)
+  r.cur.XY = iy;
+  r.cur.XZ = iz;
+  r.cur.YX = ix;
+  r.cur.YZ = iz;
+  r.cur.ZX = ix;
+  r.cur.ZY = iy;
+  r.cur.tmx = tmin + ((vmx - stx) / dx);
+  r.cur.tmy = tmin + ((vmy - sty) / dy);
+  r.cur.tmz = tmin + ((vmz - stz) / dz);
+  r.stp = DdaStep{stx_, sty_, stz_, d / fabsf(dx), d / fabsf(dy), d / fabsf(dz)};
+  r.prev = 0.0f;
+  r.dx = dx;
+  r.dy = dy;
+  r.dz = dz;
+  return true;
+}
+
+// Walks up to max_batches batches of the traversal (voxel_traversal.cu:203-
+// 421).  The voxel sequence does not depend on the TSDF values, only where
+// the walk stops does, so each batch generates kRayBatch steps ahead, issues
+// their loads together and then applies the reference's per-voxel tests
+// (:207-222) in order: the same voxels are visited and the same decisions
+// taken.  Returns true when the ray is finished (hit or not); on false the
+// state is ready to continue.
+template <int PREC, int kRayBatch>
+__device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, const kf_vol& vg,
+                                         int fix_bounds, const float* cam, float flip_hi,
+                                         int max_batches, float4& ov, float4& on, int& hit) {
+  const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
+  for (int bt = 0; bt < max_batches; ++bt) {
+    // generate the next kRayBatch voxels of the walk (branch-free: the walk
+    // keeps stepping past its exit, but those voxels are never read)
+    int vis[kRayBatch];
+    int nv = 0;
+    bool alive = true;
+    DdaState s = r.cur;
+#pragma unroll
+    for (int k = 0; k < kRayBatch; ++k) {
+      const bool in =
+          s.ix < Nx - 1 && s.ix > 0 && s.iy < Ny - 1 && s.iy > 0 && s.iz < Nz - 1 && s.iz > 0;
+      alive = alive && in;
+      vis[k] = alive ? kf_store_index(vg, s.ix - 1, s.iy - 1, s.iz - 1) : 0;
+      nv += alive ? 1 : 0;
+      dda_step(s, r.stp);
+    }
+    float fs[kRayBatch];
+#pragma unroll
+    for (int k = 0; k < kRayBatch; ++k) fs[k] = F(vis[k]);
+    // the reference's per-voxel tests, in order
+    bool done = false;
+    int kc = -1;
+    float fc = 0.0f;
+    float prev = r.prev;
+#pragma unroll
+    for (int k = 0; k < kRayBatch; ++k) {
+      const float f = fs[k];
+      const bool valid = k < nv && !done;
+      const bool brk = (f != f && prev < 0.f) || (f < 0.f && prev != prev) || (f > 0.f && prev < 0.f);
+      const bool cross = !brk && f < 0.f && prev > 0.f;
+      kc = (valid && cross) ? k : kc;
+      fc = (valid && cross) ? f : fc;
+      prev = (valid && !brk && !cross) ? f : prev;
+      done = done || (valid && (brk || cross));
+    }
+    r.prev = prev;
+    if (kc >= 0) {
+      DdaState h = r.cur;
+      for (int j = 0; j < kc; ++j) dda_step(h, r.stp);
+      hit = crossing<PREC>(F, h, fc, vg, fix_bounds, cam[0], cam[1], cam[2], r.dx, r.dy, r.dz,
+                           flip_hi, ov, on);
+    }
+    if (done || nv < kRayBatch) return true;
+    r.cur = s;
+  }
+  return false;
+}
+
+// Pass 1: every pixel with depth != 0 casts its ray (voxel_traversal.cu:
+// 75-424) for up to kPass1Batches batches.  Rays still walking after that
+// (a few per cent: long walks through unobserved or free space) are appended
+// to a queue and finished by k_raycast_rest in full waves, instead of keeping
+// a whole wave busy for one long ray.
+//
+// A 256-thread block covers 16x16 pixels as four 8x8 wave tiles, so a wave's
+// rays walk neighbouring voxels.  Consecutive block ids are dealt round-robin
+// to the 8 XCDs; the id is remapped so that each XCD gets a contiguous band
+// of the image and its L2 sees the band's voxels.
+constexpr int kPass1Batches = 2;  // of 16 steps
+
 template <int PREC>
 __global__ __launch_bounds__(256) void k_raycast(
     const void* __restrict__ volume, const float* __restrict__ depth, kf_cam cam, kf_vol vg,
     int H, int W, float trunc, int fix_bounds, float flip_hi,
-    const kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
-    int32_t* __restrict__ out_hit, int tiles_x, int n_blocks) {
+    kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
+    int32_t* __restrict__ out_hit, RayState* __restrict__ queue, int tiles_x, int n_blocks) {
   int bid = blockIdx.x;
   {
-    const int per = (n_blocks + 7) / 8;
+    // XCD x receives blocks x, x+8, ...: q+1 of them for x < r, q otherwise.
+    const int q = n_blocks >> 3, r = n_blocks & 7;
     const int xcd = bid & 7, loc = bid >> 3;
-    const int rem = n_blocks - (n_blocks / 8) * 8;  // XCDs with one block more
-    const int base = xcd < rem ? xcd * per : rem * per + (xcd - rem) * (per - 1);
+    const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
     bid = base + loc;
   }
   const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int bx = bid % tiles_x, by = bid / tiles_x;
   const int x = bx * 16 + (tile & 1) * 8 + (ln & 7);
   const int y = by * 16 + (tile >> 1) * 8 + (ln >> 3);
-  if (x >= W || y >= H) return;
+  const bool active = x < W && y < H;
   const int p = y * W + x;
 
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
   int hit = -1;
-  const float dval = depth[p];
+  bool pending = false;
+  RayState r;
+  const float dval = active ? depth[p] : 0.0f;
   if (dval != 0) {
     float T[12];
     load_uniform<12>(st->T, T);
-    const float* Ki = cam.Kinv;
-    const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
-    const float d = vg.d;
-    const float X = (float)x + 1.0f, Y = (float)y + 1.0f;
-    const float lamx = Ki[0] * X + Ki[1] * Y + Ki[2] * 1.f;
-    const float lamy = Ki[3] * X + Ki[4] * Y + Ki[5] * 1.f;
-    const float lamz = Ki[6] * X + Ki[7] * Y + Ki[8] * 1.f;
-    const float lam = sqrtf(lamx * lamx + lamy * lamy + lamz * lamz);
-    const float sc = (lam * dval - trunc) / lam;
-    const float slx = sc * lamx, sly = sc * lamy, slz = sc * lamz;
-    const float sgx = T[0] * slx + T[1] * sly + T[2] * slz + T[3] * 1.f;
-    const float sgy = T[4] * slx + T[5] * sly + T[6] * slz + T[7] * 1.f;
-    const float sgz = T[8] * slx + T[9] * sly + T[10] * slz + T[11] * 1.f;
-    const float camx = T[3], camy = T[7], camz = T[11];
-    const float rgx = T[0] * lamx + T[1] * lamy + T[2] * lamz + T[3] * 1.f;
-    const float rgy = T[4] * lamx + T[5] * lamy + T[6] * lamz + T[7] * 1.f;
-    const float rgz = T[8] * lamx + T[9] * lamy + T[10] * lamz + T[11] * 1.f;
-    float dx = rgx - camx, dy = rgy - camy, dz = rgz - camz;
-    normalize3(dx, dy, dz);
-    const float mnx = (float)vg.sx * d, mny = (float)vg.sy * d, mnz = (float)vg.sz * d;
-    const float mxx = (float)(vg.sx + Nx) * d, mxy = (float)(vg.sy + Ny) * d,
-                mxz = (float)(vg.sz + Nz) * d;
-    // Ray_Box_Intersection (voxel_traversal.cu:4-73)
-    bool flag = true;
-    float tmin, tmax, tymin, tymax, tzmin, tzmax;
-    if (dx >= 0) {
-      tmin = (mnx - sgx) / dx;
-      tmax = (mxx - sgx) / dx;
-    } else {
-      tmin = (mxx - sgx) / dx;
-      tmax = (mnx - sgx) / dx;
-    }
-    if (dy >= 0) {
-      tymin = (mny - sgy) / dy;
-      tymax = (mxy - sgy) / dy;
-    } else {
-      tymin = (mxy - sgy) / dy;
-      tymax = (mny - sgy) / dy;
-    }
-    if ((tmin > tymax) || (tymin > tmax)) {
-      flag = false;
-    } else {
-      if (tymin > tmin) tmin = tymin;
-      if (tymax < tmax) tmax = tymax;
-      if (dz >= 0) {
-        tzmin = (mnz - sgz) / dz;
-        tzmax = (mxz - sgz) / dz;
-      } else {
-        tzmin = (mxz - sgz) / dz;
-        tzmax = (mnz - sgz) / dz;
-      }
-      if ((tmin > tzmax) || (tzmin > tmax)) {
-        flag = false;
-      } else {
-        if (tzmin > tmin) tmin = tzmin;
-      }
-    }
-    if (flag) {
-      if (tmin < 0) tmin = 0;
-      const float stx = sgx + tmin * dx, sty = sgy + tmin * dy, stz = sgz + tmin * dz;
-      const float bxs = mxx - mnx, bys = mxy - mny, bzs = mxz - mnz;
-      int ix = kf_floor_to_int(((stx - mnx) / bxs) * (float)Nx) + 1;
-      int iy = kf_floor_to_int(((sty - mny) / bys) * (float)Ny) + 1;
-      int iz = kf_floor_to_int(((stz - mnz) / bzs) * (float)Nz) + 1;
-      if (ix == Nx + 1) ix = ix - 1;
-      if (iy == Ny + 1) iy = iy - 1;
-      if (iz == Nz + 1) iz = iz - 1;
-      int XY = iy, XZ = iz, YX = ix, YZ = iz, ZX = ix, ZY = iy;
-      float tvx, tvy, tvz;
-      int stx_, sty_, stz_;
-      if (dx >= 0) {
-        tvx = (float)(ix) / (float)Nx;
-        stx_ = 1;
-      } else {
-        tvx = (float)(ix - 1) / (float)Nx;
-        stx_ = -1;
-      }
-      if (dy >= 0) {
-        tvy = (float)(iy) / (float)Ny;
-        sty_ = 1;
-      } else {
-        tvy = (float)(iy - 1) / (float)Ny;
-        sty_ = -1;
-      }
-      if (dz >= 0) {
-        tvz = (float)(iz) / (float)Nz;
-        stz_ = 1;
-      } else {
-        tvz = (float)(iz - 1) / (float)Nz;
-        stz_ = -1;
-      }
-      const float vmx = mnx + tvx * bxs, vmy = mny + tvy * bys, vmz = mnz + tvz * bzs;
-      float tmx = tmin + ((vmx - stx) / dx);
-      float tmy = tmin + ((vmy - sty) / dy);
-      float tmz = tmin + ((vmz - stz) / dz);
-      const float tdx = d / fabsf(dx), tdy = d / fabsf(dy), tdz = d / fabsf(dz);
-      float prev = 0.0f;
-      const int NyNz = Ny * Nz;
-      int vi = (ix - 1) * NyNz + (iy - 1) * Nz + (iz - 1);
-      const int bXY = fix_bounds ? Ny : Nx, bXZ = fix_bounds ? Nz : Nx;
-      const int bYX = fix_bounds ? Nx : Ny, bYZ = fix_bounds ? Nz : Ny;
-      const int bZX = fix_bounds ? Nx : Nz, bZY = fix_bounds ? Ny : Nz;
-      while (ix < Nx - 1 && ix > 0 && iy < Ny - 1 && iy > 0 && iz < Nz - 1 && iz > 0) {
-        const float f = F(vi);
-        if (f != f && prev < 0.f) break;
-        if (f < 0.f && prev != prev) break;
-        if (f > 0.f && prev < 0.f) break;
-        if (f < 0.f && prev > 0.f) {
-          const int xn = (ix - 1) - 1, xp = (ix - 1) + 1;
-          const int yn = (iy - 1) - 1, yp = (iy - 1) + 1;
-          const int zn = (iz - 1) - 1, zp = (iz - 1) + 1;
-          if (xn >= Nx || xn < 0 || xp >= Nx || xp < 0) break;
-          if ((XY - 1) >= bXY || (XY - 1) < 0 || (XZ - 1) >= bXZ || (XZ - 1) < 0) break;
-          if (yn >= Ny || yn < 0 || yp >= Ny || yp < 0) break;
-          if ((YX - 1) >= bYX || (YX - 1) < 0 || (YZ - 1) >= bYZ || (YZ - 1) < 0) break;
-          if (zn >= Nz || zn < 0 || zp >= Nz || zp < 0) break;
-          if ((ZX - 1) >= bZX || (ZX - 1) < 0 || (ZY - 1) >= bZY || (ZY - 1) < 0) break;
-          const int x_np = xn * NyNz + (XY - 1) * Nz + XZ - 1;
-          const int x_pp = xp * NyNz + (XY - 1) * Nz + XZ - 1;
-          const int y_np = (YX - 1) * NyNz + yn * Nz + YZ - 1;
-          const int y_pp = (YX - 1) * NyNz + yp * Nz + YZ - 1;
-          const int z_np = (ZX - 1) * NyNz + (ZY - 1) * Nz + zn;
-          const int z_pp = (ZX - 1) * NyNz + (ZY - 1) * Nz + zp;
-          const int x_nn = vi - NyNz, x_pn = vi + NyNz;
-          const int y_nn = vi - Nz, y_pn = vi + Nz;
-          const int z_nn = vi - 1, z_pn = vi + 1;
-          const float f_xnp = F(x_np), f_xpp = F(x_pp), f_ynp = F(y_np), f_ypp = F(y_pp);
-          const float f_znp = F(z_np), f_zpp = F(z_pp);
-          const float f_xnn = F(x_nn), f_xpn = F(x_pn), f_ynn = F(y_nn), f_ypn = F(y_pn);
-          const float f_znn = F(z_nn), f_zpn = F(z_pn);
-          if (f_xnp != f_xnp || f_xpp != f_xpp || f_ynp != f_ynp || f_ypp != f_ypp ||
-              f_znp != f_znp || f_zpp != f_zpp || f != f)
-            break;
-          if (f_xnn != f_xnn || f_xpn != f_xpn || f_ynn != f_ynn || f_ypn != f_ypn ||
-              f_znn != f_znn || f_zpn != f_zpn)
-            break;
-          const float hd = d / 2.f;
-          float spx, spy, spz;
-          if (dx > 0) {
-            const float g = (float)(xn + vg.sx) * d + hd;
-            spx = g + d * (fabsf(f_xnp) / (fabsf(f_xnp - f)));
-          } else {
-            const float g = (float)(xp + vg.sx) * d + hd;
-            spx = g - d * (fabsf(f_xpp) / (fabsf(f_xpp - f)));
-          }
-          if (dy > 0) {
-            const float g = (float)(yn + vg.sy) * d + hd;
-            spy = g + d * (fabsf(f_ynp) / (fabsf(f_ynp - f)));
-          } else {
-            const float g = (float)(yp + vg.sy) * d + hd;
-            spy = g - d * (fabsf(f_ypp) / (fabsf(f_ypp - f)));
-          }
-          if (dz > 0) {
-            const float g = (float)(zn + vg.sz) * d + hd;
-            spz = g + d * (fabsf(f_znp) / (fabsf(f_znp - f)));
-          } else {
-            const float g = (float)(zp + vg.sz) * d + hd;
-            spz = g - d * (fabsf(f_zpp) / (fabsf(f_zpp - f)));
-          }
-          float bxn = (f_xpn - f_xnn) / 2.0f;
-          float byn = (f_ypn - f_ynn) / 2.0f;
-          float bzn = (f_zpn - f_znn) / 2.0f;
-          float ax = camx - spx, ay = camy - spy, az = camz - spz;
-          normalize3(ax, ay, az);
-          normalize3(bxn, byn, bzn);
-          const float c = ax * bxn + ay * byn + az * bzn;
-          if (c < flip_hi && c >= -1.0f) {
-            bxn = -bxn;
-            byn = -byn;
-            bzn = -bzn;
-          }
-          ov = make_float4(spx, spy, spz, 0.f);
-          on = make_float4(bxn, byn, bzn, 0.f);
-          hit = vi;
-          break;
-        }
-        prev = f;
-        if (tmx < tmy) {
-          if (tmx < tmz) {
-            ix = ix + stx_;
-            XY = iy;
-            XZ = iz;
-            tmx = tmx + tdx;
-          } else {
-            iz = iz + stz_;
-            ZX = ix;
-            ZY = iy;
-            tmz = tmz + tdz;
-          }
-        } else {
-          if (tmy < tmz) {
-            iy = iy + sty_;
-            YX = ix;
-            YZ = iz;
-            tmy = tmy + tdy;
-          } else {
-            iz = iz + stz_;
-            ZX = ix;
-            ZY = iy;
-            tmz = tmz + tdz;
-          }
-        }
-        vi = (ix - 1) * NyNz + (iy - 1) * Nz + (iz - 1);
-      }
+    const float camv[3] = {T[3], T[7], T[11]};
+    if (ray_setup(x, y, dval, cam.Kinv, T, vg, trunc, r)) {
+      r.pix = p;
+      pending =
+          !ray_walk<PREC, 16>(F, r, vg, fix_bounds, camv, flip_hi, kPass1Batches, ov, on, hit);
     }
   }
+  // wave-aggregated append of the unfinished rays
+  const uint64_t mask = __ballot(pending);
+  if (mask) {
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (ln == leader) base = atomicAdd(&st->ray_queue, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pending) queue[base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
+  }
+  if (!active) return;
   out_v[p] = ov;
   out_n[p] = on;
   out_hit[p] = hit;
+}
+
+// Pass 2: finishes the queued rays, grid-stride over the queue.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_raycast_rest(
+    const void* __restrict__ volume, kf_vol vg, int fix_bounds, float flip_hi,
+    const kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
+    int32_t* __restrict__ out_hit, const RayState* __restrict__ queue) {
+  const uint32_t n = st->ray_queue;
+  VolRead<PREC> F;
+  F.v = reinterpret_cast<decltype(F.v)>(volume);
+  const float camv[3] = {st->T[3], st->T[7], st->T[11]};
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    RayState r = queue[i];
+    float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
+    int hit = -1;
+    ray_walk<PREC, 16>(F, r, vg, fix_bounds, camv, flip_hi, 0x7fffffff, ov, on, hit);
+    out_v[r.pix] = ov;
+    out_n[r.pix] = on;
+    out_hit[r.pix] = hit;
+  }
+}
+
+__global__ void k_zero_queue(kf_dev_state* st) {
+  if (threadIdx.x == 0) st->ray_queue = 0u;
 }
 
 }  // namespace
@@ -906,27 +1258,39 @@ hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam) {
   return hipGetLastError();
 }
 
-hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
-  k_icp_begin<<<1, 64, 0, c->stream>>>(c->state);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+template <bool KAHAN, int PX>
+static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
   const int iters = c->prm.icp_iterations;
   const int n = c->H * c->W;
+  const int nb = c->icp_blocks;
+  const size_t rows = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
   for (int it = 0; it < iters; ++it) {
     uint8_t* m = (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
-    const int last = it == iters - 1;
-    if (c->prm.icp_compensated)
-      k_icp_iter<true><<<c->icp_blocks, KF_ICP_BLOCK, 0, c->stream>>>(
-          c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
-          c->icp_cos_lo, c->state, c->icp_partials, m, it, last);
-    else
-      k_icp_iter<false><<<c->icp_blocks, KF_ICP_BLOCK, 0, c->stream>>>(
-          c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
-          c->icp_cos_lo, c->state, c->icp_partials, m, it, last);
-    e = hipGetLastError();
+    const float* pin = c->icp_partials + (size_t)((it + 1) & 1) * rows * nb;
+    float* pout = c->icp_partials + (size_t)(it & 1) * rows * nb;
+    k_icp_iter<KAHAN, PX><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
+        c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
+        c->icp_cos_lo, c->state, pin, pout, m, it);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  if (iters > 0) {
+    const float* pin = c->icp_partials + (size_t)((iters - 1) & 1) * rows * nb;
+    k_icp_finish<KAHAN><<<1, KF_ICP_BLOCK, 0, c->stream>>>(c->state, pin, nb, iters - 1);
+  }
+  return hipGetLastError();
+}
+
+hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
+  const bool kahan = c->prm.icp_compensated != 0;
+  switch (c->icp_px) {
+    case 1:
+      return kahan ? launch_icp_t<true, 1>(c, cam) : launch_icp_t<false, 1>(c, cam);
+    case 2:
+      return kahan ? launch_icp_t<true, 2>(c, cam) : launch_icp_t<false, 2>(c, cam);
+    default:
+      return kahan ? launch_icp_t<true, 4>(c, cam) : launch_icp_t<false, 4>(c, cam);
+  }
 }
 
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam) {
@@ -937,13 +1301,13 @@ hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam) {
 }
 
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
-  const int nv = (int)c->n_voxels;
-  const int blocks = (int)((c->n_voxels + 256 * kIntegVPT - 1) / (256 * kIntegVPT));
+  const int blocks = (c->n_bricks + kBricksPerBlock - 1) / kBricksPerBlock;
   const kf_params& p = c->prm;
-#define KF_INTEG(P, FI)                                                                  \
-  k_integrate<P, FI><<<blocks, 256, 0, c->stream>>>(c->volume, c->integ_tab, cam, c->vg, \
-                                                    c->H, c->W, p.truncation_mm,         \
-                                                    p.sdf_min_mm, p.sdf_max_mm, c->state, nv)
+#define KF_INTEG(P, FI)                                                                    \
+  k_integrate<P, FI><<<blocks, 256, 0, c->stream>>>(c->tsdf, c->weight, c->integ_tab, cam, \
+                                                    c->vg, c->H, c->W, p.truncation_mm,    \
+                                                    p.sdf_min_mm, p.sdf_max_mm, c->state,  \
+                                                    c->n_bricks)
   if (c->vol.precision == KF_TSDF_F32) {
     if (first)
       KF_INTEG(KF_TSDF_F32, true);
@@ -963,20 +1327,31 @@ hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam) {
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
   const kf_params& p = c->prm;
-  if (c->vol.precision == KF_TSDF_F32)
+  RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
+  k_zero_queue<<<1, 64, 0, c->stream>>>(c->state);
+  if (c->vol.precision == KF_TSDF_F32) {
     k_raycast<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(
-        c->volume, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
-        c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, tiles_x, nb);
-  else
+        c->tsdf, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
+        c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
+    k_raycast_rest<KF_TSDF_F32><<<256, 256, 0, c->stream>>>(
+        c->tsdf, c->vg, p.raycast_fix_bounds, c->flip_cos_hi, c->state, c->model_v, c->model_n,
+        c->hit, q);
+  } else {
     k_raycast<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(
-        c->volume, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
-        c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, tiles_x, nb);
+        c->tsdf, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
+        c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
+    k_raycast_rest<KF_TSDF_F16><<<256, 256, 0, c->stream>>>(
+        c->tsdf, c->vg, p.raycast_fix_bounds, c->flip_cos_hi, c->state, c->model_v, c->model_n,
+        c->hit, q);
+  }
   return hipGetLastError();
 }
 
+size_t kf_ray_state_bytes() { return sizeof(RayState); }
+
 hipError_t kf_launch_volume_fill(kf_context* c) {
-  const int64_t nv = c->n_voxels;
-  k_volume_fill<<<(unsigned)((nv + 255) / 256), 256, 0, c->stream>>>(c->volume,
-                                                                      c->vol.precision, nv);
+  const int64_t n = c->n_store;
+  k_volume_fill<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->tsdf, c->weight,
+                                                                     c->vol.precision, n);
   return hipGetLastError();
 }

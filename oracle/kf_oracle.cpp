@@ -815,7 +815,7 @@ void oracle_icp(const float* vx, const float* vy, const float* vz, const float* 
         sums[27] += 1.0;
       }
     }
-    const int st = kf_icp_update(sums, Te, nullptr);
+    const int st = kf_icp_update(sums, Te);
     if (T_iters)
       for (int i = 0; i < 16; ++i) T_iters[it * 16 + i] = Te[i];
     if (sums_out)
@@ -906,5 +906,26 @@ void oracle_f32_to_f16(const float* in, uint16_t* out, int64_t n) {
 }
 void oracle_f16_to_f32(const uint16_t* in, float* out, int64_t n) {
   for (int64_t i = 0; i < n; ++i) out[i] = kf_f16_bits_to_f32(in[i]);
+}
+}  // extern "C"
+
+extern "C" {
+// Per-pixel traversal step counts of the F32 raycast (for the cost model of
+// the raycast and its tests).
+void oracle_raycast_steps_f32(const float* depth, int H, int W, const float* Kf, const float* Tf,
+                              const float* tsdf, const int* dims, const int* start,
+                              float voxel_mm, float trunc, int fix_bounds, int32_t* steps) {
+  float Kif[9];
+  kf_m3_inverse(Kf, Kif);
+  const int64_t n = (int64_t)H * W;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t p = 0; p < n; ++p) {
+    int64_t s = 0;
+    float o[6];
+    if (depth[p] != 0)
+      raycast_pixel<float>((int)(p % W), (int)(p / W), W, depth[p], Kif, Tf, F32Reader{tsdf}, dims,
+                           start, voxel_mm, trunc, fix_bounds, o, &s);
+    steps[p] = (int32_t)s;
+  }
 }
 }  // extern "C"
