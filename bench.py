@@ -1,22 +1,30 @@
 """Full-pipeline throughput on one MI355X.
 
-Workload (the headline configuration): a synthetic 640x480 uint16 depth stream
-(box room, 6 spheres, 4 boxes, Kinect axial noise, 2% holes) into a
-360x360x300 volume at 10 mm (38.88 M voxels), truncation 30 mm.  One step is
-one IncomingFrameProcessing frame: depth gate -> 5x5 bilateral -> vertex and
-normal maps -> 10 point-to-plane ICP iterations -> TSDF integration ->
-raycast, all enqueued on the device with no host round trip.  Frames are
-resident in HBM before the timed region.
+Workload: a synthetic 640x480 uint16 depth stream (box room, 6 spheres,
+4 boxes, Kinect axial noise, 2% holes) into a 360x360x300 volume at 10 mm
+(38.88 M voxels), truncation 30 mm.  One step is one IncomingFrameProcessing
+frame: depth gate -> 5x5 bilateral -> vertex and normal maps -> 10
+point-to-plane ICP iterations -> TSDF integration -> raycast, all enqueued on
+the device with no host round trip.  Frames are resident in HBM before the
+timed region.
 
-Prints one JSON line: frames/s with an FP32 volume (value), the same stream
-with an FP16 TSDF + uint16 weight, trajectory error against ground truth,
-per-stage ms, the integration kernel against the HBM roofline and the CPU
-oracle on the host cores.
+Runs, each on a fresh context over the same frames:
+  1. FP32 volume, timed                         -> value, ms_per_step
+  2. FP16 TSDF + u16 weight, timed              -> fp16
+  3. FP32 with per-stage hipEvents               -> stage_ms, roofline
+  4. FP32 with the raycast work counters on      -> algorithmic bytes
+and the CPU oracle on a bounded sample of the same workload.
+
+Prints one JSON line.  The roofline object is for the raycast stage (the
+largest share of a frame): k_raycast + k_raycast_rest, timed with hipEvents
+on the context stream, against its algorithmic bytes (voxels visited x
+sizeof(TSDF) + hits x 13 taps x sizeof(TSDF) + depth in + 6 maps and the hit
+index out).
 """
 import argparse
 import json
-import os
 import statistics
+import os
 import sys
 import time
 
@@ -27,6 +35,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s full pipeline 640x480 38.9M voxels, fp16 vs fp32 TSDF at ATE<=1mm"
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per frame of the raycast stage (k_raycast + k_raycast_rest), from
+# two rocprofv3 --pmc passes over `python3 bench.py --steps 30 --warmup 5
+# --no-cpu` (FETCH_SIZE x 2 + WRITE_SIZE x 1; the factors are what the same
+# counters read for a device-to-device copy of 256 MiB).  The passes and the
+# per-kernel figures are in profiles/raycast_room_640x480_pmc.txt.
+RAYCAST_TRAFFIC_BYTES = 22361670
 
 
 def ensure_built():
@@ -42,8 +57,9 @@ def ate_mm(poses, gt):
     return float(np.sqrt((err ** 2).mean()))
 
 
-def run_gpu(kf, precision, frames, gt, K, warmup, steps, timing):
-    vol, prm, _, H, W = kf.preset(1)
+def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False):
+    """One pass over frames[0 .. warmup+steps]; times frames warmup+1 ..."""
+    vol, prm, _, H, W = kf.preset(kf.PRESET_ROOM)
     vol.precision = precision
     prm.stage_timing = 1 if timing else 0
     ctx = kf.KinfuContext(vol, prm, H, W)
@@ -54,34 +70,55 @@ def run_gpu(kf, precision, frames, gt, K, warmup, steps, timing):
             ctx.pipeline_incoming(K, f)
         ctx.synchronize()
         ctx.reset_stage_times()
+        if count:
+            ctx.count_raycast_work(True)
         t0 = time.perf_counter()
         for f in range(warmup + 1, warmup + 1 + steps):
             ctx.pipeline_incoming(K, f)
         ctx.synchronize()
         elapsed = time.perf_counter() - t0
         stages = ctx.stage_times()
+        work = ctx.raycast_work() if count else None
         poses = ctx.poses(warmup + 1 + steps)
     finally:
         ctx.close()
-    return elapsed, stages, poses
+    return elapsed, stages, poses, work
 
 
 def cpu_baseline(kf, repeats=3):
     from oracle import kf_oracle as oracle
-    vol, prm, K, H, W = kf.preset(0)
-    scene = kf.preset_scene(0)
-    frames, _ = kf.stream(scene, K, H, W, 0, 10)
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM_CPU)
+    scene = kf.preset_scene(kf.PRESET_ROOM_CPU)
+    n = 40
+    frames, _ = kf.stream(scene, K, H, W, 0, n)
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
         oracle.pipeline(frames, K, vol, prm, reduce_mode=0)
         times.append(time.perf_counter() - t0)
     med = statistics.median(times)
-    return {"value": round(10 / med, 3), "unit": "frames/s", "cores": oracle.max_threads(),
+    return {"value": round(n / med, 3), "unit": "frames/s", "cores": oracle.max_threads(),
             "kind": "port",
-            "sample": f"CPU oracle (C++/OpenMP restatement, -O2, FP32), config 0: 10 frames "
-                      f"640x480 into 360x360x300 @10mm, full frame sequence; median of "
-                      f"{repeats} runs, {med:.2f} s each"}
+            "sample": f"CPU oracle (C++/OpenMP restatement of the reference kernels, g++ -O2, "
+                      f"FP32): the same stream's first {n} frames, 640x480 into 360x360x300 "
+                      f"@10mm, full frame sequence incl. initial frame; median of {repeats} "
+                      f"runs of {med:.2f} s"}
+
+
+def copy_calibration():
+    """Three 256 MiB device-to-device copies: the known traffic against which
+    rocprofv3's FETCH_SIZE and WRITE_SIZE counters are scaled."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
 
 
 def main():
@@ -90,42 +127,71 @@ def main():
     ap.add_argument("--steps", type=int, default=280)
     ap.add_argument("--warmup", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the known-size copies used to scale the HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding of the "
-                                   "volume is described in DESIGN.md but not implemented",
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus above 1 is not "
+                                   "supported (the volume is not sharded across GPUs)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     ensure_built()
     import kinect_fusion_slam_amd as kf
 
-    vol, prm, K, H, W = kf.preset(1)
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
     n_frames = 1 + args.warmup + args.steps
     t0 = time.perf_counter()
-    frames, gt = kf.stream(kf.preset_scene(1), K, H, W, 0, n_frames, threads=16)
-    gen_s = time.perf_counter() - t0
-    print(f"[bench] generated {n_frames} frames in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    frames, gt = kf.stream(kf.preset_scene(kf.PRESET_ROOM), K, H, W, 0, n_frames, threads=16)
+    print(f"[bench] generated {n_frames} frames in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
 
-    el32, st32, p32 = run_gpu(kf, kf.KF_TSDF_F32, frames, gt, K, args.warmup, args.steps, True)
+    el32, _, p32, _ = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps)
     print(f"[bench] fp32 {args.steps / el32:.1f} frames/s", file=sys.stderr, flush=True)
-    el16, st16, p16 = run_gpu(kf, kf.KF_TSDF_F16, frames, gt, K, args.warmup, args.steps, True)
+    el16, _, p16, _ = run_gpu(kf, kf.KF_TSDF_F16, frames, K, args.warmup, args.steps)
     print(f"[bench] fp16 {args.steps / el16:.1f} frames/s", file=sys.stderr, flush=True)
+    _, st32, _, _ = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps, timing=True)
+    _, st16, _, _ = run_gpu(kf, kf.KF_TSDF_F16, frames, K, args.warmup, args.steps, timing=True)
+    _, _, _, work = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps, count=True)
+    print("[bench] stage timing and work counters done", file=sys.stderr, flush=True)
 
     def per_stage(st):
         return {k: round(ms / max(cnt, 1), 4) for k, (ms, cnt) in st.items() if cnt}
 
-    nvox = vol.dims[0] * vol.dims[1] * vol.dims[2]
-    integ_ms, integ_n = st32["integrate"]
-    integ_avg_s = integ_ms / max(integ_n, 1) / 1e3
-    alg_bytes = nvox * 16  # dense read-modify-write of {float tsdf, float weight}
-    achieved = alg_bytes / integ_avg_s / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_integrate<F32>", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "algorithmic_bytes": alg_bytes,
-                "avg_launch_us": round(integ_avg_s * 1e6, 2)}
-    i16_ms, i16_n = st16["integrate"]
-    fp16_integ_s = i16_ms / max(i16_n, 1) / 1e3
+    def avg_s(st, stage):
+        ms, cnt = st[stage]
+        return ms / max(cnt, 1) / 1e3
 
+    npx = H * W
+    nvox = int(vol.dims[0]) * int(vol.dims[1]) * int(vol.dims[2])
+    visited, hits = work[0] / args.steps, work[1] / args.steps
+    ray_bytes = visited * 4 + hits * 13 * 4 + 8 * npx * 4
+    ray_s = avg_s(st32, "raycast")
+    achieved = ray_bytes / ray_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": RAYCAST_TRAFFIC_BYTES,
+                "kernel": "raycast stage: k_raycast<F32> + k_raycast_rest<F32>",
+                "algorithmic_bytes": int(ray_bytes), "avg_launch_us": round(ray_s * 1e6, 2),
+                "voxels_visited_per_frame": int(visited), "hits_per_frame": int(hits)}
+
+    # the other stages against the same peak (algorithmic bytes per launch)
+    icp_iters = prm.icp_iterations
+    other = {
+        "integrate_dense_model": {
+            "bytes": nvox * 16, "us": round(avg_s(st32, "integrate") * 1e6, 2),
+            "note": "dense read-modify-write of every voxel (16 B); the kernel culls bricks "
+                    "outside the view frustum, so it can run above this figure"},
+        "icp": {"bytes": icp_iters * 13 * npx * 4, "us": round(avg_s(st32, "icp") * 1e6, 2)},
+        "vertex": {"bytes": 6 * npx * 4 + npx, "us": round(avg_s(st32, "vertex") * 1e6, 2)},
+        "bilateral": {"bytes": 2 * npx * 4, "us": round(avg_s(st32, "bilateral") * 1e6, 2)},
+    }
+    for v in other.values():
+        v["GBs"] = round(v["bytes"] / (v["us"] * 1e-6) / 1e9, 1) if v["us"] else None
+
+    a32, a16 = ate_mm(p32, gt), ate_mm(p16, gt)
     out = {
         "metric": METRIC,
         "value": round(args.steps / el32, 2),
@@ -139,19 +205,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "config1: 640x480 synthetic stream, 360x360x300 voxels @10mm "
-                               "(38.88M) FP32 TSDF+weight, trunc 30mm, bilateral 5x5, "
+        "config": {"workload": "640x480 synthetic room stream into 360x360x300 voxels @10mm "
+                               "(38.88M), FP32 TSDF+weight, trunc 30mm, bilateral 5x5, "
                                "10 ICP iterations, raycast per frame",
                    "frames_timed": args.steps, "image": [H, W], "volume": list(vol.dims),
                    "parallelism": "single GPU"},
         "fp16": {"value": round(args.steps / el16, 2), "unit": "frames/s",
-                 "ratio_vs_fp32": round(el32 / el16, 3),
-                 "integrate_achieved_GBs": round(nvox * 8 / fp16_integ_s / 1e9, 1),
-                 "stage_ms": per_stage(st16)},
-        "ate_mm": {"fp32": round(ate_mm(p32, gt), 3), "fp16": round(ate_mm(p16, gt), 3),
-                   "fp16_minus_fp32": round(ate_mm(p16, gt) - ate_mm(p32, gt), 3)},
+                 "ratio_vs_fp32": round(el32 / el16, 3), "stage_ms": per_stage(st16)},
+        "ate_mm": {"fp32": round(a32, 3), "fp16": round(a16, 3),
+                   "fp16_minus_fp32": round(a16 - a32, 3)},
         "stage_ms": per_stage(st32),
         "roofline": roofline,
+        "stage_roofline": other,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(kf)

@@ -70,11 +70,11 @@ extern "C" {
 
 /* Parameter presets for kf_preset(). */
 #define KF_PRESET_REFERENCE 100 /* the reference's compiled-in defaults          */
-#define KF_PRESET_CONFIG0 0     /* 640x480, 360x360x300 @10mm, FP32 (CPU case)    */
-#define KF_PRESET_CONFIG1 1     /* same geometry, GPU stream                      */
-#define KF_PRESET_CONFIG2 2     /* 720x720x600 @5mm, FP16+u16                     */
-#define KF_PRESET_CONFIG3 3     /* 128^3 @10mm, 320x240 differentiable render     */
-#define KF_PRESET_CONFIG4 4     /* 1024x1024x768 @4mm, FP16+u16, 1280x720         */
+#define KF_PRESET_ROOM_CPU 0     /* 640x480, 360x360x300 @10mm, FP32 (CPU case)    */
+#define KF_PRESET_ROOM 1     /* same geometry, GPU stream                      */
+#define KF_PRESET_ROOM_5MM_F16 2     /* 720x720x600 @5mm, FP16+u16                     */
+#define KF_PRESET_SPHERES_128 3     /* 128^3 @10mm, 320x240 differentiable render     */
+#define KF_PRESET_ROOM_HD_F16 4     /* 1024x1024x768 @4mm, FP16+u16, 1280x720         */
 
 typedef struct kf_context kf_context;
 
@@ -226,6 +226,13 @@ int kf_download_poses(kf_context* ctx, float* poses, int32_t n_frames);
  * recorded on the context stream while stage_timing is set. */
 int kf_stage_times(kf_context* ctx, double* ms, int64_t* counts);
 int kf_reset_stage_times(kf_context* ctx);
+/* Raycast work counters (no reference counterpart; a measurement aid).  While
+ * enabled, every raycast adds the voxels its walks visit (iterations of the
+ * loop at voxel_traversal.cu:203-421) and its hit count into two device
+ * counters; enabling or disabling zeroes them.  Counting uses separate kernel
+ * instantiations, so it is left off when timing. */
+int kf_count_raycast_work(kf_context* ctx, int32_t enable);
+int kf_raycast_work(kf_context* ctx, int64_t* voxels_visited, int64_t* hits);
 
 /* ---- conventions exposed for tests ---------------------------------------
  * The angle gates are evaluated as cosine thresholds that are exactly
@@ -273,8 +280,10 @@ int kf_synth_stream(const kf_synth_scene* scene, const float K[9], int32_t heigh
                     int32_t width, int32_t first, int32_t n, float noise_scale,
                     float hole_frac, float max_depth_mm, uint16_t* frames,
                     float* gt_poses, int32_t n_threads);
-/* The scene each preset uses (6 spheres + 4 boxes for configs 0-2, 12 + 8
- * for config 4; seed 0; room inset from the volume by truncation + 2 voxels). */
+/* The scene each preset uses (6 spheres + 4 boxes for the 640x480 room
+ * presets, 12 + 8 for KF_PRESET_ROOM_HD_F16, 3 spheres for
+ * KF_PRESET_SPHERES_128; seed 0; room inset from the volume by truncation +
+ * 2 voxels). */
 int kf_synth_preset_scene(int32_t which, kf_synth_scene* out);
 
 #ifdef __cplusplus

@@ -19,7 +19,13 @@ KF_MAX_ICP_ITERS = 64
 KF_ICP_SUMS = 28
 STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep")
 PRESET_REFERENCE = 100
-PRESET_CONFIG = (0, 1, 2, 3, 4)
+PRESET_ROOM_CPU = 0       # 640x480, 360x360x300 @10mm FP32, 10-frame CPU case
+PRESET_ROOM = 1           # same geometry, the GPU stream
+PRESET_ROOM_5MM_F16 = 2   # 720x720x600 @5mm, FP16 TSDF + u16 weight
+PRESET_SPHERES_128 = 3    # 128^3 @10mm, 320x240 differentiable render
+PRESET_ROOM_HD_F16 = 4    # 1024x1024x768 @4mm FP16, 1280x720
+PRESETS = (PRESET_ROOM_CPU, PRESET_ROOM, PRESET_ROOM_5MM_F16, PRESET_SPHERES_128,
+           PRESET_ROOM_HD_F16)
 
 
 class KinfuError(RuntimeError):
@@ -96,6 +102,8 @@ _SIGNATURES = {
     "kf_download_poses": (C.c_int, [_P, _F, C.c_int32]),
     "kf_stage_times": (C.c_int, [_P, _D, _I64]),
     "kf_reset_stage_times": (C.c_int, [_P]),
+    "kf_count_raycast_work": (C.c_int, [_P, C.c_int32]),
+    "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
     "kf_mat4_inverse": (C.c_int, [_F, _F]),
@@ -428,3 +436,11 @@ class KinfuContext:
 
     def reset_stage_times(self):
         check(lib.kf_reset_stage_times(self.h))
+
+    def count_raycast_work(self, enable=True):
+        check(lib.kf_count_raycast_work(self.h, 1 if enable else 0))
+
+    def raycast_work(self):
+        v, h = C.c_int64(), C.c_int64()
+        check(lib.kf_raycast_work(self.h, C.byref(v), C.byref(h)))
+        return v.value, h.value

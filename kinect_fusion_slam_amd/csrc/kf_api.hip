@@ -231,7 +231,7 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   kf_params p{};
   float k[9] = {525.0f, 0.0f, 319.5f, 0.0f, 525.0f, 239.5f, 0.0f, 0.0f, 1.0f};
   int H = 480, W = 640;
-  // settings shared by the synthetic configs.  The SDF band keeps the
+  // settings shared by the synthetic presets.  The SDF band keeps the
   // reference's shape [-trunc, 10 trunc] (kinfu_pipeline.h:52-55 has trunc
   // 300, band [-30, 300]): the traversal's interpolation taps reach back to
   // voxels several steps in front of the surface, which must be observed.
@@ -268,17 +268,17 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
       p.icp_angle_deg = 30.0f;
       break;
     }
-    case KF_PRESET_CONFIG0:
-    case KF_PRESET_CONFIG1:
+    case KF_PRESET_ROOM_CPU:
+    case KF_PRESET_ROOM:
       break;
-    case KF_PRESET_CONFIG2:
+    case KF_PRESET_ROOM_5MM_F16:
       v = {{720, 720, 600}, {-400, -400, -240}, 5.0f, KF_TSDF_F16};
       p.truncation_mm = 15.0f;
       p.sdf_min_mm = -15.0f;
       p.sdf_max_mm = 150.0f;
       p.icp_compensated = 1;
       break;
-    case KF_PRESET_CONFIG3: {
+    case KF_PRESET_SPHERES_128: {
       v = {{128, 128, 128}, {-64, -64, 36}, 10.0f, KF_TSDF_F32};
       const float k3[9] = {262.5f, 0.0f, 159.5f, 0.0f, 262.5f, 119.5f, 0.0f, 0.0f, 1.0f};
       memcpy(k, k3, sizeof k);
@@ -287,7 +287,7 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
       p.bilateral = 0;
       break;
     }
-    case KF_PRESET_CONFIG4: {
+    case KF_PRESET_ROOM_HD_F16: {
       v = {{1024, 1024, 768}, {-512, -512, -300}, 4.0f, KF_TSDF_F16};
       const float k4[9] = {1000.0f, 0.0f, 639.5f, 0.0f, 1000.0f, 359.5f, 0.0f, 0.0f, 1.0f};
       memcpy(k, k4, sizeof k);
@@ -974,6 +974,24 @@ int kf_reset_stage_times(kf_context* c) {
     c->stage_ms[i] = 0.0;
     c->stage_count[i] = 0;
   }
+  return KF_OK;
+}
+
+int kf_count_raycast_work(kf_context* c, int32_t enable) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  KF_HIP_CHECK(hipMemsetAsync(&c->state->ray_steps, 0, 2 * sizeof(unsigned long long), c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->count_rays = enable != 0;
+  return KF_OK;
+}
+
+int kf_raycast_work(kf_context* c, int64_t* voxels_visited, int64_t* hits) {
+  KF_REQUIRE(c && voxels_visited && hits, KF_ERR_INVALID, "null argument");
+  unsigned long long v[2];
+  KF_HIP_CHECK(hipMemcpyAsync(v, &c->state->ray_steps, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *voxels_visited = (int64_t)v[0];
+  *hits = (int64_t)v[1];
   return KF_OK;
 }
 

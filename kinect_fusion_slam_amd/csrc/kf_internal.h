@@ -22,6 +22,8 @@ struct kf_dev_state {
   int32_t icp_iters_done;
   uint32_t ray_queue;   // rays queued by the first raycast pass
   int32_t pad[2];
+  unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
+  unsigned long long ray_hits;
   float iter_T[KF_MAX_ICP_ITERS][16];
   double iter_sums[KF_MAX_ICP_ITERS][KF_ICP_SUMS];
   int32_t iter_status[KF_MAX_ICP_ITERS];
@@ -115,6 +117,7 @@ struct kf_context {
   double stage_ms[KF_STAGE_COUNT] = {};
   int64_t stage_count[KF_STAGE_COUNT] = {};
   bool have_depth = false;
+  bool count_rays = false;  // raycast work counters on
 };
 
 // error reporting ---------------------------------------------------------

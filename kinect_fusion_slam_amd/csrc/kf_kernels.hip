@@ -1067,10 +1067,11 @@ __device__ __forceinline__ bool ray_setup(int x, int y, float dval, const float*
 // (:207-222) in order: the same voxels are visited and the same decisions
 // taken.  Returns true when the ray is finished (hit or not); on false the
 // state is ready to continue.
-template <int PREC, int kRayBatch>
+template <int PREC, int kRayBatch, bool COUNT>
 __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, const kf_vol& vg,
                                          int fix_bounds, const float* cam, float flip_hi,
-                                         int max_batches, float4& ov, float4& on, int& hit) {
+                                         int max_batches, float4& ov, float4& on, int& hit,
+                                         int& visited) {
   const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
   for (int bt = 0; bt < max_batches; ++bt) {
     // generate the next kRayBatch voxels of the walk (branch-free: the walk
@@ -1100,6 +1101,7 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
     for (int k = 0; k < kRayBatch; ++k) {
       const float f = fs[k];
       const bool valid = k < nv && !done;
+      if constexpr (COUNT) visited += valid ? 1 : 0;
       const bool brk = (f != f && prev < 0.f) || (f < 0.f && prev != prev) || (f > 0.f && prev < 0.f);
       const bool cross = !brk && f < 0.f && prev > 0.f;
       kc = (valid && cross) ? k : kc;
@@ -1132,7 +1134,19 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
 // of the image and its L2 sees the band's voxels.
 constexpr int kPass1Batches = 2;  // of 16 steps
 
-template <int PREC>
+// Optional work counters (COUNT): voxels visited by the walks and rays that
+// hit, summed into the device state for the algorithmic-traffic figure.
+__device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool hit) {
+  uint64_t v = (uint64_t)visited;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const uint64_t h = __popcll(__ballot(hit));
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&st->ray_steps, (unsigned long long)v);
+    atomicAdd(&st->ray_hits, (unsigned long long)h);
+  }
+}
+
+template <int PREC, bool COUNT>
 __global__ __launch_bounds__(256) void k_raycast(
     const void* __restrict__ volume, const float* __restrict__ depth, kf_cam cam, kf_vol vg,
     int H, int W, float trunc, int fix_bounds, float flip_hi,
@@ -1158,6 +1172,7 @@ __global__ __launch_bounds__(256) void k_raycast(
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
   int hit = -1;
   bool pending = false;
+  int visited = 0;
   RayState r;
   const float dval = active ? depth[p] : 0.0f;
   if (dval != 0) {
@@ -1166,10 +1181,11 @@ __global__ __launch_bounds__(256) void k_raycast(
     const float camv[3] = {T[3], T[7], T[11]};
     if (ray_setup(x, y, dval, cam.Kinv, T, vg, trunc, r)) {
       r.pix = p;
-      pending =
-          !ray_walk<PREC, 16>(F, r, vg, fix_bounds, camv, flip_hi, kPass1Batches, ov, on, hit);
+      pending = !ray_walk<PREC, 16, COUNT>(F, r, vg, fix_bounds, camv, flip_hi, kPass1Batches,
+                                           ov, on, hit, visited);
     }
   }
+  if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   // wave-aggregated append of the unfinished rays
   const uint64_t mask = __ballot(pending);
   if (mask) {
@@ -1186,23 +1202,32 @@ __global__ __launch_bounds__(256) void k_raycast(
 }
 
 // Pass 2: finishes the queued rays, grid-stride over the queue.
-template <int PREC>
+template <int PREC, bool COUNT>
 __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, kf_vol vg, int fix_bounds, float flip_hi,
-    const kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
+    kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
     int32_t* __restrict__ out_hit, const RayState* __restrict__ queue) {
   const uint32_t n = st->ray_queue;
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
   const float camv[3] = {st->T[3], st->T[7], st->T[11]};
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    RayState r = queue[i];
+  const uint32_t stride = gridDim.x * 256;
+  // every lane runs the same number of trips so that the counters' wave
+  // reductions see the whole wave
+  const uint32_t trips = COUNT ? (n + stride - 1) / stride : 0;
+  uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  for (uint32_t t = 0; COUNT ? t < trips : i < n; ++t, i += stride) {
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
-    int hit = -1;
-    ray_walk<PREC, 16>(F, r, vg, fix_bounds, camv, flip_hi, 0x7fffffff, ov, on, hit);
-    out_v[r.pix] = ov;
-    out_n[r.pix] = on;
-    out_hit[r.pix] = hit;
+    int hit = -1, visited = 0;
+    if (i < n) {
+      RayState r = queue[i];
+      ray_walk<PREC, 16, COUNT>(F, r, vg, fix_bounds, camv, flip_hi, 0x7fffffff, ov, on, hit,
+                                visited);
+      out_v[r.pix] = ov;
+      out_n[r.pix] = on;
+      out_hit[r.pix] = hit;
+    }
+    if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   }
 }
 
@@ -1323,26 +1348,33 @@ hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
   return hipGetLastError();
 }
 
+template <int PREC, bool COUNT>
+static void launch_raycast_t(kf_context* c, const kf_cam& cam, int tiles_x, int nb) {
+  const kf_params& p = c->prm;
+  RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
+  k_raycast<PREC, COUNT><<<nb, 256, 0, c->stream>>>(
+      c->tsdf, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
+      c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
+  k_raycast_rest<PREC, COUNT><<<256, 256, 0, c->stream>>>(
+      c->tsdf, c->vg, p.raycast_fix_bounds, c->flip_cos_hi, c->state, c->model_v, c->model_n,
+      c->hit, q);
+}
+
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam) {
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
-  const kf_params& p = c->prm;
-  RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
   k_zero_queue<<<1, 64, 0, c->stream>>>(c->state);
+  const bool count = c->count_rays;
   if (c->vol.precision == KF_TSDF_F32) {
-    k_raycast<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
-        c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
-    k_raycast_rest<KF_TSDF_F32><<<256, 256, 0, c->stream>>>(
-        c->tsdf, c->vg, p.raycast_fix_bounds, c->flip_cos_hi, c->state, c->model_v, c->model_n,
-        c->hit, q);
+    if (count)
+      launch_raycast_t<KF_TSDF_F32, true>(c, cam, tiles_x, nb);
+    else
+      launch_raycast_t<KF_TSDF_F32, false>(c, cam, tiles_x, nb);
   } else {
-    k_raycast<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
-        c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
-    k_raycast_rest<KF_TSDF_F16><<<256, 256, 0, c->stream>>>(
-        c->tsdf, c->vg, p.raycast_fix_bounds, c->flip_cos_hi, c->state, c->model_v, c->model_n,
-        c->hit, q);
+    if (count)
+      launch_raycast_t<KF_TSDF_F16, true>(c, cam, tiles_x, nb);
+    else
+      launch_raycast_t<KF_TSDF_F16, false>(c, cam, tiles_x, nb);
   }
   return hipGetLastError();
 }

@@ -155,10 +155,10 @@ int kf_synth_build_scene(int32_t n_spheres, int32_t n_boxes, uint64_t seed,
 
 int kf_synth_preset_scene(int32_t which, kf_synth_scene* out) {
   int ns = 6, nb = 4;
-  if (which == KF_PRESET_CONFIG4) ns = 12, nb = 8;
-  else if (which == KF_PRESET_CONFIG3) ns = 3, nb = 0;
-  else if (which != KF_PRESET_CONFIG0 && which != KF_PRESET_CONFIG1 &&
-           which != KF_PRESET_CONFIG2 && which != KF_PRESET_REFERENCE)
+  if (which == KF_PRESET_ROOM_HD_F16) ns = 12, nb = 8;
+  else if (which == KF_PRESET_SPHERES_128) ns = 3, nb = 0;
+  else if (which != KF_PRESET_ROOM_CPU && which != KF_PRESET_ROOM &&
+           which != KF_PRESET_ROOM_5MM_F16 && which != KF_PRESET_REFERENCE)
     return KF_ERR_INVALID;
   return kf_synth_build_scene(ns, nb, 0, nullptr, nullptr, out);
 }

@@ -31,17 +31,17 @@ def say(msg):
 
 
 @pytest.fixture(scope="session")
-def config0_stream(kf):
-    """Config 0/1 geometry: first 10 frames of the synthetic orbit."""
-    vol, prm, K, H, W = kf.preset(1)
-    scene = kf.preset_scene(1)
+def room_stream(kf):
+    """640x480 room, 360x360x300 @10mm: first 10 frames of the synthetic orbit."""
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
+    scene = kf.preset_scene(kf.PRESET_ROOM)
     frames, gt = kf.stream(scene, K, H, W, 0, 10)
     return vol, prm, K, H, W, frames, gt
 
 
 def small_setup(kf, H=48, W=64, dims=(40, 40, 36), start=(-20, -20, 4), voxel=20.0):
     """A small volume/camera for fast exhaustive parity tests."""
-    vol, prm, K, _, _ = kf.preset(1)
+    vol, prm, K, _, _ = kf.preset(kf.PRESET_ROOM)
     vol.dims[:] = list(dims)
     vol.start[:] = list(start)
     vol.voxel_mm = voxel

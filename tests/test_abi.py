@@ -42,20 +42,20 @@ def test_reference_preset_matches_reference_constants(kf):
     assert (prm.sigma_s_px, prm.sigma_r_mm) == (4.5, 30.0)
 
 
-def test_config_presets(kf):
-    vol, prm, K, H, W = kf.preset(1)
+def test_workload_presets(kf):
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
     assert np.prod(list(vol.dims)) == 38_880_000 and (H, W) == (480, 640)
     assert K[0] == K[4] == 525.0 and K[2] == 319.5 and K[5] == 239.5
     assert prm.truncation_mm == 30.0 and prm.bilateral_radius == 2
     assert (prm.icp_dist_mm, prm.icp_angle_deg) == (100.0, 20.0)
-    vol2, prm2, *_ = kf.preset(2)
+    vol2, prm2, *_ = kf.preset(kf.PRESET_ROOM_5MM_F16)
     assert np.prod(list(vol2.dims)) == 311_040_000 and vol2.precision == kf.KF_TSDF_F16
-    vol4, prm4, K4, H4, W4 = kf.preset(4)
+    vol4, prm4, K4, H4, W4 = kf.preset(kf.PRESET_ROOM_HD_F16)
     assert np.prod(list(vol4.dims)) == 1024 * 1024 * 768 and (H4, W4) == (720, 1280)
 
 
 def test_create_rejects_bad_arguments_without_a_gpu(kf):
-    vol, prm, K, H, W = kf.preset(1)
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
     h = ctypes.c_void_p()
     vol.dims[0] = 1
     rc = kf.lib.kf_create(ctypes.byref(vol), H, W, ctypes.byref(prm), ctypes.byref(h))

@@ -47,46 +47,46 @@ def gated(oracle, frame, prm):
 
 
 @pytest.fixture(scope="module")
-def ctx1(kf):
-    say("[gpu] create config-1 context (38.88M voxels, F32)")
-    c = kf.KinfuContext.from_preset(1, record_icp=1)
+def ctx_room(kf):
+    say("[gpu] create 640x480 room context (38.88M voxels, F32)")
+    c = kf.KinfuContext.from_preset(kf.PRESET_ROOM, record_icp=1)
     yield c
     c.close()
 
 
-def test_depth_gate_bitexact(ctx1, config0_stream, oracle):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_depth_gate_bitexact(ctx_room, room_stream, oracle):
+    vol, prm, K, H, W, frames, gt = room_stream
     for f in (0, 5):
         say(f"[gpu] depth gate frame {f}")
-        ctx1.upload_depth_u16(frames[f])
-        assert_same(ctx1.depth(), oracle.range_gate(frames[f], prm.depth_min_mm,
+        ctx_room.upload_depth_u16(frames[f])
+        assert_same(ctx_room.depth(), oracle.range_gate(frames[f], prm.depth_min_mm,
                                                     prm.depth_max_mm), "depth")
 
 
 @pytest.mark.parametrize("radius", [1, 2, 3, 5])
-def test_bilateral_bitexact(ctx1, config0_stream, oracle, radius):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_bilateral_bitexact(ctx_room, room_stream, oracle, radius):
+    vol, prm, K, H, W, frames, gt = room_stream
     say(f"[gpu] bilateral r={radius}")
-    ctx1.set_params(bilateral_radius=radius)
+    ctx_room.set_params(bilateral_radius=radius)
     try:
-        ctx1.upload_depth_u16(frames[2])
-        ctx1.bilateral()
-        got = ctx1.depth()
+        ctx_room.upload_depth_u16(frames[2])
+        ctx_room.bilateral()
+        got = ctx_room.depth()
     finally:
-        ctx1.set_params(bilateral_radius=prm.bilateral_radius)
+        ctx_room.set_params(bilateral_radius=prm.bilateral_radius)
     d = oracle.range_gate(frames[2], prm.depth_min_mm, prm.depth_max_mm)
     want = oracle.bilateral(d, radius, prm.sigma_s_px, prm.sigma_r_mm)
     assert_same(got, want, "bilateral")
     assert np.all(got[:radius] == 0) and np.all(got[:, :radius] == 0)
 
 
-def test_vertex_normals_bitexact(kf, ctx1, config0_stream, oracle):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_vertex_normals_bitexact(kf, ctx_room, room_stream, oracle):
+    vol, prm, K, H, W, frames, gt = room_stream
     say("[gpu] vertex/normal maps")
     d = gated(oracle, frames[4], prm)
-    ctx1.upload_depth_f32(d)
-    ctx1.vertex_normals(K)
-    got = ctx1.vertex_maps()
+    ctx_room.upload_depth_f32(d)
+    ctx_room.vertex_normals(K)
+    got = ctx_room.vertex_maps()
     want = oracle.vertex_normals(d, kf.mat3_inverse(K).ravel())
     for k in ("vx", "vy", "nx", "ny", "nz", "mask"):
         assert_same(got[k], want[k], k)
@@ -104,11 +104,11 @@ def _integrate_both(kf, ctx, oracle, vol, prm, K, depths, poses, kind):
     return t, w
 
 
-def test_integrate_f32_bitexact(kf, ctx1, config0_stream, oracle):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_integrate_f32_bitexact(kf, ctx_room, room_stream, oracle):
+    vol, prm, K, H, W, frames, gt = room_stream
     depths = [gated(oracle, frames[i], prm) for i in (0, 3, 6)]
-    t, w = _integrate_both(kf, ctx1, oracle, vol, prm, K, depths, gt[[0, 3, 6]], "f32")
-    gt_t, gt_w = ctx1.volume()
+    t, w = _integrate_both(kf, ctx_room, oracle, vol, prm, K, depths, gt[[0, 3, 6]], "f32")
+    gt_t, gt_w = ctx_room.volume()
     assert_same(gt_t, t, "tsdf")
     assert_same(gt_w, w, "weight")
     obs = np.isfinite(w)
@@ -117,10 +117,10 @@ def test_integrate_f32_bitexact(kf, ctx1, config0_stream, oracle):
     assert np.all(np.isnan(t[~obs]))
 
 
-def test_integrate_f16_bitexact(kf, config0_stream, oracle):
-    vol, prm, K, H, W, frames, gt = config0_stream
-    say("[gpu] create config-1 context in F16")
-    ctx = kf.KinfuContext.from_preset(1, precision=kf.KF_TSDF_F16)
+def test_integrate_f16_bitexact(kf, room_stream, oracle):
+    vol, prm, K, H, W, frames, gt = room_stream
+    say("[gpu] create 640x480 room context in F16")
+    ctx = kf.KinfuContext.from_preset(kf.PRESET_ROOM, precision=kf.KF_TSDF_F16)
     try:
         vol16 = ctx.vol
         depths = [gated(oracle, frames[i], prm) for i in (0, 2, 4)]
@@ -135,9 +135,9 @@ def test_integrate_f16_bitexact(kf, config0_stream, oracle):
 
 
 @pytest.fixture(scope="module")
-def model_volume(kf, config0_stream, oracle):
+def model_volume(kf, room_stream, oracle):
     """Oracle F32 volume after frames 0 and 1 at ground-truth poses."""
-    vol, prm, K, H, W, frames, gt = config0_stream
+    vol, prm, K, H, W, frames, gt = room_stream
     t, w = oracle.new_volume(vol.dims, "f32")
     for i in (0, 1):
         oracle.integrate(gated(oracle, frames[i], prm), K, gt[i], vol, prm, i == 0, t, w)
@@ -145,19 +145,19 @@ def model_volume(kf, config0_stream, oracle):
 
 
 @pytest.mark.parametrize("fix", [0, 1])
-def test_raycast_f32_bitexact(kf, ctx1, config0_stream, oracle, model_volume, fix):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_raycast_f32_bitexact(kf, ctx_room, room_stream, oracle, model_volume, fix):
+    vol, prm, K, H, W, frames, gt = room_stream
     t, w = model_volume
     say(f"[gpu] raycast F32 fix_bounds={fix}")
-    ctx1.set_params(raycast_fix_bounds=fix)
+    ctx_room.set_params(raycast_fix_bounds=fix)
     try:
-        ctx1.upload_volume(t, w)
+        ctx_room.upload_volume(t, w)
         d = gated(oracle, frames[1], prm)
-        ctx1.upload_depth_f32(d)
-        ctx1.raycast(K, gt[1])
-        got = ctx1.model_maps()
+        ctx_room.upload_depth_f32(d)
+        ctx_room.raycast(K, gt[1])
+        got = ctx_room.model_maps()
     finally:
-        ctx1.set_params(raycast_fix_bounds=0)
+        ctx_room.set_params(raycast_fix_bounds=0)
     want = oracle.raycast(d, K, gt[1], t, vol, prm.truncation_mm, fix)
     assert_same(got["hit"], want["hit"], "hit index")
     for k in ("x", "y", "z", "nx", "ny", "nz"):
@@ -165,10 +165,33 @@ def test_raycast_f32_bitexact(kf, ctx1, config0_stream, oracle, model_volume, fi
     assert (got["hit"] >= 0).mean() > 0.5
 
 
-def test_raycast_f16_bitexact(kf, config0_stream, oracle):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_raycast_work_counters_match_oracle(kf, ctx_room, room_stream, oracle, model_volume):
+    """The counting kernels visit exactly the oracle's voxels and hits, and
+    produce the same maps as the plain ones."""
+    vol, prm, K, H, W, frames, gt = room_stream
+    t, w = model_volume
+    say("[gpu] raycast work counters")
+    ctx_room.upload_volume(t, w)
+    d = gated(oracle, frames[1], prm)
+    ctx_room.upload_depth_f32(d)
+    ctx_room.count_raycast_work(True)
+    try:
+        ctx_room.raycast(K, gt[1])
+        ctx_room.raycast(K, gt[1])
+        visited, hits = ctx_room.raycast_work()
+        got = ctx_room.model_maps()
+    finally:
+        ctx_room.count_raycast_work(False)
+    want = oracle.raycast(d, K, gt[1], t, vol, prm.truncation_mm, 0)
+    assert (visited, hits) == (2 * want["steps"], 2 * want["hits"])
+    assert_same(got["hit"], want["hit"], "hit index")
+    assert_same(got["z"], want["z"], "z")
+
+
+def test_raycast_f16_bitexact(kf, room_stream, oracle):
+    vol, prm, K, H, W, frames, gt = room_stream
     say("[gpu] raycast F16")
-    ctx = kf.KinfuContext.from_preset(1, precision=kf.KF_TSDF_F16)
+    ctx = kf.KinfuContext.from_preset(kf.PRESET_ROOM, precision=kf.KF_TSDF_F16)
     try:
         t, w = oracle.new_volume(vol.dims, "f16")
         for i in (0, 1):
@@ -187,9 +210,9 @@ def test_raycast_f16_bitexact(kf, config0_stream, oracle):
 
 
 @pytest.mark.parametrize("compensated", [0, 1])
-def test_icp_masks_bitexact_and_pose(kf, ctx1, config0_stream, oracle, model_volume,
+def test_icp_masks_bitexact_and_pose(kf, ctx_room, room_stream, oracle, model_volume,
                                      compensated):
-    vol, prm, K, H, W, frames, gt = config0_stream
+    vol, prm, K, H, W, frames, gt = room_stream
     t, w = model_volume
     d1 = gated(oracle, frames[1], prm)
     model = oracle.raycast(d1, K, gt[1], t, vol, prm.truncation_mm, 0)
@@ -197,14 +220,14 @@ def test_icp_masks_bitexact_and_pose(kf, ctx1, config0_stream, oracle, model_vol
     fr = oracle.vertex_normals(d2, kf.mat3_inverse(K).ravel())
     fr["vz"] = d2
     say(f"[gpu] ICP compensated={compensated}")
-    ctx1.set_params(icp_compensated=compensated)
+    ctx_room.set_params(icp_compensated=compensated)
     try:
-        ctx1.upload_vertex_maps(fr["vx"], fr["vy"], d2, fr["nx"], fr["ny"], fr["nz"], fr["mask"])
-        ctx1.upload_model_maps(*(model[k] for k in ("x", "y", "z", "nx", "ny", "nz")))
-        T = ctx1.estimate_pose(K, gt[1])
-        hist = ctx1.icp_history(masks=True)
+        ctx_room.upload_vertex_maps(fr["vx"], fr["vy"], d2, fr["nx"], fr["ny"], fr["nz"], fr["mask"])
+        ctx_room.upload_model_maps(*(model[k] for k in ("x", "y", "z", "nx", "ny", "nz")))
+        T = ctx_room.estimate_pose(K, gt[1])
+        hist = ctx_room.icp_history(masks=True)
     finally:
-        ctx1.set_params(icp_compensated=0)
+        ctx_room.set_params(icp_compensated=0)
     n = prm.icp_iterations
     assert hist["T"].shape[0] == n
     # replay each GPU iteration's starting estimate through the oracle: the
@@ -230,14 +253,14 @@ def test_icp_masks_bitexact_and_pose(kf, ctx1, config0_stream, oracle, model_vol
     assert np.abs(T[:3, 3] - want["T"][:3, 3]).max() < TRANS_TOL_MM
 
 
-def test_pipeline_first_frames_match_oracle(kf, ctx1, config0_stream, oracle):
-    vol, prm, K, H, W, frames, gt = config0_stream
+def test_pipeline_first_frames_match_oracle(kf, ctx_room, room_stream, oracle):
+    vol, prm, K, H, W, frames, gt = room_stream
     n = frames.shape[0]
     say(f"[gpu] pipeline {n} frames")
-    ctx1.reset()
-    ctx1.stream_upload(frames)
-    ctx1.pipeline_run(K, 0, n)
-    poses = ctx1.poses(n)
+    ctx_room.reset()
+    ctx_room.stream_upload(frames)
+    ctx_room.pipeline_run(K, 0, n)
+    poses = ctx_room.poses(n)
     want, t, w = oracle.pipeline(frames, K, vol, prm, reduce_mode=0)
     for i in range(n):
         assert rot_err(poses[i], want[i]) < ROT_TOL, i
