@@ -183,8 +183,13 @@ int kf_integrate(kf_context* ctx, const float K[9], const float T[16],
  * contexts widen; F16 weight 0 is returned as NaN, "never observed"). */
 int kf_download_volume(kf_context* ctx, float* tsdf, float* weight);
 int kf_upload_volume(kf_context* ctx, const float* tsdf, const float* weight);
-/* Native storage: F32 = interleaved {tsdf, weight} float pairs,
- * F16 = uint32 {half tsdf (bits 0-15), uint16 weight (bits 16-31)}. */
+/* Native storage, two arrays back to back: all TSDF values, then all
+ * weights (F32: float + float; F16: IEEE half bits + uint16).  Each array is
+ * brick-tiled: voxel (x, y, z) sits at
+ *   ((((x>>3)*By + (y>>3))*Bz + (z>>3)) << 9) | ((x&7)<<6) | ((y&7)<<3) | (z&7)
+ * with By = ceil(Ny/8), Bz = ceil(Nz/8), and each array holds
+ * ceil(Nx/8)*By*Bz*512 entries (padding voxels stay "never observed").
+ * bytes must equal that size; kf_volume_device_ptr reports it. */
 int kf_download_volume_raw(kf_context* ctx, void* out, size_t bytes);
 int kf_upload_volume_raw(kf_context* ctx, const void* in, size_t bytes);
 /* Device address of the native volume (for zero-copy consumers). */

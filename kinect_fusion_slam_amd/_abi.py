@@ -367,8 +367,10 @@ class KinfuContext:
         return t, w
 
     def volume_raw(self):
-        isz = 8 if self.vol.precision == KF_TSDF_F32 else 4
-        out = np.zeros(self.nvox * isz, np.uint8)
+        """Native brick-tiled storage (TSDF array then weight array)."""
+        ptr, nbytes = C.c_void_p(), C.c_size_t()
+        check(lib.kf_volume_device_ptr(self.h, C.byref(ptr), C.byref(nbytes)))
+        out = np.zeros(nbytes.value, np.uint8)
         check(lib.kf_download_volume_raw(self.h, out.ctypes.data_as(_P), out.size))
         return out
 
