@@ -66,7 +66,8 @@ extern "C" {
 #define KF_STAGE_INTEGRATE 4 /* projective TSDF integration, per-voxel kernel  */
 #define KF_STAGE_RAYCAST 5   /* voxel-traversal raycast                       */
 #define KF_STAGE_INTEG_PREP 6 /* integration per-pixel table                  */
-#define KF_STAGE_COUNT 7
+#define KF_STAGE_RAYCAST_ADJOINT 7 /* kf_raycast_adjoint kernels              */
+#define KF_STAGE_COUNT 8
 
 /* Parameter presets for kf_preset(). */
 #define KF_PRESET_REFERENCE 100 /* the reference's compiled-in defaults          */
@@ -109,7 +110,17 @@ typedef struct kf_params {
                                  checks (voxel_traversal.cu:233-281, which
                                  test ZX/ZY against voxel_height); 1: test
                                  every index against its own axis        */
+  int32_t raycast_mode;     /* KF_RAYCAST_REFERENCE or KF_RAYCAST_TRILINEAR */
 } kf_params;
+
+/* Raycast hit handling.  REFERENCE: the reference's per-axis interpolation
+ * and central-difference normal (voxel_traversal.cu:222-377).  TRILINEAR:
+ * the same walk, then the zero crossing of the trilinearly interpolated TSDF
+ * between samples spaced one voxel along the ray from the skip point, and
+ * the normal as the normalised trilinear gradient there.  TRILINEAR is
+ * differentiable (kf_raycast_adjoint) and has no reference counterpart. */
+#define KF_RAYCAST_REFERENCE 0
+#define KF_RAYCAST_TRILINEAR 1
 
 /* ---- context ------------------------------------------------------------ */
 
@@ -207,6 +218,21 @@ int kf_download_model_maps(kf_context* ctx, float* x, float* y, float* z,
 int kf_upload_model_maps(kf_context* ctx, const float* x, const float* y,
                          const float* z, const float* nx, const float* ny,
                          const float* nz);
+/* Camera-frame depth (mm) of the raycast vertices, 0 for no hit; written by
+ * KF_RAYCAST_TRILINEAR raycasts (0 everywhere after a REFERENCE raycast). */
+int kf_download_rendered_depth(kf_context* ctx, float* depth_mm);
+/* Adjoint of the TRILINEAR raycast at pose T with the context's depth and
+ * volume (no reference counterpart; the differentiable renderer).  Given
+ * dL/d(rendered depth) [H*W] and optionally dL/d(normal) [H*W*3, xyz
+ * interleaved, world frame], returns dL/dTSDF in the reference layout
+ * [Nx*Ny*Nz] (nullable) and dL/dxi (nullable) for the left perturbation
+ * T(xi) = exp([omega, v]^) T, xi = (omega_x, omega_y, omega_z, v_x, v_y,
+ * v_z) in rad and mm.  The walk and the refinement are recomputed per ray;
+ * nothing is stored by the forward pass.  dL/dTSDF is accumulated with
+ * float atomics, so its low bits depend on the order of arrival. */
+int kf_raycast_adjoint(kf_context* ctx, const float K[9], const float T[16],
+                       const float* dl_ddepth, const float* dl_dnormal, float* dl_dtsdf,
+                       double dl_dxi[6]);
 
 /* ---- frame sequence (KinfuPipeline, kinfu_pipeline.cpp:91-193) ----------- */
 /* Keeps n frames of uint16 depth resident in HBM for the pipeline calls. */

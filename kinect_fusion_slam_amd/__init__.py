@@ -6,9 +6,10 @@ as hand-written HIP kernels behind the C ABI in include/kinfu_amd.h.
 This package is the Python view of that ABI (ctypes).  Every stage runs in
 libkinfu_amd.so; importing without the built library fails.
 """
-from ._abi import (KF_MEM_DEVICE, KF_MEM_HOST, KF_TSDF_F16, KF_TSDF_F32,  # noqa: F401
-                   PRESET_REFERENCE, PRESET_ROOM, PRESET_ROOM_5MM_F16, PRESET_ROOM_CPU,
-                   PRESET_ROOM_HD_F16, PRESET_SPHERES_128, PRESETS, STAGES, KinfuContext, KinfuError, Params, SynthScene,
+from ._abi import (KF_MEM_DEVICE, KF_MEM_HOST, KF_RAYCAST_REFERENCE,  # noqa: F401
+                   KF_RAYCAST_TRILINEAR, KF_TSDF_F16, KF_TSDF_F32, PRESET_REFERENCE, PRESET_ROOM,
+                   PRESET_ROOM_5MM_F16, PRESET_ROOM_CPU, PRESET_ROOM_HD_F16, PRESET_SPHERES_128,
+                   PRESETS, STAGES, KinfuContext, KinfuError, Params, SynthScene,
                    VolumeDesc, exported_symbols, gate_thresholds, ldlt6, lib, mat3_inverse,
                    mat4_inverse, orbit_pose, preset, preset_scene, render, spatial_kernel,
                    stream)

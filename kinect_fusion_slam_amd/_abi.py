@@ -17,7 +17,9 @@ KF_TSDF_F32, KF_TSDF_F16 = 0, 1
 KF_MEM_HOST, KF_MEM_DEVICE = 0, 1
 KF_MAX_ICP_ITERS = 64
 KF_ICP_SUMS = 28
-STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep")
+STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep",
+          "raycast_adjoint")
+KF_RAYCAST_REFERENCE, KF_RAYCAST_TRILINEAR = 0, 1
 PRESET_REFERENCE = 100
 PRESET_ROOM_CPU = 0       # 640x480, 360x360x300 @10mm FP32, 10-frame CPU case
 PRESET_ROOM = 1           # same geometry, the GPU stream
@@ -45,7 +47,8 @@ class Params(C.Structure):
                 ("sigma_r_mm", C.c_float), ("icp_iterations", C.c_int32),
                 ("icp_dist_mm", C.c_float), ("icp_angle_deg", C.c_float),
                 ("icp_compensated", C.c_int32), ("record_icp", C.c_int32),
-                ("stage_timing", C.c_int32), ("raycast_fix_bounds", C.c_int32)]
+                ("stage_timing", C.c_int32), ("raycast_fix_bounds", C.c_int32),
+                ("raycast_mode", C.c_int32)]
 
 
 class SynthScene(C.Structure):
@@ -103,6 +106,8 @@ _SIGNATURES = {
     "kf_stage_times": (C.c_int, [_P, _D, _I64]),
     "kf_reset_stage_times": (C.c_int, [_P]),
     "kf_count_raycast_work": (C.c_int, [_P, C.c_int32]),
+    "kf_download_rendered_depth": (C.c_int, [_P, _F]),
+    "kf_raycast_adjoint": (C.c_int, [_P, _F, _F, _F, _F, _F, _D]),
     "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
@@ -438,6 +443,24 @@ class KinfuContext:
 
     def reset_stage_times(self):
         check(lib.kf_reset_stage_times(self.h))
+
+    def rendered_depth(self):
+        out = np.zeros((self.height, self.width), np.float32)
+        check(lib.kf_download_rendered_depth(self.h, fptr(out)))
+        return out
+
+    def raycast_adjoint(self, K, T, grad_depth, grad_normal=None, want_tsdf=True):
+        """Adjoint of the trilinear raycast: (dL/dTSDF in the reference
+        layout or None, dL/dxi as 6 float64, omega then v)."""
+        gD = _f32(grad_depth, self.n)
+        gN = _f32(grad_normal, 3 * self.n) if grad_normal is not None else None
+        gF = np.zeros(self.nvox, np.float32) if want_tsdf else None
+        gxi = np.zeros(6, np.float64)
+        check(lib.kf_raycast_adjoint(self.h, fptr(_f32(K, 9)), fptr(_f32(T, 16)), fptr(gD),
+                                     fptr(gN) if gN is not None else None,
+                                     fptr(gF) if gF is not None else None,
+                                     gxi.ctypes.data_as(_D)))
+        return gF, gxi
 
     def count_raycast_work(self, enable=True):
         check(lib.kf_count_raycast_work(self.h, 1 if enable else 0))

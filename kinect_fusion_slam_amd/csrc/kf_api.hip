@@ -116,6 +116,10 @@ bool params_valid(const kf_params* p, std::string* why) {
     *why = "truncation and sigmas must be positive";
     return false;
   }
+  if (p->raycast_mode != KF_RAYCAST_REFERENCE && p->raycast_mode != KF_RAYCAST_TRILINEAR) {
+    *why = "raycast_mode must be KF_RAYCAST_REFERENCE or KF_RAYCAST_TRILINEAR";
+    return false;
+  }
   return true;
 }
 
@@ -252,6 +256,7 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   p.record_icp = 0;
   p.stage_timing = 0;
   p.raycast_fix_bounds = 0;
+  p.raycast_mode = KF_RAYCAST_REFERENCE;
   switch (which) {
     case KF_PRESET_REFERENCE: {
       // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,
@@ -285,6 +290,10 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
       H = 240;
       W = 320;
       p.bilateral = 0;
+      p.truncation_mm = 50.0f;  // covers the 2 cm pose jitter of the gradient checks
+      p.sdf_min_mm = -50.0f;
+      p.sdf_max_mm = 500.0f;
+      p.raycast_mode = KF_RAYCAST_TRILINEAR;
       break;
     }
     case KF_PRESET_ROOM_HD_F16: {
@@ -349,7 +358,7 @@ int kf_destroy(kf_context* c) {
   void* bufs[] = {c->ray_queue, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
-                  c->frames,  c->poses};
+                  c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -824,6 +833,57 @@ int kf_download_model_maps(kf_context* c, float* x, float* y, float* z, float* n
     if (ny) ny[i] = N[i].y;
     if (nz) nz[i] = N[i].z;
   }
+  return KF_OK;
+}
+
+int kf_download_rendered_depth(kf_context* c, float* depth_mm) {
+  KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
+  const size_t n = (size_t)c->H * c->W;
+  std::vector<float4> V(n);
+  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->model_v, n * 16, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; ++i) depth_mm[i] = V[i].w;
+  return KF_OK;
+}
+
+int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const float* dl_ddepth,
+                       const float* dl_dnormal, float* dl_dtsdf, double dl_dxi[6]) {
+  KF_REQUIRE(c && K && T && dl_ddepth, KF_ERR_INVALID, "null argument");
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  const size_t n = (size_t)c->H * c->W;
+  if (!c->grad_tsdf) {
+    KF_HIP_CHECK(hipMalloc((void**)&c->grad_tsdf, (size_t)c->n_store * 4));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_depth, n * 4));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_normal, n * 16));
+  }
+  int rc = kf_set_pose(c, T);
+  if (rc != KF_OK) return rc;
+  KF_HIP_CHECK(hipMemcpyAsync(c->adj_depth, dl_ddepth, n * 4, hipMemcpyHostToDevice, c->stream));
+  c->adj_normal_set = dl_dnormal != nullptr;
+  std::vector<float4> gn;
+  if (dl_dnormal) {
+    gn.resize(n);
+    for (size_t i = 0; i < n; ++i)
+      gn[i] = make_float4(dl_dnormal[3 * i], dl_dnormal[3 * i + 1], dl_dnormal[3 * i + 2], 0.f);
+    KF_HIP_CHECK(hipMemcpyAsync(c->adj_normal, gn.data(), n * 16, hipMemcpyHostToDevice,
+                                c->stream));
+  }
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  kf_stage_begin(c, KF_STAGE_RAYCAST_ADJOINT);
+  KF_LAUNCH(kf_launch_raycast_adjoint(c, cam));
+  kf_stage_end(c, KF_STAGE_RAYCAST_ADJOINT);
+  double xi[6];
+  KF_HIP_CHECK(hipMemcpyAsync(xi, c->state->xi_grad, sizeof(xi), hipMemcpyDeviceToHost,
+                              c->stream));
+  std::vector<float> G(dl_dtsdf ? (size_t)c->n_store : 0);
+  if (dl_dtsdf)
+    KF_HIP_CHECK(hipMemcpyAsync(G.data(), c->grad_tsdf, G.size() * 4, hipMemcpyDeviceToHost,
+                                c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (dl_dxi)
+    for (int i = 0; i < 6; ++i) dl_dxi[i] = xi[i];
+  if (dl_dtsdf) for_each_voxel(c, [&](int64_t lin, int64_t s) { dl_dtsdf[lin] = G[s]; });
   return KF_OK;
 }
 

@@ -24,6 +24,7 @@ struct kf_dev_state {
   int32_t pad[2];
   unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
   unsigned long long ray_hits;
+  double xi_grad[6];             // raycast adjoint: dL/dxi (omega, v)
   float iter_T[KF_MAX_ICP_ITERS][16];
   double iter_sums[KF_MAX_ICP_ITERS][KF_ICP_SUMS];
   int32_t iter_status[KF_MAX_ICP_ITERS];
@@ -113,11 +114,15 @@ struct kf_context {
   float* poses = nullptr;          // [n_frames][16]
 
   // timing
-  hipEvent_t ev[2 * KF_STAGE_COUNT] = {};
   double stage_ms[KF_STAGE_COUNT] = {};
   int64_t stage_count[KF_STAGE_COUNT] = {};
   bool have_depth = false;
   bool count_rays = false;  // raycast work counters on
+  // raycast adjoint buffers (allocated on first use)
+  float* grad_tsdf = nullptr;   // dL/dF, n_store floats, native order
+  float* adj_depth = nullptr;   // dL/d(rendered depth), H*W
+  float4* adj_normal = nullptr; // dL/d(normal), H*W
+  bool adj_normal_set = false;
 };
 
 // error reporting ---------------------------------------------------------
@@ -138,6 +143,7 @@ hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first);
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam);
+hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_volume_fill(kf_context* c);
 hipError_t kf_launch_set_pose(kf_context* c, const float* T_host);
 hipError_t kf_launch_record_pose(kf_context* c, int frame);

@@ -948,7 +948,30 @@ struct RayState {
   float dx, dy, dz;
   DdaStep stp;
   int pix;
+  float tent;  // ray parameter where the walk entered cur (trilinear modes)
 };
+
+// Launch constants of the raycast kernels.
+struct RayIO {
+  kf_vol vg;
+  int fix_bounds;
+  float flip_hi;
+  float trunc;
+  int W;
+  float Ki[9];
+  const float* depth;
+  // adjoint inputs/outputs (MODE 2)
+  const float* gD;     // dL/d(depth), H*W
+  const float4* gN;    // dL/d(normal), H*W (nullable)
+  float* gF;           // dL/d(TSDF), native store order
+};
+
+// Raycast modes: 0 = the reference's hit handling (voxel_traversal.cu:222-
+// 377); 1 = trilinear zero-crossing refinement (vertex, normal, depth);
+// 2 = adjoint of mode 1.
+constexpr int kModeReference = 0;
+constexpr int kModeTrilinear = 1;
+constexpr int kModeAdjoint = 2;
 
 // Ray setup (voxel_traversal.cu:90-200): ray skipping to (depth - trunc),
 // slab test, 1-based start voxel, steps and boundary distances.  Returns
@@ -1057,6 +1080,200 @@ __device__ __forceinline__ bool ray_setup(int x, int y, float dval, const float*
   r.dx = dx;
   r.dy = dy;
   r.dz = dz;
+  r.tent = tmin;
+  return true;
+}
+
+// Ray parameter at which the next dda_step enters its new voxel (the
+// boundary distance of the axis it steps).
+__device__ __forceinline__ float dda_entry(const DdaState& s) {
+  const bool xy = s.tmx < s.tmy;
+  const bool xm = xy && s.tmx < s.tmz;
+  const bool ym = !xy && s.tmy < s.tmz;
+  return xm ? s.tmx : (ym ? s.tmy : s.tmz);
+}
+
+// ---------------------------------------------------------------------------
+// Trilinear zero-crossing refinement: the differentiable raycast mode (no
+// reference counterpart; the reference interpolates per axis from voxel
+// values only, voxel_traversal.cu:318-349, which has no pose gradient).
+//
+// After the walk finds its crossing voxel, the TSDF is sampled trilinearly at
+// p_k = sg + (k d) dir, a grid of one-voxel spacing anchored at the skip
+// point sg, for k from floor(t_entry / d) - 2 on.  The first pair of valid
+// samples with F(p_k) > 0 >= F(p_k+1) gives
+//   t* = t_k + d F_k / (F_k - F_k+1),  p* = sg + t* dir,
+//   depth = sc + t* / |lambda|  (sc = (|lambda| depth_in - trunc)/|lambda|),
+//   normal = grad F(p*) / |grad F(p*)|.
+// A sample is taken in voxel-centre coordinates u = (p - min)/d - 1/2 and is
+// valid when its 8 corners lie inside the volume and are observed.
+constexpr int kTriSamples = 6;
+
+struct Tri {
+  float v;       // value
+  float g[3];    // gradient (per mm)
+  float w[3];    // fractional position in the cell
+  int i, j, k;   // lower corner, 0-based voxel indices
+  float c[8];    // corner values, index (dx << 2) | (dy << 1) | dz
+};
+
+template <int PREC>
+__device__ __forceinline__ bool tri_sample(const VolRead<PREC>& F, const kf_vol& vg, float px,
+                                           float py, float pz, Tri& s) {
+  const float d = vg.d;
+  const float ux = (px - (float)vg.sx * d) / d - 0.5f;
+  const float uy = (py - (float)vg.sy * d) / d - 0.5f;
+  const float uz = (pz - (float)vg.sz * d) / d - 0.5f;
+  const int i = kf_floor_to_int(ux), j = kf_floor_to_int(uy), k = kf_floor_to_int(uz);
+  if (!(i >= 0 && i < vg.nx - 1 && j >= 0 && j < vg.ny - 1 && k >= 0 && k < vg.nz - 1))
+    return false;
+  bool ok = true;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    s.c[q] = F(kf_store_index(vg, i + (q >> 2), j + ((q >> 1) & 1), k + (q & 1)));
+    ok = ok && s.c[q] == s.c[q];
+  }
+  if (!ok) return false;
+  const float fx = ux - (float)i, fy = uy - (float)j, fz = uz - (float)k;
+  const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+  const float c00 = s.c[0] * ox + s.c[4] * fx;  // y0 z0
+  const float c01 = s.c[1] * ox + s.c[5] * fx;  // y0 z1
+  const float c10 = s.c[2] * ox + s.c[6] * fx;  // y1 z0
+  const float c11 = s.c[3] * ox + s.c[7] * fx;  // y1 z1
+  const float c0 = c00 * oy + c10 * fy;
+  const float c1 = c01 * oy + c11 * fy;
+  s.v = c0 * oz + c1 * fz;
+  const float e00 = s.c[4] - s.c[0], e01 = s.c[5] - s.c[1];
+  const float e10 = s.c[6] - s.c[2], e11 = s.c[7] - s.c[3];
+  s.g[0] = ((e00 * oy + e10 * fy) * oz + (e01 * oy + e11 * fy) * fz) / d;
+  s.g[1] = ((c10 - c00) * oz + (c11 - c01) * fz) / d;
+  s.g[2] = (c1 - c0) / d;
+  s.w[0] = fx;
+  s.w[1] = fy;
+  s.w[2] = fz;
+  s.i = i;
+  s.j = j;
+  s.k = k;
+  return true;
+}
+
+// dL/dF of every corner of sample s: a * weight_q + b . grad(weight_q).
+__device__ __forceinline__ void tri_scatter(float* __restrict__ gF, const kf_vol& vg,
+                                            const Tri& s, float a, float bx, float by,
+                                            float bz) {
+  const float d = vg.d;
+  const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
+  const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+    const float wx = qx ? fx : ox, wy = qy ? fy : oy, wz = qz ? fz : oz;
+    const float sx = qx ? 1.0f : -1.0f, sy = qy ? 1.0f : -1.0f, sz = qz ? 1.0f : -1.0f;
+    const float val = a * (wx * wy * wz) +
+                      (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
+    if (val != 0.0f) atomicAdd(&gF[kf_store_index(vg, s.i + qx, s.j + qy, s.k + qz)], val);
+  }
+}
+
+// Trilinear hit of a ray whose walk crossed at a voxel entered at parameter
+// te.  MODE 1 writes vertex (w = depth) and normal; MODE 2 adds the adjoint
+// contributions of this pixel (dL/dF scattered, dL/dxi into xi).
+template <int PREC, int MODE>
+__device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, const float* T,
+                                     int pix, float te, float dx, float dy, float dz,
+                                     float4& ov, float4& on, double* xi) {
+  const kf_vol& vg = io.vg;
+  const float* Ki = io.Ki;
+  const int x = pix % io.W, y = pix / io.W;
+  const float dval = io.depth[pix];
+  // skip point, as in ray_setup
+  const float X = (float)x + 1.0f, Y = (float)y + 1.0f;
+  const float lamx = Ki[0] * X + Ki[1] * Y + Ki[2] * 1.f;
+  const float lamy = Ki[3] * X + Ki[4] * Y + Ki[5] * 1.f;
+  const float lamz = Ki[6] * X + Ki[7] * Y + Ki[8] * 1.f;
+  const float lam = sqrtf(lamx * lamx + lamy * lamy + lamz * lamz);
+  const float sc = (lam * dval - io.trunc) / lam;
+  const float slx = sc * lamx, sly = sc * lamy, slz = sc * lamz;
+  const float sgx = T[0] * slx + T[1] * sly + T[2] * slz + T[3] * 1.f;
+  const float sgy = T[4] * slx + T[5] * sly + T[6] * slz + T[7] * 1.f;
+  const float sgz = T[8] * slx + T[9] * sly + T[10] * slz + T[11] * 1.f;
+  const float d = vg.d;
+  if (!(te >= 0.0f && te < 1.0e9f)) return false;
+  const int kb = kf_floor_to_int(te / d) - 2;
+  Tri a, b;
+  bool va = false, found = false;
+  int ka = 0;
+  for (int k = kb; k < kb + kTriSamples; ++k) {
+    const float t = (float)k * d;
+    const bool ok = tri_sample<PREC>(F, vg, sgx + t * dx, sgy + t * dy, sgz + t * dz, b);
+    if (ok && va && a.v > 0.0f && b.v <= 0.0f) {
+      ka = k - 1;
+      found = true;
+      break;
+    }
+    va = ok;
+    if (ok) a = b;
+  }
+  if (!found) return false;
+  const float ta = (float)ka * d;
+  const float den = a.v - b.v;
+  const float ts = ta + d * (a.v / den);
+  const float psx = sgx + ts * dx, psy = sgy + ts * dy, psz = sgz + ts * dz;
+  Tri s;
+  if (!tri_sample<PREC>(F, vg, psx, psy, psz, s)) return false;
+  const float gn = sqrtf(s.g[0] * s.g[0] + s.g[1] * s.g[1] + s.g[2] * s.g[2]);
+  if (!(gn > 0.0f)) return false;
+  const float nx = s.g[0] / gn, ny = s.g[1] / gn, nz = s.g[2] / gn;
+  if constexpr (MODE == kModeTrilinear) {
+    ov = make_float4(psx, psy, psz, sc + ts / lam);
+    on = make_float4(nx, ny, nz, 0.0f);
+  } else {
+    const float gD = io.gD[pix];
+    const float4 gNv = io.gN ? io.gN[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float dts = gD / lam;  // dL/dt* through the depth
+    // normal: n = g/|g|, g = grad F(p*) (linear in the corners of p*)
+    const float ndot = nx * gNv.x + ny * gNv.y + nz * gNv.z;
+    const float lgx = (gNv.x - nx * ndot) / gn;
+    const float lgy = (gNv.y - ny * ndot) / gn;
+    const float lgz = (gNv.z - nz * ndot) / gn;
+    tri_scatter(io.gF, vg, s, 0.0f, lgx, lgy, lgz);
+    // dL/dp* = H lg, H the trilinear Hessian at p* (zero diagonal)
+    const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
+    const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+    const float e00 = s.c[4] - s.c[0], e01 = s.c[5] - s.c[1];
+    const float e10 = s.c[6] - s.c[2], e11 = s.c[7] - s.c[3];
+    const float c00 = s.c[0] * ox + s.c[4] * fx, c01 = s.c[1] * ox + s.c[5] * fx;
+    const float c10 = s.c[2] * ox + s.c[6] * fx, c11 = s.c[3] * ox + s.c[7] * fx;
+    const float dd = d * d;
+    const float hxy = ((e10 - e00) * oz + (e11 - e01) * fz) / dd;
+    const float hxz = ((e01 * oy + e11 * fy) - (e00 * oy + e10 * fy)) / dd;
+    const float hyz = ((c11 - c01) - (c10 - c00)) / dd;
+    const float lpx = hxy * lgy + hxz * lgz;
+    const float lpy = hxy * lgx + hyz * lgz;
+    const float lpz = hxz * lgx + hyz * lgy;
+    dts += lpx * dx + lpy * dy + lpz * dz;
+    xi[0] += (double)(psy * lpz - psz * lpy);
+    xi[1] += (double)(psz * lpx - psx * lpz);
+    xi[2] += (double)(psx * lpy - psy * lpx);
+    xi[3] += (double)lpx;
+    xi[4] += (double)lpy;
+    xi[5] += (double)lpz;
+    // secant: t* = t_a + d F_a / (F_a - F_b)
+    const float den2 = den * den;
+    const float dFa = dts * d * (-b.v) / den2;
+    const float dFb = dts * d * a.v / den2;
+    tri_scatter(io.gF, vg, a, dFa, 0.0f, 0.0f, 0.0f);
+    tri_scatter(io.gF, vg, b, dFb, 0.0f, 0.0f, 0.0f);
+    const float tb = ta + d;
+    const float pax = sgx + ta * dx, pay = sgy + ta * dy, paz = sgz + ta * dz;
+    const float pbx = sgx + tb * dx, pby = sgy + tb * dy, pbz = sgz + tb * dz;
+    xi[0] += (double)(dFa * (pay * a.g[2] - paz * a.g[1]) + dFb * (pby * b.g[2] - pbz * b.g[1]));
+    xi[1] += (double)(dFa * (paz * a.g[0] - pax * a.g[2]) + dFb * (pbz * b.g[0] - pbx * b.g[2]));
+    xi[2] += (double)(dFa * (pax * a.g[1] - pay * a.g[0]) + dFb * (pbx * b.g[1] - pby * b.g[0]));
+    xi[3] += (double)(dFa * a.g[0] + dFb * b.g[0]);
+    xi[4] += (double)(dFa * a.g[1] + dFb * b.g[1]);
+    xi[5] += (double)(dFa * a.g[2] + dFb * b.g[2]);
+  }
   return true;
 }
 
@@ -1067,11 +1284,11 @@ __device__ __forceinline__ bool ray_setup(int x, int y, float dval, const float*
 // (:207-222) in order: the same voxels are visited and the same decisions
 // taken.  Returns true when the ray is finished (hit or not); on false the
 // state is ready to continue.
-template <int PREC, int kRayBatch, bool COUNT>
-__device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, const kf_vol& vg,
-                                         int fix_bounds, const float* cam, float flip_hi,
-                                         int max_batches, float4& ov, float4& on, int& hit,
-                                         int& visited) {
+template <int PREC, int kRayBatch, bool COUNT, int MODE>
+__device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, const RayIO& io,
+                                         const float* T, int max_batches, float4& ov,
+                                         float4& on, int& hit, int& visited, double* xi) {
+  const kf_vol& vg = io.vg;
   const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
   for (int bt = 0; bt < max_batches; ++bt) {
     // generate the next kRayBatch voxels of the walk (branch-free: the walk
@@ -1080,6 +1297,7 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
     int nv = 0;
     bool alive = true;
     DdaState s = r.cur;
+    float te_s = r.tent;
 #pragma unroll
     for (int k = 0; k < kRayBatch; ++k) {
       const bool in =
@@ -1087,6 +1305,7 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
       alive = alive && in;
       vis[k] = alive ? kf_store_index(vg, s.ix - 1, s.iy - 1, s.iz - 1) : 0;
       nv += alive ? 1 : 0;
+      if constexpr (MODE != kModeReference) te_s = dda_entry(s);
       dda_step(s, r.stp);
     }
     float fs[kRayBatch];
@@ -1112,14 +1331,46 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
     r.prev = prev;
     if (kc >= 0) {
       DdaState h = r.cur;
-      for (int j = 0; j < kc; ++j) dda_step(h, r.stp);
-      hit = crossing<PREC>(F, h, fc, vg, fix_bounds, cam[0], cam[1], cam[2], r.dx, r.dy, r.dz,
-                           flip_hi, ov, on);
+      float te = r.tent;
+      for (int j = 0; j < kc; ++j) {
+        if constexpr (MODE != kModeReference) te = dda_entry(h);
+        dda_step(h, r.stp);
+      }
+      if constexpr (MODE == kModeReference) {
+        hit = crossing<PREC>(F, h, fc, vg, io.fix_bounds, T[3], T[7], T[11], r.dx, r.dy, r.dz,
+                             io.flip_hi, ov, on);
+      } else {
+        const bool ok = tri_hit<PREC, MODE>(F, io, T, r.pix, te, r.dx, r.dy, r.dz, ov, on, xi);
+        hit = ok ? (h.ix - 1) * (Ny * Nz) + (h.iy - 1) * Nz + (h.iz - 1) : -1;
+      }
     }
     if (done || nv < kRayBatch) return true;
     r.cur = s;
+    r.tent = te_s;
   }
   return false;
+}
+
+// Optional work counters (COUNT): voxels visited by the walks and rays that
+// hit, summed into the device state for the algorithmic-traffic figure.
+__device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool hit) {
+  uint64_t v = (uint64_t)visited;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const uint64_t h = __popcll(__ballot(hit));
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&st->ray_steps, (unsigned long long)v);
+    atomicAdd(&st->ray_hits, (unsigned long long)h);
+  }
+}
+
+// Adjoint: the wave's dL/dxi partials into the device state.
+__device__ __forceinline__ void add_xi(kf_dev_state* st, const double* xi) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double v = xi[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&st->xi_grad[q], v);
+  }
 }
 
 // Pass 1: every pixel with depth != 0 casts its ray (voxel_traversal.cu:
@@ -1134,24 +1385,11 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
 // of the image and its L2 sees the band's voxels.
 constexpr int kPass1Batches = 2;  // of 16 steps
 
-// Optional work counters (COUNT): voxels visited by the walks and rays that
-// hit, summed into the device state for the algorithmic-traffic figure.
-__device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool hit) {
-  uint64_t v = (uint64_t)visited;
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  const uint64_t h = __popcll(__ballot(hit));
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&st->ray_steps, (unsigned long long)v);
-    atomicAdd(&st->ray_hits, (unsigned long long)h);
-  }
-}
-
-template <int PREC, bool COUNT>
+template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast(
-    const void* __restrict__ volume, const float* __restrict__ depth, kf_cam cam, kf_vol vg,
-    int H, int W, float trunc, int fix_bounds, float flip_hi,
-    kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
-    int32_t* __restrict__ out_hit, RayState* __restrict__ queue, int tiles_x, int n_blocks) {
+    const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
+    float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
+    RayState* __restrict__ queue, int tiles_x, int n_blocks) {
   int bid = blockIdx.x;
   {
     // XCD x receives blocks x, x+8, ...: q+1 of them for x < r, q otherwise.
@@ -1160,6 +1398,7 @@ __global__ __launch_bounds__(256) void k_raycast(
     const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
     bid = base + loc;
   }
+  const int W = io.W;
   const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int bx = bid % tiles_x, by = bid / tiles_x;
   const int x = bx * 16 + (tile & 1) * 8 + (ln & 7);
@@ -1173,19 +1412,20 @@ __global__ __launch_bounds__(256) void k_raycast(
   int hit = -1;
   bool pending = false;
   int visited = 0;
+  double xi[6] = {0, 0, 0, 0, 0, 0};
   RayState r;
-  const float dval = active ? depth[p] : 0.0f;
+  float T[12];
+  load_uniform<12>(st->T, T);
+  const float dval = active ? io.depth[p] : 0.0f;
   if (dval != 0) {
-    float T[12];
-    load_uniform<12>(st->T, T);
-    const float camv[3] = {T[3], T[7], T[11]};
-    if (ray_setup(x, y, dval, cam.Kinv, T, vg, trunc, r)) {
+    if (ray_setup(x, y, dval, io.Ki, T, io.vg, io.trunc, r)) {
       r.pix = p;
-      pending = !ray_walk<PREC, 16, COUNT>(F, r, vg, fix_bounds, camv, flip_hi, kPass1Batches,
-                                           ov, on, hit, visited);
+      pending = !ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, kPass1Batches, ov, on, hit,
+                                                 visited, xi);
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
+  if constexpr (MODE == kModeAdjoint) add_xi(st, xi);
   // wave-aggregated append of the unfinished rays
   const uint64_t mask = __ballot(pending);
   if (mask) {
@@ -1195,44 +1435,56 @@ __global__ __launch_bounds__(256) void k_raycast(
     base = __shfl(base, leader);
     if (pending) queue[base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
-  if (!active) return;
+  if (!active || MODE == kModeAdjoint) return;
   out_v[p] = ov;
   out_n[p] = on;
   out_hit[p] = hit;
 }
 
 // Pass 2: finishes the queued rays, grid-stride over the queue.
-template <int PREC, bool COUNT>
+template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast_rest(
-    const void* __restrict__ volume, kf_vol vg, int fix_bounds, float flip_hi,
-    kf_dev_state* __restrict__ st, float4* __restrict__ out_v, float4* __restrict__ out_n,
-    int32_t* __restrict__ out_hit, const RayState* __restrict__ queue) {
+    const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
+    float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
+    const RayState* __restrict__ queue) {
   const uint32_t n = st->ray_queue;
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
-  const float camv[3] = {st->T[3], st->T[7], st->T[11]};
+  float T[12];
+  load_uniform<12>(st->T, T);
   const uint32_t stride = gridDim.x * 256;
-  // every lane runs the same number of trips so that the counters' wave
-  // reductions see the whole wave
-  const uint32_t trips = COUNT ? (n + stride - 1) / stride : 0;
+  // with wave reductions at the end of a trip, every lane runs the same
+  // number of trips
+  constexpr bool kUniform = COUNT || MODE == kModeAdjoint;
+  const uint32_t trips = kUniform ? (n + stride - 1) / stride : 0;
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  for (uint32_t t = 0; COUNT ? t < trips : i < n; ++t, i += stride) {
+  for (uint32_t t = 0; kUniform ? t < trips : i < n; ++t, i += stride) {
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
+    double xi[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
       RayState r = queue[i];
-      ray_walk<PREC, 16, COUNT>(F, r, vg, fix_bounds, camv, flip_hi, 0x7fffffff, ov, on, hit,
-                                visited);
-      out_v[r.pix] = ov;
-      out_n[r.pix] = on;
-      out_hit[r.pix] = hit;
+      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi);
+      if constexpr (MODE != kModeAdjoint) {
+        out_v[r.pix] = ov;
+        out_n[r.pix] = on;
+        out_hit[r.pix] = hit;
+      }
     }
     if constexpr (COUNT) add_counts(st, visited, hit >= 0);
+    if constexpr (MODE == kModeAdjoint) add_xi(st, xi);
   }
 }
 
 __global__ void k_zero_queue(kf_dev_state* st) {
   if (threadIdx.x == 0) st->ray_queue = 0u;
+}
+
+// Zeroes the adjoint outputs: dL/dxi in the state and the dL/dF array.
+__global__ __launch_bounds__(256) void k_zero_adjoint(kf_dev_state* st, float* gF, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < 6) st->xi_grad[i] = 0.0;
+  if (i < n) gF[i] = 0.0f;
 }
 
 }  // namespace
@@ -1348,35 +1600,65 @@ hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
   return hipGetLastError();
 }
 
-template <int PREC, bool COUNT>
-static void launch_raycast_t(kf_context* c, const kf_cam& cam, int tiles_x, int nb) {
-  const kf_params& p = c->prm;
+static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
+  RayIO io;
+  io.vg = c->vg;
+  io.fix_bounds = c->prm.raycast_fix_bounds;
+  io.flip_hi = c->flip_cos_hi;
+  io.trunc = c->prm.truncation_mm;
+  io.W = c->W;
+  for (int i = 0; i < 9; ++i) io.Ki[i] = cam.Kinv[i];
+  io.depth = c->depth;
+  io.gD = c->adj_depth;
+  io.gN = c->adj_normal_set ? c->adj_normal : nullptr;
+  io.gF = c->grad_tsdf;
+  return io;
+}
+
+template <int PREC, bool COUNT, int MODE>
+static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
   RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
-  k_raycast<PREC, COUNT><<<nb, 256, 0, c->stream>>>(
-      c->tsdf, c->depth, cam, c->vg, c->H, c->W, p.truncation_mm, p.raycast_fix_bounds,
-      c->flip_cos_hi, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
-  k_raycast_rest<PREC, COUNT><<<256, 256, 0, c->stream>>>(
-      c->tsdf, c->vg, p.raycast_fix_bounds, c->flip_cos_hi, c->state, c->model_v, c->model_n,
-      c->hit, q);
+  k_raycast<PREC, COUNT, MODE><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
+                                                          c->model_v, c->model_n, c->hit, q,
+                                                          tiles_x, nb);
+  k_raycast_rest<PREC, COUNT, MODE><<<256, 256, 0, c->stream>>>(c->tsdf, io, c->state,
+                                                                c->model_v, c->model_n, c->hit,
+                                                                q);
+}
+
+template <int PREC>
+static void launch_raycast_p(kf_context* c, const RayIO& io, int tiles_x, int nb, bool count,
+                             int mode) {
+  if (mode == kModeAdjoint)
+    launch_raycast_t<PREC, false, kModeAdjoint>(c, io, tiles_x, nb);
+  else if (mode == kModeTrilinear)
+    count ? launch_raycast_t<PREC, true, kModeTrilinear>(c, io, tiles_x, nb)
+          : launch_raycast_t<PREC, false, kModeTrilinear>(c, io, tiles_x, nb);
+  else
+    count ? launch_raycast_t<PREC, true, kModeReference>(c, io, tiles_x, nb)
+          : launch_raycast_t<PREC, false, kModeReference>(c, io, tiles_x, nb);
+}
+
+static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode) {
+  const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
+  const int nb = tiles_x * tiles_y;
+  const RayIO io = make_ray_io(c, cam);
+  k_zero_queue<<<1, 64, 0, c->stream>>>(c->state);
+  if (c->vol.precision == KF_TSDF_F32)
+    launch_raycast_p<KF_TSDF_F32>(c, io, tiles_x, nb, c->count_rays, mode);
+  else
+    launch_raycast_p<KF_TSDF_F16>(c, io, tiles_x, nb, c->count_rays, mode);
+  return hipGetLastError();
 }
 
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam) {
-  const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
-  const int nb = tiles_x * tiles_y;
-  k_zero_queue<<<1, 64, 0, c->stream>>>(c->state);
-  const bool count = c->count_rays;
-  if (c->vol.precision == KF_TSDF_F32) {
-    if (count)
-      launch_raycast_t<KF_TSDF_F32, true>(c, cam, tiles_x, nb);
-    else
-      launch_raycast_t<KF_TSDF_F32, false>(c, cam, tiles_x, nb);
-  } else {
-    if (count)
-      launch_raycast_t<KF_TSDF_F16, true>(c, cam, tiles_x, nb);
-    else
-      launch_raycast_t<KF_TSDF_F16, false>(c, cam, tiles_x, nb);
-  }
-  return hipGetLastError();
+  return launch_raycast_mode(c, cam, c->prm.raycast_mode == 1 ? kModeTrilinear : kModeReference);
+}
+
+hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam) {
+  const int64_t n = c->n_store;
+  k_zero_adjoint<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->state, c->grad_tsdf, n);
+  return launch_raycast_mode(c, cam, kModeAdjoint);
 }
 
 size_t kf_ray_state_bytes() { return sizeof(RayState); }

@@ -405,10 +405,185 @@ struct RayStats {
   int64_t hits;
 };
 
+// Trilinear zero-crossing refinement (the differentiable raycast mode; no
+// reference counterpart).  After the walk's crossing voxel, entered at ray
+// parameter te, the TSDF is sampled trilinearly at p_k = sg + (k d) dir for
+// k = floor(te/d) - 2 ... + 5; the first valid pair with F_k > 0 >= F_k+1
+// gives t* = t_k + d F_k / (F_k - F_k+1), p* = sg + t* dir, the camera depth
+// sc + t*/|lambda| and the normal grad F(p*) / |grad F(p*)|.  Samples are in
+// voxel-centre coordinates u = (p - min)/d - 1/2, valid when all 8 corners
+// are inside the volume and observed.
+template <class S>
+struct TriS {
+  S v, g[3], w[3];
+  int i, j, k;
+  S c[8];  // corner (dx<<2)|(dy<<1)|dz
+};
+
+template <class S>
+struct TriCtx {
+  S* out7;          // x, y, z, nx, ny, nz, depth of this pixel
+  int32_t* disc;    // discrete state of the hit: sample k and 3 cells (nullable)
+  // adjoint (gF non-null): dL/d(depth), dL/d(normal)
+  S gD;
+  const S* gN;      // 3 values (nullable)
+  S* gF;            // dense dL/dF, reference layout
+  S* gxi;           // 6 values
+};
+
+template <class S, class Vol>
+inline bool tri_sample(const Vol& F, const int* dims, const int* start, S d, S px, S py, S pz,
+                       TriS<S>& s) {
+  const int Nx = dims[0], Ny = dims[1], Nz = dims[2];
+  const S ux = (px - (S)(float)start[0] * d) / d - S(0.5);
+  const S uy = (py - (S)(float)start[1] * d) / d - S(0.5);
+  const S uz = (pz - (S)(float)start[2] * d) / d - S(0.5);
+  const int i = floor_i(ux), j = floor_i(uy), k = floor_i(uz);
+  if (!(i >= 0 && i < Nx - 1 && j >= 0 && j < Ny - 1 && k >= 0 && k < Nz - 1)) return false;
+  for (int q = 0; q < 8; ++q) {
+    const int64_t lin = ((int64_t)(i + (q >> 2)) * Ny + (j + ((q >> 1) & 1))) * Nz + (k + (q & 1));
+    s.c[q] = F(lin);
+    if (std::isnan(s.c[q])) return false;
+  }
+  const S fx = ux - (S)i, fy = uy - (S)j, fz = uz - (S)k;
+  const S ox = S(1) - fx, oy = S(1) - fy, oz = S(1) - fz;
+  const S c00 = s.c[0] * ox + s.c[4] * fx;
+  const S c01 = s.c[1] * ox + s.c[5] * fx;
+  const S c10 = s.c[2] * ox + s.c[6] * fx;
+  const S c11 = s.c[3] * ox + s.c[7] * fx;
+  const S c0 = c00 * oy + c10 * fy;
+  const S c1 = c01 * oy + c11 * fy;
+  s.v = c0 * oz + c1 * fz;
+  const S e00 = s.c[4] - s.c[0], e01 = s.c[5] - s.c[1];
+  const S e10 = s.c[6] - s.c[2], e11 = s.c[7] - s.c[3];
+  s.g[0] = ((e00 * oy + e10 * fy) * oz + (e01 * oy + e11 * fy) * fz) / d;
+  s.g[1] = ((c10 - c00) * oz + (c11 - c01) * fz) / d;
+  s.g[2] = (c1 - c0) / d;
+  s.w[0] = fx;
+  s.w[1] = fy;
+  s.w[2] = fz;
+  s.i = i;
+  s.j = j;
+  s.k = k;
+  return true;
+}
+
+// dL/dF of the corners of s: a * weight + b . grad(weight)
+template <class S>
+inline void tri_scatter(S* gF, const int* dims, S d, const TriS<S>& s, S a, S bx, S by, S bz) {
+  const int Ny = dims[1], Nz = dims[2];
+  const S fx = s.w[0], fy = s.w[1], fz = s.w[2];
+  const S ox = S(1) - fx, oy = S(1) - fy, oz = S(1) - fz;
+  for (int q = 0; q < 8; ++q) {
+    const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+    const S wx = qx ? fx : ox, wy = qy ? fy : oy, wz = qz ? fz : oz;
+    const S sx = qx ? S(1) : S(-1), sy = qy ? S(1) : S(-1), sz = qz ? S(1) : S(-1);
+    const S val = a * (wx * wy * wz) +
+                  (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
+    const int64_t lin = ((int64_t)(s.i + qx) * Ny + (s.j + qy)) * Nz + (s.k + qz);
+#pragma omp atomic
+    gF[lin] += val;
+  }
+}
+
+template <class S, class Vol>
+inline bool tri_cross(const Vol& F, const int* dims, const int* start, S d, S sgx, S sgy, S sgz,
+                      S dx, S dy, S dz, S te, S sc, S lam, const TriCtx<S>& tc) {
+  if (!(te >= S(0) && te < S(1.0e9))) return false;
+  const int kb = floor_i(te / d) - 2;
+  TriS<S> a{}, b{};
+  bool va = false, found = false;
+  int ka = 0;
+  for (int k = kb; k < kb + 6; ++k) {
+    const S t = (S)(float)k * d;
+    const bool ok = tri_sample<S>(F, dims, start, d, sgx + t * dx, sgy + t * dy, sgz + t * dz, b);
+    if (ok && va && a.v > S(0) && b.v <= S(0)) {
+      ka = k - 1;
+      found = true;
+      break;
+    }
+    va = ok;
+    if (ok) a = b;
+  }
+  if (!found) return false;
+  const S ta = (S)(float)ka * d;
+  const S den = a.v - b.v;
+  const S ts = ta + d * (a.v / den);
+  const S psx = sgx + ts * dx, psy = sgy + ts * dy, psz = sgz + ts * dz;
+  TriS<S> s;
+  if (!tri_sample<S>(F, dims, start, d, psx, psy, psz, s)) return false;
+  const S gn = std::sqrt(s.g[0] * s.g[0] + s.g[1] * s.g[1] + s.g[2] * s.g[2]);
+  if (!(gn > S(0))) return false;
+  const S nx = s.g[0] / gn, ny = s.g[1] / gn, nz = s.g[2] / gn;
+  S* o = tc.out7;
+  o[0] = psx;
+  o[1] = psy;
+  o[2] = psz;
+  o[3] = nx;
+  o[4] = ny;
+  o[5] = nz;
+  o[6] = sc + ts / lam;
+  if (tc.disc) {
+    const int Ny = dims[1], Nz = dims[2];
+    tc.disc[0] = ka;
+    tc.disc[1] = (a.i * Ny + a.j) * Nz + a.k;
+    tc.disc[2] = (b.i * Ny + b.j) * Nz + b.k;
+    tc.disc[3] = (s.i * Ny + s.j) * Nz + s.k;
+  }
+  if (!tc.gF) return true;
+  // adjoint
+  S dts = tc.gD / lam;
+  const S gx = tc.gN ? tc.gN[0] : S(0), gy = tc.gN ? tc.gN[1] : S(0), gz = tc.gN ? tc.gN[2] : S(0);
+  const S ndot = nx * gx + ny * gy + nz * gz;
+  const S lgx = (gx - nx * ndot) / gn, lgy = (gy - ny * ndot) / gn, lgz = (gz - nz * ndot) / gn;
+  tri_scatter<S>(tc.gF, dims, d, s, S(0), lgx, lgy, lgz);
+  const S fx = s.w[0], fy = s.w[1], fz = s.w[2];
+  const S ox = S(1) - fx, oy = S(1) - fy, oz = S(1) - fz;
+  const S e00 = s.c[4] - s.c[0], e01 = s.c[5] - s.c[1];
+  const S e10 = s.c[6] - s.c[2], e11 = s.c[7] - s.c[3];
+  const S c00 = s.c[0] * ox + s.c[4] * fx, c01 = s.c[1] * ox + s.c[5] * fx;
+  const S c10 = s.c[2] * ox + s.c[6] * fx, c11 = s.c[3] * ox + s.c[7] * fx;
+  const S dd = d * d;
+  const S hxy = ((e10 - e00) * oz + (e11 - e01) * fz) / dd;
+  const S hxz = ((e01 * oy + e11 * fy) - (e00 * oy + e10 * fy)) / dd;
+  const S hyz = ((c11 - c01) - (c10 - c00)) / dd;
+  const S lpx = hxy * lgy + hxz * lgz;
+  const S lpy = hxy * lgx + hyz * lgz;
+  const S lpz = hxz * lgx + hyz * lgy;
+  dts += lpx * dx + lpy * dy + lpz * dz;
+  S xi[6];
+  xi[0] = psy * lpz - psz * lpy;
+  xi[1] = psz * lpx - psx * lpz;
+  xi[2] = psx * lpy - psy * lpx;
+  xi[3] = lpx;
+  xi[4] = lpy;
+  xi[5] = lpz;
+  const S den2 = den * den;
+  const S dFa = dts * d * (-b.v) / den2;
+  const S dFb = dts * d * a.v / den2;
+  tri_scatter<S>(tc.gF, dims, d, a, dFa, S(0), S(0), S(0));
+  tri_scatter<S>(tc.gF, dims, d, b, dFb, S(0), S(0), S(0));
+  const S tb = ta + d;
+  const S pax = sgx + ta * dx, pay = sgy + ta * dy, paz = sgz + ta * dz;
+  const S pbx = sgx + tb * dx, pby = sgy + tb * dy, pbz = sgz + tb * dz;
+  xi[0] += dFa * (pay * a.g[2] - paz * a.g[1]) + dFb * (pby * b.g[2] - pbz * b.g[1]);
+  xi[1] += dFa * (paz * a.g[0] - pax * a.g[2]) + dFb * (pbz * b.g[0] - pbx * b.g[2]);
+  xi[2] += dFa * (pax * a.g[1] - pay * a.g[0]) + dFb * (pbx * b.g[1] - pby * b.g[0]);
+  xi[3] += dFa * a.g[0] + dFb * b.g[0];
+  xi[4] += dFa * a.g[1] + dFb * b.g[1];
+  xi[5] += dFa * a.g[2] + dFb * b.g[2];
+  for (int q = 0; q < 6; ++q) {
+#pragma omp atomic
+    tc.gxi[q] += xi[q];
+  }
+  return true;
+}
+
 template <class S, class Vol>
 inline int raycast_pixel(int x, int y, int W, float dval_f, const S* Ki, const S* T,
                          const Vol& F, const int* dims, const int* start, S d, S trunc,
-                         int fix_bounds, S* out6, int64_t* steps) {
+                         int fix_bounds, S* out6, int64_t* steps,
+                         const TriCtx<S>* tri = nullptr) {
   const int Nx = dims[0], Ny = dims[1], Nz = dims[2];
   const S dval = (S)dval_f;
   const S X = (S)(float)x + S(1), Y = (S)(float)y + S(1);
@@ -473,6 +648,7 @@ inline int raycast_pixel(int x, int y, int W, float dval_f, const S* Ki, const S
   S tmz = tmin + ((vmz - stz) / dz);
   const S tdx = d / std::fabs(dx), tdy = d / std::fabs(dy), tdz = d / std::fabs(dz);
   S prev = 0;
+  S te = tmin;  // ray parameter where the walk entered the current voxel
   const int64_t NyNz = (int64_t)Ny * Nz;
   int64_t vi = (int64_t)(ix - 1) * NyNz + (int64_t)(iy - 1) * Nz + (iz - 1);
   const int bXY = fix_bounds ? Ny : Nx, bXZ = fix_bounds ? Nz : Nx;
@@ -484,6 +660,12 @@ inline int raycast_pixel(int x, int y, int W, float dval_f, const S* Ki, const S
     if (std::isnan(f) && prev < 0) break;
     if (f < 0 && std::isnan(prev)) break;
     if (f > 0 && prev < 0) break;
+    if (f < 0 && prev > 0 && tri) {
+      const S sc = (lam * dval - trunc) / lam;
+      return tri_cross<S>(F, dims, start, d, sgx, sgy, sgz, dx, dy, dz, te, sc, lam, *tri)
+                 ? (int)vi
+                 : -1;
+    }
     if (f < 0 && prev > 0) {
       const int xn = (ix - 1) - 1, xp = (ix - 1) + 1;
       const int yn = (iy - 1) - 1, yp = (iy - 1) + 1;
@@ -566,11 +748,13 @@ inline int raycast_pixel(int x, int y, int W, float dval_f, const S* Ki, const S
     prev = f;
     if (tmx < tmy) {
       if (tmx < tmz) {
+        te = tmx;
         ix = ix + sx_;
         XY = iy;
         XZ = iz;
         tmx = tmx + tdx;
       } else {
+        te = tmz;
         iz = iz + sz_;
         ZX = ix;
         ZY = iy;
@@ -578,11 +762,13 @@ inline int raycast_pixel(int x, int y, int W, float dval_f, const S* Ki, const S
       }
     } else {
       if (tmy < tmz) {
+        te = tmy;
         iy = iy + sy_;
         YX = ix;
         YZ = iz;
         tmy = tmy + tdy;
       } else {
+        te = tmz;
         iz = iz + sz_;
         ZX = ix;
         ZY = iy;
@@ -631,6 +817,44 @@ void raycast_impl(const float* depth, int H, int W, const float* Kf, const float
   }
 }
 
+// Trilinear raycast over an image; T in the scalar type S (FP64 for the
+// finite-difference checks).  out: x, y, z, nx, ny, nz, depth planes; disc
+// (nullable): 4 ints per pixel.  With gF non-null, also the adjoint for
+// dL/d(depth) gD (and dL/d(normal) gN, 3 per pixel, nullable).
+template <class S, class Vol>
+void raycast_tri_impl(const float* depth, int H, int W, const float* Kf, const S* T,
+                      const Vol& F, const int* dims, const int* start, float voxel_mm,
+                      float trunc, S* const* out, int32_t* hit, int32_t* disc, const S* gD,
+                      const S* gN, S* gF, S* gxi) {
+  float Kif[9];
+  kf_m3_inverse(Kf, Kif);
+  S Ki[9];
+  for (int i = 0; i < 9; ++i) Ki[i] = (S)Kif[i];
+  const int64_t n = (int64_t)H * W;
+  if (gxi)
+    for (int q = 0; q < 6; ++q) gxi[q] = 0;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t p = 0; p < n; ++p) {
+    S o[7] = {0, 0, 0, 0, 0, 0, 0};
+    int h = -1;
+    int32_t dsc[4] = {-1, -1, -1, -1};
+    if (depth[p] != 0) {
+      TriCtx<S> tc{o, dsc, gD ? gD[p] : S(0), gN ? gN + 3 * p : nullptr, gF, gxi};
+      S o6[6];
+      int64_t steps = 0;
+      h = raycast_pixel<S>((int)(p % W), (int)(p / W), W, depth[p], Ki, T, F, dims, start,
+                           (S)voxel_mm, (S)trunc, 0, o6, &steps, &tc);
+      if (h < 0)
+        for (int k = 0; k < 7; ++k) o[k] = 0;
+    }
+    for (int k = 0; k < 7; ++k)
+      if (out[k]) out[k][p] = o[k];
+    if (hit) hit[p] = h;
+    if (disc)
+      for (int k = 0; k < 4; ++k) disc[4 * p + k] = h >= 0 ? dsc[k] : -1;
+  }
+}
+
 struct F32Reader {
   const float* t;
   inline float operator()(int64_t i) const { return t[i]; }
@@ -672,6 +896,34 @@ void oracle_raycast_f64(const float* depth, int H, int W, const float* K, const 
                         double* nx, double* ny, double* nz, int32_t* hit, int64_t* stats) {
   raycast_impl<double>(depth, H, W, K, T, F64Reader{tsdf}, dims, start, voxel_mm, trunc,
                        fix_bounds, vx, vy, vz, nx, ny, nz, hit, stats);
+}
+
+// Trilinear raycast, FP32 (parity with the kernels).  out7: 7 planes.
+void oracle_raycast_tri_f32(const float* depth, int H, int W, const float* K, const float* T,
+                            const float* tsdf, const int* dims, const int* start,
+                            float voxel_mm, float trunc, float** out7, int32_t* hit,
+                            int32_t* disc) {
+  raycast_tri_impl<float>(depth, H, W, K, T, F32Reader{tsdf}, dims, start, voxel_mm, trunc,
+                          out7, hit, disc, nullptr, nullptr, nullptr, nullptr);
+}
+
+void oracle_raycast_tri_f16(const float* depth, int H, int W, const float* K, const float* T,
+                            const uint16_t* tsdf_h, const int* dims, const int* start,
+                            float voxel_mm, float trunc, float** out7, int32_t* hit,
+                            int32_t* disc) {
+  raycast_tri_impl<float>(depth, H, W, K, T, F16Reader{tsdf_h}, dims, start, voxel_mm, trunc,
+                          out7, hit, disc, nullptr, nullptr, nullptr, nullptr);
+}
+
+// FP64 forward with an FP64 pose, and its adjoint when gF is non-null
+// (gF: dense dL/dF in the reference layout, zeroed by the caller).
+void oracle_raycast_tri_f64(const float* depth, int H, int W, const float* K, const double* T,
+                            const double* tsdf, const int* dims, const int* start,
+                            float voxel_mm, float trunc, double** out7, int32_t* hit,
+                            int32_t* disc, const double* gD, const double* gN, double* gF,
+                            double* gxi) {
+  raycast_tri_impl<double>(depth, H, W, K, T, F64Reader{tsdf}, dims, start, voxel_mm, trunc,
+                           out7, hit, disc, gD, gN, gF, gxi);
 }
 
 // The literal gates, for tests of the cosine thresholds.

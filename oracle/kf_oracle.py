@@ -36,6 +36,13 @@ def _load():
                                         C.c_float, C.c_float, C.c_float, C.c_int, _D, _D]),
         "oracle_integrate_f16": (None, [_F, C.c_int, C.c_int, _F, _F, _I, _I, C.c_float,
                                         C.c_float, C.c_float, C.c_float, C.c_int, _U16, _U16]),
+        "oracle_raycast_tri_f32": (None, [_F, C.c_int, C.c_int, _F, _F, _F, _I, _I, C.c_float,
+                                          C.c_float, C.POINTER(_F), _I32, _I32]),
+        "oracle_raycast_tri_f16": (None, [_F, C.c_int, C.c_int, _F, _F, _U16, _I, _I, C.c_float,
+                                          C.c_float, C.POINTER(_F), _I32, _I32]),
+        "oracle_raycast_tri_f64": (None, [_F, C.c_int, C.c_int, _F, _D, _D, _I, _I, C.c_float,
+                                          C.c_float, C.POINTER(_D), _I32, _I32, _D, _D, _D,
+                                          _D]),
         "oracle_raycast_f32": (None, [_F, C.c_int, C.c_int, _F, _F, _F, _I, _I, C.c_float,
                                       C.c_float, C.c_int, _F, _F, _F, _F, _F, _F, _I32, _I64]),
         "oracle_raycast_f16": (None, [_F, C.c_int, C.c_int, _F, _F, _U16, _I, _I, C.c_float,
@@ -170,6 +177,52 @@ def raycast(depth, K, T, tsdf, vol, trunc, fix_bounds=0):
     res = dict(zip(("x", "y", "z", "nx", "ny", "nz"), outs))
     res["hit"] = hit
     res["steps"], res["hits"] = int(stats[0]), int(stats[1])
+    return res
+
+
+TRI_KEYS = ("x", "y", "z", "nx", "ny", "nz", "depth")
+
+
+def raycast_tri(depth, K, T, tsdf, vol, trunc, grad_depth=None, grad_normal=None):
+    """Trilinear raycast.  tsdf float32 / uint16 (half bits): FP32 forward with
+    a float32 pose.  tsdf float64: FP64 forward with a float64 pose, plus the
+    adjoint when grad_depth is given (returns "grad_tsdf", "grad_xi")."""
+    d = _c32(depth)
+    H, W = d.shape
+    hit = np.zeros((H, W), np.int32)
+    disc = np.zeros((H, W, 4), np.int32)
+    tail = (_ivec(vol.dims), _ivec(vol.start), vol.voxel_mm, trunc)
+    if tsdf.dtype == np.float64:
+        outs = [np.zeros((H, W), np.float64) for _ in range(7)]
+        arr = (_D * 7)(*(o.ctypes.data_as(_D) for o in outs))
+        Td = np.ascontiguousarray(T, np.float64).reshape(16)
+        gF = gxi = gD = gN = None
+        if grad_depth is not None:
+            gD = np.ascontiguousarray(grad_depth, np.float64).reshape(H, W)
+            gN = (np.ascontiguousarray(grad_normal, np.float64).reshape(H, W, 3)
+                  if grad_normal is not None else None)
+            gF = np.zeros(tsdf.size, np.float64)
+            gxi = np.zeros(6, np.float64)
+        ptr = lambda a: a.ctypes.data_as(_D) if a is not None else None  # noqa: E731
+        lib.oracle_raycast_tri_f64(_f(d), H, W, _f(_c32(K, 9)), Td.ctypes.data_as(_D),
+                                   tsdf.ctypes.data_as(_D), *tail, arr,
+                                   hit.ctypes.data_as(_I32), disc.ctypes.data_as(_I32),
+                                   ptr(gD), ptr(gN), ptr(gF), ptr(gxi))
+    else:
+        outs = [np.zeros((H, W), np.float32) for _ in range(7)]
+        arr = (_F * 7)(*(_f(o) for o in outs))
+        args = (_f(d), H, W, _f(_c32(K, 9)), _f(_c32(T, 16)))
+        if tsdf.dtype == np.float32:
+            lib.oracle_raycast_tri_f32(*args, _f(tsdf), *tail, arr, hit.ctypes.data_as(_I32),
+                                       disc.ctypes.data_as(_I32))
+        else:
+            lib.oracle_raycast_tri_f16(*args, tsdf.ctypes.data_as(_U16), *tail, arr,
+                                       hit.ctypes.data_as(_I32), disc.ctypes.data_as(_I32))
+        gF = gxi = None
+    res = dict(zip(TRI_KEYS, outs))
+    res["hit"], res["disc"] = hit, disc
+    if gF is not None:
+        res["grad_tsdf"], res["grad_xi"] = gF, gxi
     return res
 
 
