@@ -13,6 +13,9 @@ Runs, each on a fresh context over the same frames:
   2. FP16 TSDF + u16 weight, timed              -> fp16
   3. FP32 with per-stage hipEvents               -> stage_ms, roofline
   4. FP32 with the raycast work counters on      -> algorithmic bytes
+  5. FP32 with the trilinear raycast, timed      -> trilinear.value, ATE
+  6. the same with stage events, then 20 raycast + raycast-adjoint pairs on
+     the final state                             -> adjoint cost
 and the CPU oracle on a bounded sample of the same workload.
 
 Prints one JSON line.  The roofline object is for the raycast stage (the
@@ -57,11 +60,15 @@ def ate_mm(poses, gt):
     return float(np.sqrt((err ** 2).mean()))
 
 
-def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False):
-    """One pass over frames[0 .. warmup+steps]; times frames warmup+1 ..."""
+def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, mode=0,
+            adjoint_reps=0):
+    """One pass over frames[0 .. warmup+steps]; times frames warmup+1 ...
+    With adjoint_reps, then times that many (raycast, raycast adjoint) pairs
+    on the final volume, pose and depth."""
     vol, prm, _, H, W = kf.preset(kf.PRESET_ROOM)
     vol.precision = precision
     prm.stage_timing = 1 if timing else 0
+    prm.raycast_mode = mode
     ctx = kf.KinfuContext(vol, prm, H, W)
     try:
         ctx.stream_upload(frames)
@@ -80,6 +87,19 @@ def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False):
         stages = ctx.stage_times()
         work = ctx.raycast_work() if count else None
         poses = ctx.poses(warmup + 1 + steps)
+        if adjoint_reps:
+            T = ctx.pose()
+            ones = np.ones((H, W), np.float32)
+            work = {}
+            for full in (False, True):
+                ctx.reset_stage_times()
+                for _ in range(adjoint_reps if not full else max(1, adjoint_reps // 4)):
+                    ctx.raycast(K, T)
+                    ctx.raycast_adjoint(K, T, ones, want_tsdf=full)
+                adj = ctx.stage_times()
+                key = "full" if full else "pose_only"
+                work["raycast"] = adj["raycast"][0] / max(adj["raycast"][1], 1)
+                work[key] = adj["raycast_adjoint"][0] / max(adj["raycast_adjoint"][1], 1)
     finally:
         ctx.close()
     return elapsed, stages, poses, work
@@ -156,6 +176,11 @@ def main():
     _, st16, _, _ = run_gpu(kf, kf.KF_TSDF_F16, frames, K, args.warmup, args.steps, timing=True)
     _, _, _, work = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps, count=True)
     print("[bench] stage timing and work counters done", file=sys.stderr, flush=True)
+    elt, _, ptri, _ = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps,
+                              mode=kf.KF_RAYCAST_TRILINEAR)
+    _, sttri, _, adj = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps,
+                               timing=True, mode=kf.KF_RAYCAST_TRILINEAR, adjoint_reps=20)
+    print(f"[bench] trilinear {args.steps / elt:.1f} frames/s", file=sys.stderr, flush=True)
 
     def per_stage(st):
         return {k: round(ms / max(cnt, 1), 4) for k, (ms, cnt) in st.items() if cnt}
@@ -215,6 +240,16 @@ def main():
         "ate_mm": {"fp32": round(a32, 3), "fp16": round(a16, 3),
                    "fp16_minus_fp32": round(a16 - a32, 3)},
         "stage_ms": per_stage(st32),
+        "trilinear": {
+            "note": "same stream and precision with the trilinear zero-crossing raycast "
+                    "(differentiable mode) instead of the reference interpolation",
+            "value": round(args.steps / elt, 2), "unit": "frames/s",
+            "ate_mm": round(ate_mm(ptri, gt), 3), "stage_ms": per_stage(sttri),
+            "raycast_forward_ms": round(adj["raycast"], 4),
+            "adjoint_pose_only_ms": round(adj["pose_only"], 4),
+            "adjoint_pose_and_tsdf_ms": round(adj["full"], 4),
+            "adjoint_vs_forward": round(adj["full"] / adj["raycast"], 3),
+            "adjoint_pose_only_vs_forward": round(adj["pose_only"] / adj["raycast"], 3)},
         "roofline": roofline,
         "stage_roofline": other,
     }

@@ -229,7 +229,8 @@ int kf_download_rendered_depth(kf_context* ctx, float* depth_mm);
  * T(xi) = exp([omega, v]^) T, xi = (omega_x, omega_y, omega_z, v_x, v_y,
  * v_z) in rad and mm.  The walk and the refinement are recomputed per ray;
  * nothing is stored by the forward pass.  dL/dTSDF is accumulated with
- * float atomics, so its low bits depend on the order of arrival. */
+ * float atomics, so its low bits depend on the order of arrival; with
+ * dl_dtsdf NULL it is not computed at all (pose-only adjoint). */
 int kf_raycast_adjoint(kf_context* ctx, const float K[9], const float T[16],
                        const float* dl_ddepth, const float* dl_dnormal, float* dl_dtsdf,
                        double dl_dxi[6]);

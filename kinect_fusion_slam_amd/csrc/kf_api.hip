@@ -358,7 +358,8 @@ int kf_destroy(kf_context* c) {
   void* bufs[] = {c->ray_queue, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
-                  c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal};
+                  c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
+                  c->adj_partials};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -855,11 +856,13 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
     KF_HIP_CHECK(hipMalloc((void**)&c->grad_tsdf, (size_t)c->n_store * 4));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_depth, n * 4));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_normal, n * 16));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_partials, kf_adjoint_partials(c) * 8));
   }
   int rc = kf_set_pose(c, T);
   if (rc != KF_OK) return rc;
   KF_HIP_CHECK(hipMemcpyAsync(c->adj_depth, dl_ddepth, n * 4, hipMemcpyHostToDevice, c->stream));
   c->adj_normal_set = dl_dnormal != nullptr;
+  c->adj_want_tsdf = dl_dtsdf != nullptr;
   std::vector<float4> gn;
   if (dl_dnormal) {
     gn.resize(n);

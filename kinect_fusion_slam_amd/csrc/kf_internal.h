@@ -123,6 +123,8 @@ struct kf_context {
   float* adj_depth = nullptr;   // dL/d(rendered depth), H*W
   float4* adj_normal = nullptr; // dL/d(normal), H*W
   bool adj_normal_set = false;
+  bool adj_want_tsdf = false;   // the current adjoint call returns dL/dTSDF
+  double* adj_partials = nullptr; // dL/dxi block partials
 };
 
 // error reporting ---------------------------------------------------------
@@ -144,6 +146,7 @@ hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first);
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam);
+size_t kf_adjoint_partials(const kf_context* c);  // doubles
 hipError_t kf_launch_volume_fill(kf_context* c);
 hipError_t kf_launch_set_pose(kf_context* c, const float* T_host);
 hipError_t kf_launch_record_pose(kf_context* c, int frame);

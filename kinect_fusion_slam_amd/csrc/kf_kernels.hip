@@ -963,7 +963,9 @@ struct RayIO {
   // adjoint inputs/outputs (MODE 2)
   const float* gD;     // dL/d(depth), H*W
   const float4* gN;    // dL/d(normal), H*W (nullable)
-  float* gF;           // dL/d(TSDF), native store order
+  float* gF;           // dL/d(TSDF), native store order (null: not wanted)
+  double* xi_part;     // dL/dxi partials, 6 per block
+  int part_base;       // first partial slot of this launch
 };
 
 // Raycast modes: 0 = the reference's hit handling (voxel_traversal.cu:222-
@@ -1157,10 +1159,39 @@ __device__ __forceinline__ bool tri_sample(const VolRead<PREC>& F, const kf_vol&
   return true;
 }
 
+// Where the adjoint's dL/dF contributions go.  In pass 1 a block's rays hit
+// a compact set of voxels, so contributions are first summed in a per-block
+// LDS hash table (key = native voxel index) and flushed with one global
+// atomic per distinct voxel; a full probe sequence, and pass 2, fall back to
+// global atomics.
+constexpr int kHashBits = 12;
+constexpr int kHash = 1 << kHashBits;
+constexpr int kHashProbes = 16;
+
+struct Sink {
+  float* gF;   // global dL/dF (native order); null: no TSDF gradient
+  int* key;    // LDS table keys (-1 = empty); null: global atomics only
+  float* val;  // LDS table values
+};
+
+__device__ __forceinline__ void sink_add(const Sink& k, int idx, float v) {
+  if (k.key) {
+    uint32_t h = ((uint32_t)idx * 2654435761u) >> (32 - kHashBits);
+    for (int probe = 0; probe < kHashProbes; ++probe) {
+      const int old = atomicCAS(&k.key[h], -1, idx);
+      if (old == -1 || old == idx) {
+        atomicAdd(&k.val[h], v);
+        return;
+      }
+      h = (h + 1) & (kHash - 1);
+    }
+  }
+  atomicAdd(&k.gF[idx], v);
+}
+
 // dL/dF of every corner of sample s: a * weight_q + b . grad(weight_q).
-__device__ __forceinline__ void tri_scatter(float* __restrict__ gF, const kf_vol& vg,
-                                            const Tri& s, float a, float bx, float by,
-                                            float bz) {
+__device__ __forceinline__ void tri_scatter(const Sink& sink, const kf_vol& vg, const Tri& s,
+                                            float a, float bx, float by, float bz) {
   const float d = vg.d;
   const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
   const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
@@ -1171,7 +1202,7 @@ __device__ __forceinline__ void tri_scatter(float* __restrict__ gF, const kf_vol
     const float sx = qx ? 1.0f : -1.0f, sy = qy ? 1.0f : -1.0f, sz = qz ? 1.0f : -1.0f;
     const float val = a * (wx * wy * wz) +
                       (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
-    if (val != 0.0f) atomicAdd(&gF[kf_store_index(vg, s.i + qx, s.j + qy, s.k + qz)], val);
+    if (val != 0.0f) sink_add(sink, kf_store_index(vg, s.i + qx, s.j + qy, s.k + qz), val);
   }
 }
 
@@ -1181,7 +1212,7 @@ __device__ __forceinline__ void tri_scatter(float* __restrict__ gF, const kf_vol
 template <int PREC, int MODE>
 __device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, const float* T,
                                      int pix, float te, float dx, float dy, float dz,
-                                     float4& ov, float4& on, double* xi) {
+                                     float4& ov, float4& on, double* xi, const Sink& sink) {
   const kf_vol& vg = io.vg;
   const float* Ki = io.Ki;
   const int x = pix % io.W, y = pix / io.W;
@@ -1236,7 +1267,7 @@ __device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, co
     const float lgx = (gNv.x - nx * ndot) / gn;
     const float lgy = (gNv.y - ny * ndot) / gn;
     const float lgz = (gNv.z - nz * ndot) / gn;
-    tri_scatter(io.gF, vg, s, 0.0f, lgx, lgy, lgz);
+    if (sink.gF) tri_scatter(sink, vg, s, 0.0f, lgx, lgy, lgz);
     // dL/dp* = H lg, H the trilinear Hessian at p* (zero diagonal)
     const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
     const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
@@ -1262,8 +1293,10 @@ __device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, co
     const float den2 = den * den;
     const float dFa = dts * d * (-b.v) / den2;
     const float dFb = dts * d * a.v / den2;
-    tri_scatter(io.gF, vg, a, dFa, 0.0f, 0.0f, 0.0f);
-    tri_scatter(io.gF, vg, b, dFb, 0.0f, 0.0f, 0.0f);
+    if (sink.gF) {
+      tri_scatter(sink, vg, a, dFa, 0.0f, 0.0f, 0.0f);
+      tri_scatter(sink, vg, b, dFb, 0.0f, 0.0f, 0.0f);
+    }
     const float tb = ta + d;
     const float pax = sgx + ta * dx, pay = sgy + ta * dy, paz = sgz + ta * dz;
     const float pbx = sgx + tb * dx, pby = sgy + tb * dy, pbz = sgz + tb * dz;
@@ -1287,7 +1320,8 @@ __device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, co
 template <int PREC, int kRayBatch, bool COUNT, int MODE>
 __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, const RayIO& io,
                                          const float* T, int max_batches, float4& ov,
-                                         float4& on, int& hit, int& visited, double* xi) {
+                                         float4& on, int& hit, int& visited, double* xi,
+                                         const Sink& sink) {
   const kf_vol& vg = io.vg;
   const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
   for (int bt = 0; bt < max_batches; ++bt) {
@@ -1340,7 +1374,8 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
         hit = crossing<PREC>(F, h, fc, vg, io.fix_bounds, T[3], T[7], T[11], r.dx, r.dy, r.dz,
                              io.flip_hi, ov, on);
       } else {
-        const bool ok = tri_hit<PREC, MODE>(F, io, T, r.pix, te, r.dx, r.dy, r.dz, ov, on, xi);
+        const bool ok =
+            tri_hit<PREC, MODE>(F, io, T, r.pix, te, r.dx, r.dy, r.dz, ov, on, xi, sink);
         hit = ok ? (h.ix - 1) * (Ny * Nz) + (h.iy - 1) * Nz + (h.iz - 1) : -1;
       }
     }
@@ -1364,13 +1399,21 @@ __device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool h
 }
 
 // Adjoint: the wave's dL/dxi partials into the device state.
-__device__ __forceinline__ void add_xi(kf_dev_state* st, const double* xi) {
+// Adjoint: the block's dL/dxi partial sum, written to its slot of part
+// (6 doubles per block; summed by k_adjoint_finish in a fixed order).
+__device__ __forceinline__ void block_xi(const double* xi, double* __restrict__ part) {
+  __shared__ double s_xi[4][6];
+  const int wave = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     double v = xi[q];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(&st->xi_grad[q], v);
+    if ((threadIdx.x & 63) == 0) s_xi[wave][q] = v;
   }
+  __syncthreads();
+  if (threadIdx.x < 6)
+    part[threadIdx.x] = ((s_xi[0][threadIdx.x] + s_xi[1][threadIdx.x]) + s_xi[2][threadIdx.x]) +
+                        s_xi[3][threadIdx.x];
 }
 
 // Pass 1: every pixel with depth != 0 casts its ray (voxel_traversal.cu:
@@ -1408,6 +1451,18 @@ __global__ __launch_bounds__(256) void k_raycast(
 
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
+  constexpr int kTable = MODE == kModeAdjoint ? kHash : 1;
+  __shared__ int s_key[kTable];
+  __shared__ float s_val[kTable];
+  const bool use_table = MODE == kModeAdjoint && io.gF != nullptr;
+  if (use_table) {
+    for (int e = threadIdx.x; e < kTable; e += 256) {
+      s_key[e] = -1;
+      s_val[e] = 0.0f;
+    }
+    __syncthreads();
+  }
+  const Sink sink{io.gF, use_table ? s_key : nullptr, s_val};
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
   int hit = -1;
   bool pending = false;
@@ -1421,11 +1476,11 @@ __global__ __launch_bounds__(256) void k_raycast(
     if (ray_setup(x, y, dval, io.Ki, T, io.vg, io.trunc, r)) {
       r.pix = p;
       pending = !ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, kPass1Batches, ov, on, hit,
-                                                 visited, xi);
+                                                 visited, xi, sink);
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
-  if constexpr (MODE == kModeAdjoint) add_xi(st, xi);
+  if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * blockIdx.x);
   // wave-aggregated append of the unfinished rays
   const uint64_t mask = __ballot(pending);
   if (mask) {
@@ -1434,6 +1489,11 @@ __global__ __launch_bounds__(256) void k_raycast(
     if (ln == leader) base = atomicAdd(&st->ray_queue, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     if (pending) queue[base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
+  }
+  if (use_table) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTable; e += 256)
+      if (s_key[e] >= 0 && s_val[e] != 0.0f) atomicAdd(&io.gF[s_key[e]], s_val[e]);
   }
   if (!active || MODE == kModeAdjoint) return;
   out_v[p] = ov;
@@ -1455,16 +1515,17 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   const uint32_t stride = gridDim.x * 256;
   // with wave reductions at the end of a trip, every lane runs the same
   // number of trips
-  constexpr bool kUniform = COUNT || MODE == kModeAdjoint;
+  constexpr bool kUniform = COUNT;
   const uint32_t trips = kUniform ? (n + stride - 1) / stride : 0;
+  double xi[6] = {0, 0, 0, 0, 0, 0};
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   for (uint32_t t = 0; kUniform ? t < trips : i < n; ++t, i += stride) {
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
-    double xi[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
       RayState r = queue[i];
-      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi);
+      const Sink sink{io.gF, nullptr, nullptr};
+      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink);
       if constexpr (MODE != kModeAdjoint) {
         out_v[r.pix] = ov;
         out_n[r.pix] = on;
@@ -1472,19 +1533,36 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
       }
     }
     if constexpr (COUNT) add_counts(st, visited, hit >= 0);
-    if constexpr (MODE == kModeAdjoint) add_xi(st, xi);
   }
+  if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * (io.part_base + blockIdx.x));
 }
 
 __global__ void k_zero_queue(kf_dev_state* st) {
   if (threadIdx.x == 0) st->ray_queue = 0u;
 }
 
-// Zeroes the adjoint outputs: dL/dxi in the state and the dL/dF array.
-__global__ __launch_bounds__(256) void k_zero_adjoint(kf_dev_state* st, float* gF, int64_t n) {
+// Zeroes n4 float4 of dL/dF.
+__global__ __launch_bounds__(256) void k_zero_adjoint(float4* gF, int64_t n4) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < 6) st->xi_grad[i] = 0.0;
-  if (i < n) gF[i] = 0.0f;
+  if (i < n4) gF[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// dL/dxi = the sum of the n_part block partials, in a fixed order.
+__global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict__ part,
+                                                        int n_part, kf_dev_state* st) {
+  __shared__ double s[256];
+  for (int q = 0; q < 6; ++q) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < n_part; b += 256) v += part[6 * b + q];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) st->xi_grad[q] = s[0];
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -1611,9 +1689,13 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   io.depth = c->depth;
   io.gD = c->adj_depth;
   io.gN = c->adj_normal_set ? c->adj_normal : nullptr;
-  io.gF = c->grad_tsdf;
+  io.gF = c->adj_want_tsdf ? c->grad_tsdf : nullptr;
+  io.xi_part = c->adj_partials;
+  io.part_base = 0;
   return io;
 }
+
+constexpr int kRestBlocks = 256;
 
 template <int PREC, bool COUNT, int MODE>
 static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
@@ -1621,9 +1703,10 @@ static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb
   k_raycast<PREC, COUNT, MODE><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
                                                           c->model_v, c->model_n, c->hit, q,
                                                           tiles_x, nb);
-  k_raycast_rest<PREC, COUNT, MODE><<<256, 256, 0, c->stream>>>(c->tsdf, io, c->state,
-                                                                c->model_v, c->model_n, c->hit,
-                                                                q);
+  RayIO io2 = io;
+  io2.part_base = nb;
+  k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(
+      c->tsdf, io2, c->state, c->model_v, c->model_n, c->hit, q);
 }
 
 template <int PREC>
@@ -1655,10 +1738,22 @@ hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam) {
   return launch_raycast_mode(c, cam, c->prm.raycast_mode == 1 ? kModeTrilinear : kModeReference);
 }
 
+size_t kf_adjoint_partials(const kf_context* c) {
+  return (size_t)(((c->W + 15) / 16) * ((c->H + 15) / 16) + kRestBlocks) * 6;
+}
+
 hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam) {
-  const int64_t n = c->n_store;
-  k_zero_adjoint<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->state, c->grad_tsdf, n);
-  return launch_raycast_mode(c, cam, kModeAdjoint);
+  if (c->adj_want_tsdf) {
+    // n_store is a whole number of 512-voxel bricks
+    const int64_t n4 = c->n_store / 4;
+    k_zero_adjoint<<<(unsigned)((n4 + 255) / 256), 256, 0, c->stream>>>(
+        reinterpret_cast<float4*>(c->grad_tsdf), n4);
+  }
+  hipError_t e = launch_raycast_mode(c, cam, kModeAdjoint);
+  if (e != hipSuccess) return e;
+  k_adjoint_finish<<<1, 256, 0, c->stream>>>(c->adj_partials,
+                                             (int)(kf_adjoint_partials(c) / 6), c->state);
+  return hipGetLastError();
 }
 
 size_t kf_ray_state_bytes() { return sizeof(RayState); }
