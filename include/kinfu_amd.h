@@ -57,6 +57,7 @@ extern "C" {
 #define KF_MAX_ICP_ITERS 64
 #define KF_MAX_BILATERAL_RADIUS 8
 #define KF_ICP_SUMS 28 /* 21 unique J^TJ, 6 J^Tr, inlier count          */
+#define KF_MAX_REFINE_STEPS 64
 
 /* Stage indices for kf_stage_times(). */
 #define KF_STAGE_DEPTH 0     /* uint16 -> float mm + range gate                */
@@ -67,7 +68,8 @@ extern "C" {
 #define KF_STAGE_RAYCAST 5   /* voxel-traversal raycast                       */
 #define KF_STAGE_INTEG_PREP 6 /* integration per-pixel table                  */
 #define KF_STAGE_RAYCAST_ADJOINT 7 /* kf_raycast_adjoint kernels              */
-#define KF_STAGE_COUNT 8
+#define KF_STAGE_REFINE 8    /* pose refinement through the trilinear raycast  */
+#define KF_STAGE_COUNT 9
 
 /* Parameter presets for kf_preset(). */
 #define KF_PRESET_REFERENCE 100 /* the reference's compiled-in defaults          */
@@ -111,6 +113,9 @@ typedef struct kf_params {
                                  test ZX/ZY against voxel_height); 1: test
                                  every index against its own axis        */
   int32_t raycast_mode;     /* KF_RAYCAST_REFERENCE or KF_RAYCAST_TRILINEAR */
+  int32_t refine_steps;     /* pipeline: Gauss-Newton pose refinement steps
+                               after ICP (0 = off, <= KF_MAX_REFINE_STEPS) */
+  float refine_max_residual_mm; /* refinement ignores |residual| above this  */
 } kf_params;
 
 /* Raycast hit handling.  REFERENCE: the reference's per-axis interpolation
@@ -194,6 +199,10 @@ int kf_integrate(kf_context* ctx, const float K[9], const float T[16],
  * contexts widen; F16 weight 0 is returned as NaN, "never observed"). */
 int kf_download_volume(kf_context* ctx, float* tsdf, float* weight);
 int kf_upload_volume(kf_context* ctx, const float* tsdf, const float* weight);
+/* The voxels [origin, origin + size) of the volume, in the reference layout of
+ * that box (x-major, z fastest), converted as kf_download_volume does. */
+int kf_download_volume_box(kf_context* ctx, const int32_t origin[3], const int32_t size[3],
+                           float* tsdf, float* weight);
 /* Native storage, two arrays back to back: all TSDF values, then all
  * weights (F32: float + float; F16: IEEE half bits + uint16).  Each array is
  * brick-tiled: voxel (x, y, z) sits at
@@ -234,6 +243,20 @@ int kf_download_rendered_depth(kf_context* ctx, float* depth_mm);
 int kf_raycast_adjoint(kf_context* ctx, const float K[9], const float T[16],
                        const float* dl_ddepth, const float* dl_dnormal, float* dl_dtsdf,
                        double dl_dxi[6]);
+/* Pose refinement through the differentiable raycast (no reference
+ * counterpart).  Minimises sum_p (depth_p(T) - d_p)^2 over the pixels whose
+ * trilinear raycast hits and whose |residual| <= refine_max_residual_mm,
+ * where d is the context's working depth (the observed frame) and depth(T)
+ * the TSDF rendered at T: `steps` Gauss-Newton steps with J = d depth / d xi
+ * from the raycast's derivative, FP32 sums per block, FP64 sums, solve and
+ * T <- exp(-x^) T.  T_inout (nullable) sets the starting pose and receives
+ * the result; otherwise the context pose is refined in place. */
+int kf_refine_pose(kf_context* ctx, const float K[9], int32_t steps, float T_inout[16]);
+/* Per step of the last refinement: mean squared residual (mm^2) before the
+ * step, pixels used, status (0 solved; 2 too few pixels; 1 or 3 no update).
+ * Any pointer may be NULL. */
+int kf_refine_history(kf_context* ctx, int32_t* n_steps, double* msr, int32_t* count,
+                      int32_t* status);
 
 /* ---- frame sequence (KinfuPipeline, kinfu_pipeline.cpp:91-193) ----------- */
 /* Keeps n frames of uint16 depth resident in HBM for the pipeline calls. */

@@ -18,7 +18,7 @@ KF_MEM_HOST, KF_MEM_DEVICE = 0, 1
 KF_MAX_ICP_ITERS = 64
 KF_ICP_SUMS = 28
 STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep",
-          "raycast_adjoint")
+          "raycast_adjoint", "refine")
 KF_RAYCAST_REFERENCE, KF_RAYCAST_TRILINEAR = 0, 1
 PRESET_REFERENCE = 100
 PRESET_ROOM_CPU = 0       # 640x480, 360x360x300 @10mm FP32, 10-frame CPU case
@@ -48,7 +48,8 @@ class Params(C.Structure):
                 ("icp_dist_mm", C.c_float), ("icp_angle_deg", C.c_float),
                 ("icp_compensated", C.c_int32), ("record_icp", C.c_int32),
                 ("stage_timing", C.c_int32), ("raycast_fix_bounds", C.c_int32),
-                ("raycast_mode", C.c_int32)]
+                ("raycast_mode", C.c_int32), ("refine_steps", C.c_int32),
+                ("refine_max_residual_mm", C.c_float)]
 
 
 class SynthScene(C.Structure):
@@ -107,6 +108,9 @@ _SIGNATURES = {
     "kf_reset_stage_times": (C.c_int, [_P]),
     "kf_count_raycast_work": (C.c_int, [_P, C.c_int32]),
     "kf_download_rendered_depth": (C.c_int, [_P, _F]),
+    "kf_download_volume_box": (C.c_int, [_P, _I32, _I32, _F, _F]),
+    "kf_refine_pose": (C.c_int, [_P, _F, C.c_int32, _F]),
+    "kf_refine_history": (C.c_int, [_P, _I32, _D, _I32, _I32]),
     "kf_raycast_adjoint": (C.c_int, [_P, _F, _F, _F, _F, _F, _D]),
     "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
@@ -443,6 +447,34 @@ class KinfuContext:
 
     def reset_stage_times(self):
         check(lib.kf_reset_stage_times(self.h))
+
+    def volume_box(self, origin, size):
+        """(tsdf, weight) of the voxels [origin, origin+size), reference layout."""
+        o = np.ascontiguousarray(origin, np.int32)
+        z = np.ascontiguousarray(size, np.int32)
+        n = int(np.prod(z))
+        t, w = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        check(lib.kf_download_volume_box(self.h, o.ctypes.data_as(_I32), z.ctypes.data_as(_I32),
+                                         fptr(t), fptr(w)))
+        return t, w
+
+    def refine_pose(self, K, steps, T=None):
+        """Gauss-Newton refinement of the pose on the working depth; returns
+        the refined pose (4x4)."""
+        Tio = _f32(T, 16).copy() if T is not None else None
+        check(lib.kf_refine_pose(self.h, fptr(_f32(K, 9)), int(steps),
+                                 fptr(Tio) if Tio is not None else None))
+        return Tio.reshape(4, 4) if Tio is not None else self.pose()
+
+    def refine_history(self):
+        n = C.c_int32()
+        msr = np.zeros(64, np.float64)
+        cnt = np.zeros(64, np.int32)
+        st = np.zeros(64, np.int32)
+        check(lib.kf_refine_history(self.h, C.byref(n), msr.ctypes.data_as(_D),
+                                    cnt.ctypes.data_as(_I32), st.ctypes.data_as(_I32)))
+        k = n.value
+        return {"msr": msr[:k], "count": cnt[:k], "status": st[:k]}
 
     def rendered_depth(self):
         out = np.zeros((self.height, self.width), np.float32)

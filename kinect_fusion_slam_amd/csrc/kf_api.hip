@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -118,6 +119,11 @@ bool params_valid(const kf_params* p, std::string* why) {
   }
   if (p->raycast_mode != KF_RAYCAST_REFERENCE && p->raycast_mode != KF_RAYCAST_TRILINEAR) {
     *why = "raycast_mode must be KF_RAYCAST_REFERENCE or KF_RAYCAST_TRILINEAR";
+    return false;
+  }
+  if (p->refine_steps < 0 || p->refine_steps > KF_MAX_REFINE_STEPS ||
+      !(p->refine_max_residual_mm > 0)) {
+    *why = "refine_steps must be in [0, 64] and refine_max_residual_mm positive";
     return false;
   }
   return true;
@@ -257,6 +263,8 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   p.stage_timing = 0;
   p.raycast_fix_bounds = 0;
   p.raycast_mode = KF_RAYCAST_REFERENCE;
+  p.refine_steps = 0;
+  p.refine_max_residual_mm = 50.0f;
   switch (which) {
     case KF_PRESET_REFERENCE: {
       // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,
@@ -359,7 +367,7 @@ int kf_destroy(kf_context* c) {
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
-                  c->adj_partials};
+                  c->adj_partials, c->gn_partials};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -459,6 +467,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->spatial, 4 * (2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)},
       {(void**)&c->state, sizeof(kf_dev_state)},
       {(void**)&c->icp_partials, (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 2 * 4},
+      {(void**)&c->gn_partials, kf_refine_partials(c) * 4},
       {&c->volume, c->volume_bytes},
   };
   for (auto& a : allocs) {
@@ -751,6 +760,53 @@ int kf_download_volume(kf_context* c, float* tsdf, float* weight) {
   return KF_OK;
 }
 
+int kf_download_volume_box(kf_context* c, const int32_t origin[3], const int32_t size[3],
+                           float* tsdf, float* weight) {
+  KF_REQUIRE(c && origin && size && (tsdf || weight), KF_ERR_INVALID, "null argument");
+  const int N[3] = {c->vg.nx, c->vg.ny, c->vg.nz};
+  for (int a = 0; a < 3; ++a)
+    KF_REQUIRE(size[a] > 0 && origin[a] >= 0 && origin[a] + size[a] <= N[a], KF_ERR_INVALID,
+               "box outside the volume");
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  // copy the whole bricks that cover the box, one brick column (all z) at a time
+  const int isz = c->vol.precision == KF_TSDF_F32 ? 4 : 2;
+  const int bz0 = origin[2] >> 3, bz1 = (origin[2] + size[2] - 1) >> 3;
+  const size_t col = (size_t)(bz1 - bz0 + 1) * 512;
+  std::vector<uint8_t> T(col * isz), Wt(col * isz);
+  const char* tb = static_cast<const char*>(c->tsdf);
+  const char* wb = static_cast<const char*>(c->weight);
+  for (int bx = origin[0] >> 3; bx <= (origin[0] + size[0] - 1) >> 3; ++bx)
+    for (int by = origin[1] >> 3; by <= (origin[1] + size[1] - 1) >> 3; ++by) {
+      const size_t first = (((size_t)bx * c->vg.by + by) * c->vg.bz + bz0) * 512;
+      if (tsdf) KF_HIP_CHECK(hipMemcpy(T.data(), tb + first * isz, col * isz, hipMemcpyDeviceToHost));
+      if (weight)
+        KF_HIP_CHECK(hipMemcpy(Wt.data(), wb + first * isz, col * isz, hipMemcpyDeviceToHost));
+      for (int x = std::max(bx * 8, origin[0]); x < std::min(bx * 8 + 8, origin[0] + size[0]); ++x)
+        for (int y = std::max(by * 8, origin[1]); y < std::min(by * 8 + 8, origin[1] + size[1]);
+             ++y)
+          for (int z = origin[2]; z < origin[2] + size[2]; ++z) {
+            const size_t s = (size_t)kf_store_index(c->vg, x, y, z) - first;
+            const size_t o = ((size_t)(x - origin[0]) * size[1] + (y - origin[1])) * size[2] +
+                             (z - origin[2]);
+            if (c->vol.precision == KF_TSDF_F32) {
+              if (tsdf) memcpy(&tsdf[o], &T[s * 4], 4);
+              if (weight) memcpy(&weight[o], &Wt[s * 4], 4);
+            } else {
+              uint16_t t16, w16;
+              if (tsdf) {
+                memcpy(&t16, &T[s * 2], 2);
+                tsdf[o] = kf_f16_bits_to_f32(t16);
+              }
+              if (weight) {
+                memcpy(&w16, &Wt[s * 2], 2);
+                weight[o] = w16 ? (float)w16 : NAN;
+              }
+            }
+          }
+    }
+  return KF_OK;
+}
+
 int kf_upload_volume(kf_context* c, const float* tsdf, const float* weight) {
   KF_REQUIRE(c && tsdf && weight, KF_ERR_INVALID, "null argument");
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -834,6 +890,40 @@ int kf_download_model_maps(kf_context* c, float* x, float* y, float* z, float* n
     if (ny) ny[i] = N[i].y;
     if (nz) nz[i] = N[i].z;
   }
+  return KF_OK;
+}
+
+int kf_refine_pose(kf_context* c, const float K[9], int32_t steps, float T_inout[16]) {
+  KF_REQUIRE(c && K && steps >= 0 && steps <= KF_MAX_REFINE_STEPS, KF_ERR_INVALID,
+             "bad argument");
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  if (T_inout) {
+    int rc = kf_set_pose(c, T_inout);
+    if (rc != KF_OK) return rc;
+  }
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  kf_stage_begin(c, KF_STAGE_REFINE);
+  KF_LAUNCH(kf_launch_refine(c, cam, steps));
+  kf_stage_end(c, KF_STAGE_REFINE);
+  c->refine_steps_done = steps;
+  if (T_inout) return kf_get_pose(c, T_inout);
+  return KF_OK;
+}
+
+int kf_refine_history(kf_context* c, int32_t* n_steps, double* msr, int32_t* count,
+                      int32_t* status) {
+  KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  const int n = c->refine_steps_done;
+  if (n_steps) *n_steps = n;
+  if (n == 0) return KF_OK;
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (msr)
+    KF_HIP_CHECK(hipMemcpy(msr, c->state->refine_msr, n * 8, hipMemcpyDeviceToHost));
+  if (count)
+    KF_HIP_CHECK(hipMemcpy(count, c->state->refine_count, n * 4, hipMemcpyDeviceToHost));
+  if (status)
+    KF_HIP_CHECK(hipMemcpy(status, c->state->refine_status, n * 4, hipMemcpyDeviceToHost));
   return KF_OK;
 }
 
@@ -977,6 +1067,12 @@ int kf_pipeline_incoming(kf_context* c, const float K[9], int32_t frame) {
   kf_stage_begin(c, KF_STAGE_ICP);
   KF_LAUNCH(kf_launch_icp(c, cam));
   kf_stage_end(c, KF_STAGE_ICP);
+  if (c->prm.refine_steps > 0) {
+    kf_stage_begin(c, KF_STAGE_REFINE);
+    KF_LAUNCH(kf_launch_refine(c, cam, c->prm.refine_steps));
+    kf_stage_end(c, KF_STAGE_REFINE);
+    c->refine_steps_done = c->prm.refine_steps;
+  }
   if ((rc = integrate_at_current_pose(c, cam, 0)) != KF_OK) return rc;
   if ((rc = raycast_at_current_pose(c, cam)) != KF_OK) return rc;
   return record_pose(c, frame);

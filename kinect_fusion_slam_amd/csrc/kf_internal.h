@@ -25,6 +25,9 @@ struct kf_dev_state {
   unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
   unsigned long long ray_hits;
   double xi_grad[6];             // raycast adjoint: dL/dxi (omega, v)
+  double refine_msr[KF_MAX_REFINE_STEPS];     // refinement: mean squared residual
+  int32_t refine_count[KF_MAX_REFINE_STEPS];  //   pixels used
+  int32_t refine_status[KF_MAX_REFINE_STEPS]; //   0 solved, else no update
   float iter_T[KF_MAX_ICP_ITERS][16];
   double iter_sums[KF_MAX_ICP_ITERS][KF_ICP_SUMS];
   int32_t iter_status[KF_MAX_ICP_ITERS];
@@ -125,6 +128,8 @@ struct kf_context {
   bool adj_normal_set = false;
   bool adj_want_tsdf = false;   // the current adjoint call returns dL/dTSDF
   double* adj_partials = nullptr; // dL/dxi block partials
+  float* gn_partials = nullptr;   // refinement block partials
+  int refine_steps_done = 0;      // steps of the last refinement
 };
 
 // error reporting ---------------------------------------------------------
@@ -147,6 +152,8 @@ hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first);
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam);
 size_t kf_adjoint_partials(const kf_context* c);  // doubles
+size_t kf_refine_partials(const kf_context* c);   // floats
+hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps);
 hipError_t kf_launch_volume_fill(kf_context* c);
 hipError_t kf_launch_set_pose(kf_context* c, const float* T_host);
 hipError_t kf_launch_record_pose(kf_context* c, int frame);

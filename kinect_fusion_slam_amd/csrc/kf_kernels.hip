@@ -966,6 +966,8 @@ struct RayIO {
   float* gF;           // dL/d(TSDF), native store order (null: not wanted)
   double* xi_part;     // dL/dxi partials, 6 per block
   int part_base;       // first partial slot of this launch
+  float* gn_part;      // refinement partials, kGnSums per block
+  float max_res;       // refinement: residuals beyond this (mm) are ignored
 };
 
 // Raycast modes: 0 = the reference's hit handling (voxel_traversal.cu:222-
@@ -974,6 +976,10 @@ struct RayIO {
 constexpr int kModeReference = 0;
 constexpr int kModeTrilinear = 1;
 constexpr int kModeAdjoint = 2;
+// 3 = Gauss-Newton pose refinement on the depth residual of mode 1: per
+// pixel J = d(depth)/d(xi) and r = depth - observed depth.
+constexpr int kModeRefine = 3;
+constexpr int kGnSums = 29;  // 21 J^TJ (upper, row-major), 6 J^Tr, count, r^2
 
 // Ray setup (voxel_traversal.cu:90-200): ray skipping to (depth - trunc),
 // slab test, 1-based start voxel, steps and boundary distances.  Returns
@@ -1212,7 +1218,8 @@ __device__ __forceinline__ void tri_scatter(const Sink& sink, const kf_vol& vg, 
 template <int PREC, int MODE>
 __device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, const float* T,
                                      int pix, float te, float dx, float dy, float dz,
-                                     float4& ov, float4& on, double* xi, const Sink& sink) {
+                                     float4& ov, float4& on, double* xi, const Sink& sink,
+                                     float* acc) {
   const kf_vol& vg = io.vg;
   const float* Ki = io.Ki;
   const int x = pix % io.W, y = pix / io.W;
@@ -1258,6 +1265,29 @@ __device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, co
   if constexpr (MODE == kModeTrilinear) {
     ov = make_float4(psx, psy, psz, sc + ts / lam);
     on = make_float4(nx, ny, nz, 0.0f);
+  } else if constexpr (MODE == kModeRefine) {
+    const float r = (sc + ts / lam) - dval;
+    if (!(fabsf(r) <= io.max_res)) return true;
+    const float den2 = den * den;
+    const float dts = 1.0f / lam;
+    const float dFa = dts * d * (-b.v) / den2;
+    const float dFb = dts * d * a.v / den2;
+    const float tb = ta + d;
+    const float pax = sgx + ta * dx, pay = sgy + ta * dy, paz = sgz + ta * dz;
+    const float pbx = sgx + tb * dx, pby = sgy + tb * dy, pbz = sgz + tb * dz;
+    float J[6];
+    J[0] = dFa * (pay * a.g[2] - paz * a.g[1]) + dFb * (pby * b.g[2] - pbz * b.g[1]);
+    J[1] = dFa * (paz * a.g[0] - pax * a.g[2]) + dFb * (pbz * b.g[0] - pbx * b.g[2]);
+    J[2] = dFa * (pax * a.g[1] - pay * a.g[0]) + dFb * (pbx * b.g[1] - pby * b.g[0]);
+    J[3] = dFa * a.g[0] + dFb * b.g[0];
+    J[4] = dFa * a.g[1] + dFb * b.g[1];
+    J[5] = dFa * a.g[2] + dFb * b.g[2];
+    int q = 0;
+    for (int i = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j) acc[q++] += J[i] * J[j];
+    for (int i = 0; i < 6; ++i) acc[21 + i] += J[i] * r;
+    acc[27] += 1.0f;
+    acc[28] += r * r;
   } else {
     const float gD = io.gD[pix];
     const float4 gNv = io.gN ? io.gN[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1321,7 +1351,7 @@ template <int PREC, int kRayBatch, bool COUNT, int MODE>
 __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, const RayIO& io,
                                          const float* T, int max_batches, float4& ov,
                                          float4& on, int& hit, int& visited, double* xi,
-                                         const Sink& sink) {
+                                         const Sink& sink, float* gn) {
   const kf_vol& vg = io.vg;
   const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
   for (int bt = 0; bt < max_batches; ++bt) {
@@ -1375,7 +1405,7 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
                              io.flip_hi, ov, on);
       } else {
         const bool ok =
-            tri_hit<PREC, MODE>(F, io, T, r.pix, te, r.dx, r.dy, r.dz, ov, on, xi, sink);
+            tri_hit<PREC, MODE>(F, io, T, r.pix, te, r.dx, r.dy, r.dz, ov, on, xi, sink, gn);
         hit = ok ? (h.ix - 1) * (Ny * Nz) + (h.iy - 1) * Nz + (h.iz - 1) : -1;
       }
     }
@@ -1414,6 +1444,22 @@ __device__ __forceinline__ void block_xi(const double* xi, double* __restrict__ 
   if (threadIdx.x < 6)
     part[threadIdx.x] = ((s_xi[0][threadIdx.x] + s_xi[1][threadIdx.x]) + s_xi[2][threadIdx.x]) +
                         s_xi[3][threadIdx.x];
+}
+
+// Refinement: the block's kGnSums partial sums (FP32) into its slot of part.
+__device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ part) {
+  __shared__ float s_gn[4][kGnSums];
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kGnSums; ++q) {
+    float v = gn[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) s_gn[wave][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kGnSums)
+    part[threadIdx.x] = ((s_gn[0][threadIdx.x] + s_gn[1][threadIdx.x]) + s_gn[2][threadIdx.x]) +
+                        s_gn[3][threadIdx.x];
 }
 
 // Pass 1: every pixel with depth != 0 casts its ray (voxel_traversal.cu:
@@ -1468,6 +1514,7 @@ __global__ __launch_bounds__(256) void k_raycast(
   bool pending = false;
   int visited = 0;
   double xi[6] = {0, 0, 0, 0, 0, 0};
+  float gn[MODE == kModeRefine ? kGnSums : 1] = {};
   RayState r;
   float T[12];
   load_uniform<12>(st->T, T);
@@ -1476,11 +1523,12 @@ __global__ __launch_bounds__(256) void k_raycast(
     if (ray_setup(x, y, dval, io.Ki, T, io.vg, io.trunc, r)) {
       r.pix = p;
       pending = !ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, kPass1Batches, ov, on, hit,
-                                                 visited, xi, sink);
+                                                 visited, xi, sink, gn);
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * blockIdx.x);
+  if constexpr (MODE == kModeRefine) block_gn(gn, io.gn_part + kGnSums * blockIdx.x);
   // wave-aggregated append of the unfinished rays
   const uint64_t mask = __ballot(pending);
   if (mask) {
@@ -1495,7 +1543,7 @@ __global__ __launch_bounds__(256) void k_raycast(
     for (int e = threadIdx.x; e < kTable; e += 256)
       if (s_key[e] >= 0 && s_val[e] != 0.0f) atomicAdd(&io.gF[s_key[e]], s_val[e]);
   }
-  if (!active || MODE == kModeAdjoint) return;
+  if (!active || MODE == kModeAdjoint || MODE == kModeRefine) return;
   out_v[p] = ov;
   out_n[p] = on;
   out_hit[p] = hit;
@@ -1518,6 +1566,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   constexpr bool kUniform = COUNT;
   const uint32_t trips = kUniform ? (n + stride - 1) / stride : 0;
   double xi[6] = {0, 0, 0, 0, 0, 0};
+  float gn[MODE == kModeRefine ? kGnSums : 1] = {};
   uint32_t i = blockIdx.x * 256 + threadIdx.x;
   for (uint32_t t = 0; kUniform ? t < trips : i < n; ++t, i += stride) {
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1525,8 +1574,9 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     if (i < n) {
       RayState r = queue[i];
       const Sink sink{io.gF, nullptr, nullptr};
-      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink);
-      if constexpr (MODE != kModeAdjoint) {
+      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink,
+                                      gn);
+      if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
         out_v[r.pix] = ov;
         out_n[r.pix] = on;
         out_hit[r.pix] = hit;
@@ -1535,6 +1585,8 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   }
   if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * (io.part_base + blockIdx.x));
+  if constexpr (MODE == kModeRefine)
+    block_gn(gn, io.gn_part + kGnSums * (io.part_base + blockIdx.x));
 }
 
 __global__ void k_zero_queue(kf_dev_state* st) {
@@ -1565,6 +1617,79 @@ __global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict
   }
 }
 
+
+// One Gauss-Newton step of the pose refinement: the FP64 sums of the block
+// partials, the 6x6 solve (LDLT, FP64) and T <- exp(-x^) T in FP64, with
+// x = (omega, v) the solution of (J^T J) x = J^T r.  Records the mean squared
+// residual before the step, the pixel count and the solver status.
+__global__ __launch_bounds__(64) void k_refine_finish(const float* __restrict__ part,
+                                                      int n_part, kf_dev_state* st, int step) {
+  __shared__ double s[kGnSums];
+  const int q = threadIdx.x;
+  if (q < kGnSums) {
+    double v = 0.0;
+    for (int b = 0; b < n_part; ++b) v += (double)part[kGnSums * b + q];
+    s[q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double A[36], rhs[6], x[6] = {0, 0, 0, 0, 0, 0};
+  int k = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j, ++k) A[i * 6 + j] = A[j * 6 + i] = s[k];
+  for (int i = 0; i < 6; ++i) rhs[i] = s[21 + i];
+  int status = s[27] >= 6.0 ? kf_ldlt6_solve(A, rhs, x) : 2;
+  if (status != 0)
+    for (int i = 0; i < 6; ++i) x[i] = 0.0;
+  // exp of the twist (-x): Rodrigues rotation and the left Jacobian V
+  const double w0 = -x[0], w1 = -x[1], w2 = -x[2];
+  const double th2 = w0 * w0 + w1 * w1 + w2 * w2, th = sqrt(th2);
+  double a, b, c;
+  if (th < 1e-9) {
+    a = 1.0 - th2 / 6.0;
+    b = 0.5 - th2 / 24.0;
+    c = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    a = sin(th) / th;
+    b = (1.0 - cos(th)) / th2;
+    c = (th - sin(th)) / (th2 * th);
+  }
+  const double W[9] = {0.0, -w2, w1, w2, 0.0, -w0, -w1, w0, 0.0};
+  double W2[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      W2[i * 3 + j] = W[i * 3] * W[j] + W[i * 3 + 1] * W[3 + j] + W[i * 3 + 2] * W[6 + j];
+  double E[16] = {0};
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) E[i * 4 + j] = (i == j ? 1.0 : 0.0) + a * W[i * 3 + j] + b * W2[i * 3 + j];
+    double t = 0.0;
+    for (int j = 0; j < 3; ++j)
+      t += ((i == j ? 1.0 : 0.0) + b * W[i * 3 + j] + c * W2[i * 3 + j]) * (-x[3 + j]);
+    E[i * 4 + 3] = t;
+  }
+  E[15] = 1.0;
+  float Tn[16];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double v = 0.0;
+      for (int m = 0; m < 4; ++m) v += E[i * 4 + m] * (double)st->T[m * 4 + j];
+      Tn[i * 4 + j] = (float)v;
+    }
+  float Ti[16];
+  if (kf_m4_inverse(Tn, Ti) == 0) {
+    for (int i = 0; i < 16; ++i) {
+      st->T[i] = Tn[i];
+      st->Tinv[i] = Ti[i];
+    }
+  } else {
+    status = 3;
+  }
+  if (step < KF_MAX_REFINE_STEPS) {
+    st->refine_msr[step] = s[28] / (s[27] > 0.0 ? s[27] : 1.0);
+    st->refine_count[step] = (int32_t)s[27];
+    st->refine_status[step] = status;
+  }
+}
 }  // namespace
 
 // ===========================================================================
@@ -1692,6 +1817,8 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   io.gF = c->adj_want_tsdf ? c->grad_tsdf : nullptr;
   io.xi_part = c->adj_partials;
   io.part_base = 0;
+  io.gn_part = c->gn_partials;
+  io.max_res = c->prm.refine_max_residual_mm;
   return io;
 }
 
@@ -1714,6 +1841,8 @@ static void launch_raycast_p(kf_context* c, const RayIO& io, int tiles_x, int nb
                              int mode) {
   if (mode == kModeAdjoint)
     launch_raycast_t<PREC, false, kModeAdjoint>(c, io, tiles_x, nb);
+  else if (mode == kModeRefine)
+    launch_raycast_t<PREC, false, kModeRefine>(c, io, tiles_x, nb);
   else if (mode == kModeTrilinear)
     count ? launch_raycast_t<PREC, true, kModeTrilinear>(c, io, tiles_x, nb)
           : launch_raycast_t<PREC, false, kModeTrilinear>(c, io, tiles_x, nb);
@@ -1736,6 +1865,20 @@ static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode
 
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam) {
   return launch_raycast_mode(c, cam, c->prm.raycast_mode == 1 ? kModeTrilinear : kModeReference);
+}
+
+hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps) {
+  const int n_part = ((c->W + 15) / 16) * ((c->H + 15) / 16) + kRestBlocks;
+  for (int it = 0; it < steps; ++it) {
+    hipError_t e = launch_raycast_mode(c, cam, kModeRefine);
+    if (e != hipSuccess) return e;
+    k_refine_finish<<<1, 64, 0, c->stream>>>(c->gn_partials, n_part, c->state, it);
+  }
+  return hipGetLastError();
+}
+
+size_t kf_refine_partials(const kf_context* c) {
+  return (size_t)(((c->W + 15) / 16) * ((c->H + 15) / 16) + kRestBlocks) * kGnSums;
 }
 
 size_t kf_adjoint_partials(const kf_context* c) {
