@@ -1473,6 +1473,10 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 // to the 8 XCDs; the id is remapped so that each XCD gets a contiguous band
 // of the image and its L2 sees the band's voxels.
 constexpr int kPass1Batches = 2;  // of 16 steps
+#ifndef KF_REST_BATCH
+#define KF_REST_BATCH 16
+#endif
+constexpr int kRestBatch = KF_REST_BATCH;  // steps per batch in pass 2
 
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast(
@@ -1574,7 +1578,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     if (i < n) {
       RayState r = queue[i];
       const Sink sink{io.gF, nullptr, nullptr};
-      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink,
+      ray_walk<PREC, kRestBatch, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink,
                                       gn);
       if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
         out_v[r.pix] = ov;
