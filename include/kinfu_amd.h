@@ -227,6 +227,17 @@ int kf_download_model_maps(kf_context* ctx, float* x, float* y, float* z,
 int kf_upload_model_maps(kf_context* ctx, const float* x, const float* y,
                          const float* z, const float* nx, const float* ny,
                          const float* nz);
+/* Point cloud of the last raycast as a PLY file (the reference builds the
+ * same cloud for its PCL viewer, threeD_viewer.cpp:20-55): one vertex per
+ * pixel whose model z is non-zero, with its normal, and with colour from a
+ * H x W x 3 B,G,R image when bgr is non-NULL.  binary: 1 little-endian
+ * binary, 0 ASCII. */
+int kf_export_model_ply(kf_context* ctx, const char* path, const uint8_t* bgr, int32_t binary);
+/* The writer on host arrays of n points (x, y, z required; normals and
+ * colour nullable); points with z == 0 are skipped. */
+int kf_write_ply(const char* path, int64_t n, const float* x, const float* y, const float* z,
+                 const float* nx, const float* ny, const float* nz, const uint8_t* bgr,
+                 int32_t binary);
 /* Camera-frame depth (mm) of the raycast vertices, 0 for no hit; written by
  * KF_RAYCAST_TRILINEAR raycasts (0 everywhere after a REFERENCE raycast). */
 int kf_download_rendered_depth(kf_context* ctx, float* depth_mm);

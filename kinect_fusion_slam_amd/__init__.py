@@ -11,8 +11,8 @@ from ._abi import (KF_MEM_DEVICE, KF_MEM_HOST, KF_RAYCAST_REFERENCE,  # noqa: F4
                    PRESET_ROOM_5MM_F16, PRESET_ROOM_CPU, PRESET_ROOM_HD_F16, PRESET_SPHERES_128,
                    PRESETS, STAGES, KinfuContext, KinfuError, Params, SynthScene,
                    VolumeDesc, exported_symbols, gate_thresholds, ldlt6, lib, mat3_inverse,
-                   mat4_inverse, orbit_pose, preset, preset_scene, render, spatial_kernel,
-                   stream)
+                   mat4_inverse, orbit_pose, preset, preset_scene, read_ply, render,
+                   spatial_kernel, stream, write_ply)
 
 __all__ = ["KinfuContext", "KinfuError", "VolumeDesc", "Params", "SynthScene", "preset",
            "preset_scene", "orbit_pose", "render", "stream", "gate_thresholds", "mat4_inverse",

@@ -110,6 +110,8 @@ _SIGNATURES = {
     "kf_download_rendered_depth": (C.c_int, [_P, _F]),
     "kf_download_volume_box": (C.c_int, [_P, _I32, _I32, _F, _F]),
     "kf_refine_pose": (C.c_int, [_P, _F, C.c_int32, _F]),
+    "kf_export_model_ply": (C.c_int, [_P, C.c_char_p, _U8, C.c_int32]),
+    "kf_write_ply": (C.c_int, [C.c_char_p, C.c_int64, _F, _F, _F, _F, _F, _F, _U8, C.c_int32]),
     "kf_refine_history": (C.c_int, [_P, _I32, _D, _I32, _I32]),
     "kf_raycast_adjoint": (C.c_int, [_P, _F, _F, _F, _F, _F, _D]),
     "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
@@ -207,6 +209,36 @@ def spatial_kernel(radius, sigma):
     w = np.zeros((2 * radius + 1) ** 2, np.float32)
     check(lib.kf_spatial_kernel(radius, sigma, fptr(w)))
     return w.reshape(2 * radius + 1, 2 * radius + 1)
+
+
+def write_ply(path, x, y, z, nx=None, ny=None, nz=None, bgr=None, binary=True):
+    x, y, z = (_f32(a) for a in (x, y, z))
+    n = x.size
+    nrm = [_f32(a) for a in (nx, ny, nz)] if nx is not None else None
+    c = np.ascontiguousarray(bgr, np.uint8) if bgr is not None else None
+    check(lib.kf_write_ply(str(path).encode(), n, fptr(x), fptr(y), fptr(z),
+                           *((fptr(a) for a in nrm) if nrm else (None, None, None)),
+                           c.ctypes.data_as(_U8) if c is not None else None,
+                           1 if binary else 0))
+
+
+def read_ply(path):
+    """Minimal reader for the files write_ply produces: dict of arrays."""
+    with open(path, "rb") as f:
+        head = b""
+        while not head.endswith(b"end_header\n"):
+            head += f.readline()
+        body = f.read()
+    lines = head.decode().splitlines()
+    n = int([l for l in lines if l.startswith("element vertex")][0].split()[-1])
+    props = [(l.split()[1], l.split()[2]) for l in lines if l.startswith("property")]
+    dt = np.dtype([(name, "<f4" if t == "float" else "u1") for t, name in props])
+    if "binary_little_endian" in lines[1]:
+        arr = np.frombuffer(body, dt, count=n)
+    else:
+        arr = np.loadtxt(body.decode().splitlines(), ndmin=2) if n else np.zeros((0, len(props)))
+        arr = np.array([tuple(r) for r in arr], dtype=dt)
+    return {name: arr[name] for _, name in props}
 
 
 # ---------------------------------------------------------------- synthetic stream
@@ -475,6 +507,12 @@ class KinfuContext:
                                     cnt.ctypes.data_as(_I32), st.ctypes.data_as(_I32)))
         k = n.value
         return {"msr": msr[:k], "count": cnt[:k], "status": st[:k]}
+
+    def export_ply(self, path, bgr=None, binary=True):
+        c = np.ascontiguousarray(bgr, np.uint8) if bgr is not None else None
+        check(lib.kf_export_model_ply(self.h, str(path).encode(),
+                                      c.ctypes.data_as(_U8) if c is not None else None,
+                                      1 if binary else 0))
 
     def rendered_depth(self):
         out = np.zeros((self.height, self.width), np.float32)

@@ -927,6 +927,18 @@ int kf_refine_history(kf_context* c, int32_t* n_steps, double* msr, int32_t* cou
   return KF_OK;
 }
 
+int kf_export_model_ply(kf_context* c, const char* path, const uint8_t* bgr, int32_t binary) {
+  KF_REQUIRE(c && path, KF_ERR_INVALID, "null argument");
+  const size_t n = (size_t)c->H * c->W;
+  std::vector<float> P(6 * n);
+  int rc = kf_download_model_maps(c, &P[0], &P[n], &P[2 * n], &P[3 * n], &P[4 * n], &P[5 * n],
+                                  nullptr);
+  if (rc != KF_OK) return rc;
+  rc = kf_write_ply(path, (int64_t)n, &P[0], &P[n], &P[2 * n], &P[3 * n], &P[4 * n], &P[5 * n],
+                    bgr, binary);
+  return rc == KF_OK ? KF_OK : kf_fail(rc, "cannot write the PLY file");
+}
+
 int kf_download_rendered_depth(kf_context* c, float* depth_mm) {
   KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
   const size_t n = (size_t)c->H * c->W;

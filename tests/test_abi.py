@@ -97,3 +97,30 @@ def test_spatial_kernel_same_as_oracle_and_numpy(kf, oracle):
         np.testing.assert_array_equal(w, oracle.spatial_kernel(r, s))
         yy, xx = np.mgrid[-r:r + 1, -r:r + 1]
         np.testing.assert_allclose(w, np.exp(-(xx ** 2 + yy ** 2) / (2.0 * s * s)), rtol=3e-7)
+
+
+def test_ply_writer_roundtrip(kf, tmp_path):
+    # threeD_viewer.cpp:20-55 keeps pixels whose z != 0 and reads colour as B, G, R
+    rng = np.random.default_rng(9)
+    n = 500
+    x, y, z = (rng.normal(size=n).astype(np.float32) * 100 for _ in range(3))
+    z[::7] = 0.0
+    nx, ny, nz = (rng.normal(size=n).astype(np.float32) for _ in range(3))
+    bgr = rng.integers(0, 256, (n, 3)).astype(np.uint8)
+    keep = z != 0
+    for binary in (True, False):
+        p = tmp_path / f"cloud_{int(binary)}.ply"
+        kf.write_ply(p, x, y, z, nx, ny, nz, bgr, binary=binary)
+        r = kf.read_ply(p)
+        assert r["x"].size == keep.sum()
+        for k, v in (("x", x), ("y", y), ("z", z), ("nx", nx), ("ny", ny), ("nz", nz)):
+            np.testing.assert_array_equal(r[k], v[keep])
+        np.testing.assert_array_equal(r["red"], bgr[keep, 2])
+        np.testing.assert_array_equal(r["blue"], bgr[keep, 0])
+    p = tmp_path / "xyz.ply"
+    kf.write_ply(p, x, y, z)
+    r = kf.read_ply(p)
+    assert sorted(r) == ["x", "y", "z"] and r["z"].size == keep.sum()
+    assert kf.lib.kf_write_ply(str(tmp_path / "no" / "dir.ply").encode(), n,
+                               kf._abi.fptr(x), kf._abi.fptr(y), kf._abi.fptr(z),
+                               None, None, None, None, 1) == -5

@@ -44,7 +44,7 @@ def make_ctx(kf, precision=None, record_icp=0):
     return kf.KinfuContext(vol, prm, mg.H, mg.W)
 
 
-def test_golden_stages_f32(kf, golden):
+def test_golden_stages_f32(kf, golden, tmp_path):
     g = golden
     K, gt = g["K"], g["gt"]
     say("[gpu] golden case: depth gate, bilateral, vertex maps")
@@ -72,6 +72,14 @@ def test_golden_stages_f32(kf, golden):
         m = c.model_maps()
         for k in ("x", "y", "z", "nx", "ny", "nz", "hit"):
             assert_same(m[k], g["raycast1_" + k], "raycast " + k)
+        # the point cloud of that raycast
+        bgr = np.full((mg.H, mg.W, 3), (10, 20, 30), np.uint8)
+        c.export_ply(tmp_path / "model.ply", bgr)
+        ply = kf.read_ply(tmp_path / "model.ply")
+        keep = g["raycast1_z"] != 0
+        for k in ("x", "y", "z", "nx", "ny", "nz"):
+            assert_same(ply[k], g["raycast1_" + k][keep], "ply " + k)
+        assert np.all(ply["red"] == 30) and np.all(ply["blue"] == 10)
         say("[gpu] golden case: ICP of frame 2")
         c.upload_depth_u16(g["frames"][2])
         c.bilateral()
