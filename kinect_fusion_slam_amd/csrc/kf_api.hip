@@ -1038,15 +1038,18 @@ int kf_stream_upload_u16(kf_context* c, const uint16_t* frames, int32_t n_frames
 static int load_stream_frame(kf_context* c, int32_t frame) {
   KF_REQUIRE(c->frames && frame >= 0 && frame < c->n_frames, KF_ERR_INVALID,
              "frame outside the uploaded stream");
-  kf_stage_begin(c, KF_STAGE_DEPTH);
-  KF_LAUNCH(kf_launch_depth_u16(c, c->frames + (size_t)frame * c->H * c->W));
-  kf_stage_end(c, KF_STAGE_DEPTH);
-  c->have_depth = true;
+  const uint16_t* src = c->frames + (size_t)frame * c->H * c->W;
   if (c->prm.bilateral) {
+    // range gate and filter in one kernel
     kf_stage_begin(c, KF_STAGE_BILATERAL);
-    KF_LAUNCH(kf_launch_bilateral(c));
+    KF_LAUNCH(kf_launch_depth_bilateral(c, src));
     kf_stage_end(c, KF_STAGE_BILATERAL);
+  } else {
+    kf_stage_begin(c, KF_STAGE_DEPTH);
+    KF_LAUNCH(kf_launch_depth_u16(c, src));
+    kf_stage_end(c, KF_STAGE_DEPTH);
   }
+  c->have_depth = true;
   return KF_OK;
 }
 

@@ -20,8 +20,9 @@ struct kf_dev_state {
   float F[16];          // Tprev_inv * T (frame_to_frame, sensor_pose_estimation.cu:343)
   float center_dir[4];  // integration front-test direction (projective_tsdf.cu:57-71)
   int32_t icp_iters_done;
-  uint32_t ray_queue;   // rays queued by the first raycast pass
-  int32_t pad[2];
+  uint32_t ray_queue[2];  // rays queued by the first raycast pass, two slots used
+                          // by alternate raycasts (each clears the other's)
+  int32_t pad;
   unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
   unsigned long long ray_hits;
   double xi_grad[6];             // raycast adjoint: dL/dxi (omega, v)
@@ -64,7 +65,9 @@ __host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int
 }
 
 // ICP reduction geometry.
-#define KF_ICP_BLOCK 1024
+#ifndef KF_ICP_BLOCK
+#define KF_ICP_BLOCK 512
+#endif
 // Block partials one lane reads per row when an iteration is finished: the
 // pixels per thread (1, 2 or 4) are chosen so that at most 64 * 8 blocks
 // cover the image (up to 2^21 pixels).
@@ -121,6 +124,7 @@ struct kf_context {
   int64_t stage_count[KF_STAGE_COUNT] = {};
   bool have_depth = false;
   bool count_rays = false;  // raycast work counters on
+  int ray_slot = 0;         // ray_queue slot of the next raycast
   // raycast adjoint buffers (allocated on first use)
   float* grad_tsdf = nullptr;   // dL/dF, n_store floats, native order
   float* adj_depth = nullptr;   // dL/d(rendered depth), H*W
@@ -145,6 +149,7 @@ int kf_fail(int code, const std::string& msg);
 // launchers (each enqueues on ctx->stream and returns hipGetLastError) ------
 hipError_t kf_launch_depth_u16(kf_context* c, const uint16_t* src);
 hipError_t kf_launch_bilateral(kf_context* c);
+hipError_t kf_launch_depth_bilateral(kf_context* c, const uint16_t* src);
 hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);

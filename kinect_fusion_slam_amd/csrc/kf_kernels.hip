@@ -124,11 +124,13 @@ __global__ __launch_bounds__(256) void k_depth_u16(const uint16_t* __restrict__ 
 // column past the right edge wraps to the next row as it does there; indices
 // outside the image read 0.  Pixels outside [R, H-R] x [R, W-R] are written
 // as 0 (the reference's zero-initialised output replaces the depth).
-template <int R>
-__global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ in,
+// U16: the input is the raw uint16 frame and the range gate of k_depth_u16
+// is applied as the tile is loaded (the gated frame itself is not stored).
+template <int R, bool U16>
+__global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
                                                    float* __restrict__ out,
                                                    const float* __restrict__ ws, int H, int W,
-                                                   float sigma_r) {
+                                                   float sigma_r, float dmin, float dmax) {
   constexpr int TS = 16, L = 2 * R + 1, SW = TS + 2 * R;
   __shared__ float tile[SW][SW + 1];
   __shared__ float sw[L * L];
@@ -138,7 +140,16 @@ __global__ __launch_bounds__(256) void k_bilateral(const float* __restrict__ in,
   for (int i = tid; i < SW * SW; i += 256) {
     const int r = i / SW, cc = i - r * SW;
     const int idx = (y0 - R + r) * W + (x0 - R + cc);
-    tile[r][cc] = (idx >= 0 && idx < n) ? in[idx] : 0.0f;
+    float v = 0.0f;
+    if (idx >= 0 && idx < n) {
+      if constexpr (U16) {
+        const float d = (float)static_cast<const uint16_t*>(in_)[idx];
+        v = (d >= dmin && d <= dmax) ? d : 0.0f;
+      } else {
+        v = static_cast<const float*>(in_)[idx];
+      }
+    }
+    tile[r][cc] = v;
   }
   for (int i = tid; i < L * L; i += 256) sw[i] = ws[i];
   __syncthreads();
@@ -261,11 +272,11 @@ struct IcpShared {
 // those rows at once (one memory round trip), adds its up to
 // KF_ICP_MAX_PART_PER_LANE entries of each row in FP64, and the 64 lane sums
 // of a row are then added in lane order by 4 threads of 16 lanes each.
-template <int ROWS>
+template <int ROWS, int NT>
 __device__ __forceinline__ void icp_reduce_partials(const float* __restrict__ P, int nb,
                                                     IcpShared<ROWS>& sh) {
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
-  constexpr int NW = KF_ICP_BLOCK / 64;
+  constexpr int NW = NT / 64;
   constexpr int RPW = (ROWS + NW - 1) / NW;  // rows per wave
   float v[RPW][KF_ICP_MAX_PART_PER_LANE];
 #pragma unroll
@@ -369,7 +380,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       sh.T0[threadIdx.x] = iter == 1 ? st->Tprev[threadIdx.x] : st->iter_T[iter - 2][threadIdx.x];
       sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
     }
-    icp_reduce_partials<ROWS>(part_in, nb, sh);
+    icp_reduce_partials<ROWS, KF_ICP_BLOCK>(part_in, nb, sh);
     icp_update_pose<ROWS>(sh);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
   }
@@ -523,8 +534,10 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 }
 
 // Finishes the last iteration: final pose, its inverse and the record.
+constexpr int kIcpFinishThreads = 1024;
+
 template <bool KAHAN>
-__global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_finish(kf_dev_state* __restrict__ st,
+__global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* __restrict__ st,
                                                              const float* __restrict__ part_in,
                                                              int nb, int last) {
   constexpr int ROWS = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
@@ -533,7 +546,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_finish(kf_dev_state* __res
     sh.T0[threadIdx.x] = last == 0 ? st->Tprev[threadIdx.x] : st->iter_T[last - 1][threadIdx.x];
     sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
   }
-  icp_reduce_partials<ROWS>(part_in, nb, sh);
+  icp_reduce_partials<ROWS, kIcpFinishThreads>(part_in, nb, sh);
   icp_update_pose<ROWS>(sh);
   icp_record<ROWS>(st, last, sh);
   if (threadIdx.x < 16) {
@@ -968,6 +981,7 @@ struct RayIO {
   int part_base;       // first partial slot of this launch
   float* gn_part;      // refinement partials, kGnSums per block
   float max_res;       // refinement: residuals beyond this (mm) are ignored
+  int qslot;           // ray_queue slot of this raycast
 };
 
 // Raycast modes: 0 = the reference's hit handling (voxel_traversal.cu:222-
@@ -1538,7 +1552,7 @@ __global__ __launch_bounds__(256) void k_raycast(
   if (mask) {
     const int leader = __ffsll((unsigned long long)mask) - 1;
     uint32_t base = 0;
-    if (ln == leader) base = atomicAdd(&st->ray_queue, (uint32_t)__popcll(mask));
+    if (ln == leader) base = atomicAdd(&st->ray_queue[io.qslot], (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
     if (pending) queue[base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
@@ -1559,7 +1573,9 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
     float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
     const RayState* __restrict__ queue) {
-  const uint32_t n = st->ray_queue;
+  const uint32_t n = st->ray_queue[io.qslot];
+  // the next raycast appends to the other slot, which nothing reads now
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->ray_queue[io.qslot ^ 1] = 0u;
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
   float T[12];
@@ -1591,10 +1607,6 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * (io.part_base + blockIdx.x));
   if constexpr (MODE == kModeRefine)
     block_gn(gn, io.gn_part + kGnSums * (io.part_base + blockIdx.x));
-}
-
-__global__ void k_zero_queue(kf_dev_state* st) {
-  if (threadIdx.x == 0) st->ray_queue = 0u;
 }
 
 // Zeroes n4 float4 of dL/dF.
@@ -1706,14 +1718,21 @@ hipError_t kf_launch_depth_u16(kf_context* c, const uint16_t* src) {
   return hipGetLastError();
 }
 
-hipError_t kf_launch_bilateral(kf_context* c) {
+// src: nullptr to filter the working depth, else a uint16 frame to gate and
+// filter in one pass.
+static hipError_t launch_bilateral(kf_context* c, const uint16_t* src) {
   const dim3 grid((c->W + 15) / 16, (c->H + 15) / 16);
-  const float sr = c->prm.sigma_r_mm;
+  const float sr = c->prm.sigma_r_mm, lo = c->prm.depth_min_mm, hi = c->prm.depth_max_mm;
+  const void* in = src ? static_cast<const void*>(src) : static_cast<const void*>(c->depth);
   switch (c->prm.bilateral_radius) {
-#define KF_BIL(R)                                                                            \
-  case R:                                                                                    \
-    k_bilateral<R><<<grid, 256, 0, c->stream>>>(c->depth, c->depth_tmp, c->spatial, c->H,    \
-                                                c->W, sr);                                   \
+#define KF_BIL(R)                                                                              \
+  case R:                                                                                      \
+    if (src)                                                                                   \
+      k_bilateral<R, true><<<grid, 256, 0, c->stream>>>(in, c->depth_tmp, c->spatial, c->H,    \
+                                                        c->W, sr, lo, hi);                     \
+    else                                                                                       \
+      k_bilateral<R, false><<<grid, 256, 0, c->stream>>>(in, c->depth_tmp, c->spatial, c->H,   \
+                                                         c->W, sr, lo, hi);                    \
     break;
     KF_BIL(1)
     KF_BIL(2)
@@ -1733,6 +1752,12 @@ hipError_t kf_launch_bilateral(kf_context* c) {
   c->depth = c->depth_tmp;
   c->depth_tmp = t;
   return hipSuccess;
+}
+
+hipError_t kf_launch_bilateral(kf_context* c) { return launch_bilateral(c, nullptr); }
+
+hipError_t kf_launch_depth_bilateral(kf_context* c, const uint16_t* src) {
+  return launch_bilateral(c, src);
 }
 
 hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam) {
@@ -1760,7 +1785,7 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
   }
   if (iters > 0) {
     const float* pin = c->icp_partials + (size_t)((iters - 1) & 1) * rows * nb;
-    k_icp_finish<KAHAN><<<1, KF_ICP_BLOCK, 0, c->stream>>>(c->state, pin, nb, iters - 1);
+    k_icp_finish<KAHAN><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb, iters - 1);
   }
   return hipGetLastError();
 }
@@ -1858,8 +1883,9 @@ static void launch_raycast_p(kf_context* c, const RayIO& io, int tiles_x, int nb
 static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode) {
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
-  const RayIO io = make_ray_io(c, cam);
-  k_zero_queue<<<1, 64, 0, c->stream>>>(c->state);
+  RayIO io = make_ray_io(c, cam);
+  io.qslot = c->ray_slot;
+  c->ray_slot ^= 1;
   if (c->vol.precision == KF_TSDF_F32)
     launch_raycast_p<KF_TSDF_F32>(c, io, tiles_x, nb, c->count_rays, mode);
   else
