@@ -1430,6 +1430,211 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
   return false;
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative walk: one ray per wave, kWaveSteps steps of the traversal per
+// round (the second pass, where the few long rays would otherwise run one
+// lane each through a serial chain of several hundred steps).
+//
+// The walk chooses the axis with the smallest boundary distance, ties going
+// to z, then y, then x (voxel_traversal.cu:384-417).  With finite distances
+// that is the merge of the three per-axis sequences t_a(j + 1) = t_a(j) +
+// td_a by the key (t, rank), rank z 0 < y 1 < x 2: each sequence is
+// non-decreasing and a step always takes the smallest key among the heads.
+// A round:
+//   1. lanes 0-2 produce t_x, t_y, t_z (j = 0 .. kWaveSteps) with the float
+//      additions the serial walk makes, into LDS;
+//   2. lane j ranks the j-th element of each sequence in the merge (its index
+//      plus how many elements of the two other sequences come before it);
+//      the element of rank r < kWaveSteps is step r: its slot records the
+//      per-axis step counts after it and its key, and each element records
+//      the other two indices at its step (the reference's XY .. ZY);
+//   3. lane k visits voxel V_k (the voxel after k steps), loads its TSDF and
+//      applies the reference's tests with prev = the TSDF of V_{k-1} (lane
+//      k - 1; the ray's prev for k = 0): prev always equals the previous
+//      voxel's value because every visit that does not stop the walk sets it;
+//   4. the first visit that breaks or crosses ends the walk (the crossing is
+//      handled by that lane with the state of V_k rebuilt from the records),
+//      as does the first voxel outside the traversal; otherwise the state
+//      advances to V_kWaveSteps.
+// Visits, decisions and the crossing state are those of the serial walk.
+constexpr int kWaveSteps = 64;
+
+struct WaveWalkLds {
+  float seq[4][kWaveSteps + 1];  // t_a(j); row 3 takes the writes of lanes >= 3
+  int2 rec[3][kWaveSteps];       // other two indices at the j-th step along a
+  uint32_t cnt[kWaveSteps];      // step counts after step r: cx | cy << 8 | cz << 16
+  float key[kWaveSteps];         // boundary distance of step r
+};
+
+// #{j < kWaveSteps : s[j] <= v} (LE) or #{j : s[j] < v}, for a non-decreasing
+// s with s[j] close to s0 + j td: estimated from that, then corrected against
+// the stored values.
+template <bool LE>
+__device__ __forceinline__ int seq_count(const float* s, float s0, float inv_td, float v) {
+  const float q = floorf((v - s0) * inv_td) + 1.0f;
+  int c = q <= 0.f ? 0 : (q >= (float)kWaveSteps ? kWaveSteps : (int)q);
+  while (c > 0 && (LE ? s[c - 1] > v : s[c - 1] >= v)) --c;
+  while (c < kWaveSteps && (LE ? s[c] <= v : s[c] < v)) ++c;
+  return c;
+}
+
+// Wave-uniform (scalar) copies of values that every lane holds alike, so
+// that the branches they decide are scalar branches.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Walks ray r (the same state in every lane) to its end.  Returns the lane
+// that holds the result (ov, on, hit and, for the adjoint and refinement
+// modes, the gradient terms); the other lanes keep ov = on = 0, hit = -1.
+template <int PREC, bool COUNT, int MODE>
+__device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, const RayIO& io,
+                                         const float* T, WaveWalkLds& L, float4& ov,
+                                         float4& on, int& hit, int& visited, double* xi,
+                                         const Sink& sink, float* gn) {
+  const int lane = lane_id();
+  const kf_vol& vg = io.vg;
+  const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
+  const DdaStep p = r.stp;
+  if (!(isfinite(p.tdx) && isfinite(p.tdy) && isfinite(p.tdz) && isfinite(r.cur.tmx) &&
+        isfinite(r.cur.tmy) && isfinite(r.cur.tmz))) {
+    // a direction component is 0: the serial walk, in one lane.  (With finite
+    // td > 0 the heads stay finite: an axis whose next distance would
+    // overflow is never the smallest, so it never steps.)
+    if (lane == 0)
+      ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink,
+                                      gn);
+    return 0;
+  }
+  const float itx = 1.0f / p.tdx, ity = 1.0f / p.tdy, itz = 1.0f / p.tdz;
+  // Each round either ends the walk or advances it by kWaveSteps steps along
+  // the fixed step directions, so the walk leaves the volume after at most
+  // (Nx + Ny + Nz) / kWaveSteps + 1 rounds.
+  for (;;) {
+    const DdaState c0 = r.cur;
+    // 1. per-axis boundary distances (lanes 0-2; the others write row 3, so
+    //    that no lane-dependent branch precedes the cross-lane steps below)
+    {
+      float t = lane == 0 ? c0.tmx : (lane == 1 ? c0.tmy : c0.tmz);
+      const float td = lane == 0 ? p.tdx : (lane == 1 ? p.tdy : p.tdz);
+      float* sq = L.seq[min(lane, 3)];
+      for (int j = 0; j <= kWaveSteps; ++j) {
+        sq[j] = t;
+        t = t + td;
+      }
+    }
+    wave_lds_sync();
+    // 2. ranks in the merge; ties: y and z before x, z before y
+    const float ex = L.seq[0][lane], ey = L.seq[1][lane], ez = L.seq[2][lane];
+    const int y_x = seq_count<true>(L.seq[1], c0.tmy, ity, ex);
+    const int z_x = seq_count<true>(L.seq[2], c0.tmz, itz, ex);
+    const int x_y = seq_count<false>(L.seq[0], c0.tmx, itx, ey);
+    const int z_y = seq_count<true>(L.seq[2], c0.tmz, itz, ey);
+    const int x_z = seq_count<false>(L.seq[0], c0.tmx, itx, ez);
+    const int y_z = seq_count<false>(L.seq[1], c0.tmy, ity, ez);
+    L.rec[0][lane] = make_int2(c0.iy + p.sy * y_x, c0.iz + p.sz * z_x);
+    L.rec[1][lane] = make_int2(c0.ix + p.sx * x_y, c0.iz + p.sz * z_y);
+    L.rec[2][lane] = make_int2(c0.ix + p.sx * x_z, c0.iy + p.sy * y_z);
+    const int rx = lane + y_x + z_x, ry = lane + x_y + z_y, rz = lane + x_z + y_z;
+    if (rx < kWaveSteps) {
+      L.cnt[rx] = (uint32_t)(lane + 1) | ((uint32_t)y_x << 8) | ((uint32_t)z_x << 16);
+      L.key[rx] = ex;
+    }
+    if (ry < kWaveSteps) {
+      L.cnt[ry] = (uint32_t)x_y | ((uint32_t)(lane + 1) << 8) | ((uint32_t)z_y << 16);
+      L.key[ry] = ey;
+    }
+    if (rz < kWaveSteps) {
+      L.cnt[rz] = (uint32_t)x_z | ((uint32_t)y_z << 8) | ((uint32_t)(lane + 1) << 16);
+      L.key[rz] = ez;
+    }
+    wave_lds_sync();
+    // 3. lane k visits V_k
+    const uint32_t ck = lane == 0 ? 0u : L.cnt[lane - 1];
+    const int cx = ck & 0xff, cy = (ck >> 8) & 0xff, cz = ck >> 16;
+    const int vx = c0.ix + p.sx * cx, vy = c0.iy + p.sy * cy, vz = c0.iz + p.sz * cz;
+    const bool alive = vx < Nx - 1 && vx > 0 && vy < Ny - 1 && vy > 0 && vz < Nz - 1 && vz > 0;
+    const float f = alive ? F(kf_store_index(vg, vx - 1, vy - 1, vz - 1)) : 0.0f;
+    float prev = __shfl_up(f, 1);
+    if (lane == 0) prev = r.prev;
+    const bool brk =
+        (f != f && prev < 0.f) || (f < 0.f && prev != prev) || (f > 0.f && prev < 0.f);
+    const bool cross = !brk && f < 0.f && prev > 0.f;
+    const uint64_t ev = __ballot(alive && (brk || cross));
+    const uint64_t dead = __ballot(!alive);
+    const int kc = ev ? __ffsll((unsigned long long)ev) - 1 : kWaveSteps;
+    const int ke = dead ? __ffsll((unsigned long long)dead) - 1 : kWaveSteps;
+    if constexpr (COUNT) visited += lane < min(kc + 1, ke) ? 1 : 0;
+    if (kc < kWaveSteps) {
+      // 4. the walk stops at V_kc (kc < ke: only live visits raise events)
+      if (lane == kc && cross) {
+        DdaState h;
+        h.ix = vx;
+        h.iy = vy;
+        h.iz = vz;
+        const int2 rx_ = cx > 0 ? L.rec[0][cx - 1] : make_int2(c0.XY, c0.XZ);
+        const int2 ry_ = cy > 0 ? L.rec[1][cy - 1] : make_int2(c0.YX, c0.YZ);
+        const int2 rz_ = cz > 0 ? L.rec[2][cz - 1] : make_int2(c0.ZX, c0.ZY);
+        h.XY = rx_.x;
+        h.XZ = rx_.y;
+        h.YX = ry_.x;
+        h.YZ = ry_.y;
+        h.ZX = rz_.x;
+        h.ZY = rz_.y;
+        if constexpr (MODE == kModeReference) {
+          hit = crossing<PREC>(F, h, f, vg, io.fix_bounds, T[3], T[7], T[11], r.dx, r.dy, r.dz,
+                               io.flip_hi, ov, on);
+        } else {
+          const float te = lane == 0 ? r.tent : L.key[lane - 1];
+          const bool ok =
+              tri_hit<PREC, MODE>(F, io, T, r.pix, te, r.dx, r.dy, r.dz, ov, on, xi, sink, gn);
+          hit = ok ? (vx - 1) * (Ny * Nz) + (vy - 1) * Nz + (vz - 1) : -1;
+        }
+      }
+      return kc;
+    }
+    if (ke < kWaveSteps) return 0;  // left the traversal without a crossing
+    // advance to V_kWaveSteps
+    const uint32_t cn = __builtin_amdgcn_readfirstlane(L.cnt[kWaveSteps - 1]);
+    const int nx = cn & 0xff, ny = (cn >> 8) & 0xff, nz = cn >> 16;
+    // the ranks are a permutation, so the counts after the last step add up
+    // to the round's steps; anything else would stall the walk: stop instead
+    if (nx + ny + nz != kWaveSteps) return 0;
+    DdaState& c = r.cur;
+    c.ix = c0.ix + p.sx * nx;
+    c.iy = c0.iy + p.sy * ny;
+    c.iz = c0.iz + p.sz * nz;
+    if (nx > 0) {
+      const int2 q = L.rec[0][nx - 1];
+      c.XY = uni(q.x);
+      c.XZ = uni(q.y);
+    }
+    if (ny > 0) {
+      const int2 q = L.rec[1][ny - 1];
+      c.YX = uni(q.x);
+      c.YZ = uni(q.y);
+    }
+    if (nz > 0) {
+      const int2 q = L.rec[2][nz - 1];
+      c.ZX = uni(q.x);
+      c.ZY = uni(q.y);
+    }
+    c.tmx = uni(L.seq[0][nx]);
+    c.tmy = uni(L.seq[1][ny]);
+    c.tmz = uni(L.seq[2][nz]);
+    r.tent = uni(L.key[kWaveSteps - 1]);
+    r.prev = uni(__shfl(f, kWaveSteps - 1));
+    wave_lds_sync();  // every lane has read this round's records
+  }
+}
+
 // Optional work counters (COUNT): voxels visited by the walks and rays that
 // hit, summed into the device state for the algorithmic-traffic figure.
 __device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool hit) {
@@ -1478,19 +1683,20 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 
 // Pass 1: every pixel with depth != 0 casts its ray (voxel_traversal.cu:
 // 75-424) for up to kPass1Batches batches.  Rays still walking after that
-// (a few per cent: long walks through unobserved or free space) are appended
-// to a queue and finished by k_raycast_rest in full waves, instead of keeping
-// a whole wave busy for one long ray.
+// (a few per cent: long walks through unobserved or free space, such as the
+// rays of bilateral-filtered holes, whose depth of a few 1e-5 mm puts their
+// start behind the camera) are appended to a queue and finished by
+// k_raycast_rest, a whole wave per ray, instead of keeping a wave of short
+// rays waiting on one long one.
 //
 // A 256-thread block covers 16x16 pixels as four 8x8 wave tiles, so a wave's
 // rays walk neighbouring voxels.  Consecutive block ids are dealt round-robin
 // to the 8 XCDs; the id is remapped so that each XCD gets a contiguous band
 // of the image and its L2 sees the band's voxels.
-constexpr int kPass1Batches = 2;  // of 16 steps
-#ifndef KF_REST_BATCH
-#define KF_REST_BATCH 16
+#ifndef KF_PASS1_BATCHES
+#define KF_PASS1_BATCHES 2
 #endif
-constexpr int kRestBatch = KF_REST_BATCH;  // steps per batch in pass 2
+constexpr int kPass1Batches = KF_PASS1_BATCHES;  // of 16 steps
 
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast(
@@ -1567,36 +1773,51 @@ __global__ __launch_bounds__(256) void k_raycast(
   out_hit[p] = hit;
 }
 
-// Pass 2: finishes the queued rays, grid-stride over the queue.
+__device__ __forceinline__ void uniform_ray(RayState& r) {
+  DdaState& c = r.cur;
+  c.ix = uni(c.ix), c.iy = uni(c.iy), c.iz = uni(c.iz);
+  c.XY = uni(c.XY), c.XZ = uni(c.XZ), c.YX = uni(c.YX);
+  c.YZ = uni(c.YZ), c.ZX = uni(c.ZX), c.ZY = uni(c.ZY);
+  c.tmx = uni(c.tmx), c.tmy = uni(c.tmy), c.tmz = uni(c.tmz);
+  r.prev = uni(r.prev);
+  r.dx = uni(r.dx), r.dy = uni(r.dy), r.dz = uni(r.dz);
+  r.stp.sx = uni(r.stp.sx), r.stp.sy = uni(r.stp.sy), r.stp.sz = uni(r.stp.sz);
+  r.stp.tdx = uni(r.stp.tdx), r.stp.tdy = uni(r.stp.tdy), r.stp.tdz = uni(r.stp.tdz);
+  r.pix = uni(r.pix);
+  r.tent = uni(r.tent);
+}
+
+// Pass 2: finishes the queued rays, one ray per wave (wave_walk).  Rays are
+// dealt to waves in a fixed order (wave w takes rays w, w + waves, ...), so
+// that each block's gradient partial sums do not depend on timing and the
+// ray loop has no lane-dependent branch ahead of its cross-lane steps.
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
     float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
     const RayState* __restrict__ queue) {
-  const uint32_t n = st->ray_queue[io.qslot];
+  __shared__ WaveWalkLds lds[4];
+  const uint32_t n = __builtin_amdgcn_readfirstlane(st->ray_queue[io.qslot]);
   // the next raycast appends to the other slot, which nothing reads now
   if (blockIdx.x == 0 && threadIdx.x == 0) st->ray_queue[io.qslot ^ 1] = 0u;
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
   float T[12];
   load_uniform<12>(st->T, T);
-  const uint32_t stride = gridDim.x * 256;
-  // with wave reductions at the end of a trip, every lane runs the same
-  // number of trips
-  constexpr bool kUniform = COUNT;
-  const uint32_t trips = kUniform ? (n + stride - 1) / stride : 0;
+  const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const uint32_t n_waves = gridDim.x * 4;
   double xi[6] = {0, 0, 0, 0, 0, 0};
   float gn[MODE == kModeRefine ? kGnSums : 1] = {};
-  uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  for (uint32_t t = 0; kUniform ? t < trips : i < n; ++t, i += stride) {
+  for (uint32_t q = blockIdx.x * 4 + wave; q < n; q += n_waves) {
+    RayState r = queue[q];
+    uniform_ray(r);
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
-    if (i < n) {
-      RayState r = queue[i];
-      const Sink sink{io.gF, nullptr, nullptr};
-      ray_walk<PREC, kRestBatch, COUNT, MODE>(F, r, io, T, 0x7fffffff, ov, on, hit, visited, xi, sink,
-                                      gn);
-      if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
+    const Sink sink{io.gF, nullptr, nullptr};
+    const int owner =
+        wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, gn);
+    if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
+      if (lane == owner) {
         out_v[r.pix] = ov;
         out_n[r.pix] = on;
         out_hit[r.pix] = hit;
@@ -1851,7 +2072,11 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   return io;
 }
 
+// Second-pass grids: kRestBlocks (one partial-sum slot each) for the adjoint
+// and refinement modes, kRestBlocksWide for the forward modes (8192 waves: one
+// queued ray per wave at the few thousand rays a 640x480 frame queues).
 constexpr int kRestBlocks = 256;
+constexpr int kRestBlocksWide = 2048;
 
 template <int PREC, bool COUNT, int MODE>
 static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
@@ -1861,8 +2086,10 @@ static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb
                                                           tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
-  k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(
-      c->tsdf, io2, c->state, c->model_v, c->model_n, c->hit, q);
+  constexpr bool fixed = MODE == kModeAdjoint || MODE == kModeRefine;
+  k_raycast_rest<PREC, COUNT, MODE><<<fixed ? kRestBlocks : kRestBlocksWide, 256, 0,
+                                      c->stream>>>(c->tsdf, io2, c->state, c->model_v,
+                                                   c->model_n, c->hit, q);
 }
 
 template <int PREC>

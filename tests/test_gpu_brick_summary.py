@@ -1,0 +1,99 @@
+"""The raycast's brick summary (one bit per 8^3 brick: "may hold F < 0") lets
+the second raycast pass skip TSDF loads where no test of the reference walk
+can fire.  These tests check that the skipping never changes a result:
+
+  * the summary that integration maintains (bits set, never cleared): after a
+    GPU pipeline run, the GPU's own model maps equal the oracle's raycast of
+    the GPU's own volume, bit for bit, in F32 and F16;
+  * the summary that a volume upload recomputes, on an adversarial volume:
+    free space of NaN, positive values, +0 and -0, isolated negative voxels in
+    a fraction of the bricks, and rays that start behind the camera (depth of
+    a few 1e-5 mm, as a bilateral-filtered hole has) so that they walk the
+    whole volume.
+Bar: bit-exact hit indices and maps, as in the main parity tests."""
+import numpy as np
+import pytest
+
+from conftest import say, small_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    ok = (a == b) | (np.isnan(a) & np.isnan(b)) if a.dtype.kind == "f" else a == b
+    assert ok.all(), f"{what}: {int((~ok).sum())} of {a.size} differ"
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16"])
+def test_pipeline_maps_equal_oracle_raycast_of_own_volume(kf, oracle, room_stream, precision):
+    vol, prm, K, H, W, frames, gt = room_stream
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
+    vol.precision = kf.KF_TSDF_F32 if precision == "f32" else kf.KF_TSDF_F16
+    say(f"[gpu] brick summary: 10-frame pipeline, {precision}")
+    with kf.KinfuContext(vol, prm, H, W) as c:
+        c.stream_upload(frames)
+        c.pipeline_run(K, 0, len(frames))
+        got = c.model_maps()
+        d = c.depth()
+        T = c.pose()
+        t, _ = c.volume()
+        c.count_raycast_work(True)
+        c.raycast(K, T)
+        visited, _ = c.raycast_work()
+        c.count_raycast_work(False)
+    want = oracle.raycast(d, K, T, np.ascontiguousarray(t, np.float32), vol,
+                          prm.truncation_mm, 0)
+    assert_same(got["hit"], want["hit"], "hit index")
+    for k in ("x", "y", "z", "nx", "ny", "nz"):
+        assert_same(got[k], want[k], k)
+    assert visited == want["steps"]
+    # the stream has hole pixels whose filtered depth is tiny: long walks
+    assert (got["hit"] >= 0).mean() > 0.5
+
+
+def adversarial_volume(rng, dims, neg_brick_frac):
+    nx, ny, nz = dims
+    u = rng.random((nx, ny, nz))
+    t = np.where(u < 0.45, np.nan, rng.uniform(1e-3, 1.0, (nx, ny, nz))).astype(np.float32)
+    t[(u >= 0.90) & (u < 0.95)] = 0.0
+    t[u >= 0.95] = -0.0
+    # isolated negative voxels in some bricks
+    bx, by, bz = (nx + 7) // 8, (ny + 7) // 8, (nz + 7) // 8
+    pick = rng.random((bx, by, bz)) < neg_brick_frac
+    for (i, j, k) in zip(*np.nonzero(pick)):
+        for _ in range(int(rng.integers(1, 4))):
+            x = min(8 * i + int(rng.integers(0, 8)), nx - 1)
+            y = min(8 * j + int(rng.integers(0, 8)), ny - 1)
+            z = min(8 * k + int(rng.integers(0, 8)), nz - 1)
+            t[x, y, z] = -rng.uniform(1e-3, 1.0)
+    w = np.where(np.isnan(t), np.nan, 1.0).astype(np.float32)
+    return t, w
+
+
+@pytest.mark.parametrize("neg_brick_frac", [0.0, 0.05, 0.3])
+def test_upload_summary_adversarial_volume(kf, oracle, neg_brick_frac):
+    rng = np.random.default_rng(int(neg_brick_frac * 1000) + 7)
+    vol, prm, K, H, W = small_setup(kf, H=96, W=128, dims=(70, 66, 75), start=(-35, -33, -10),
+                                    voxel=20.0)
+    K = np.array([110.0, 0, 63.5, 0, 110.0, 47.5, 0, 0, 1], np.float32)
+    t, w = adversarial_volume(rng, tuple(vol.dims), neg_brick_frac)
+    d = rng.uniform(300.0, 1500.0, (H, W)).astype(np.float32)
+    d[rng.random((H, W)) < 0.5] = np.float32(4.8e-5)  # rays from behind the camera
+    d[rng.random((H, W)) < 0.05] = 0.0
+    T = np.eye(4, dtype=np.float32)
+    say(f"[gpu] brick summary: adversarial volume, negative brick fraction {neg_brick_frac}")
+    with kf.KinfuContext(vol, prm, H, W) as c:
+        c.upload_volume(t, w)
+        c.upload_depth_f32(d)
+        c.count_raycast_work(True)
+        c.raycast(K, T)
+        visited, hits = c.raycast_work()
+        got = c.model_maps()
+    want = oracle.raycast(d, K, T, t, vol, prm.truncation_mm, 0)
+    assert_same(got["hit"], want["hit"], "hit index")
+    for k in ("x", "y", "z", "nx", "ny", "nz"):
+        assert_same(got[k], want[k], k)
+    assert (visited, hits) == (want["steps"], want["hits"])
+    assert want["steps"] > 20 * H * W // 2  # long walks were exercised
