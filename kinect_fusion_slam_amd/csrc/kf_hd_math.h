@@ -15,8 +15,11 @@
 
 #ifdef __HIPCC__
 #define KF_HD __host__ __device__ inline
+// full unrolling keeps the small fixed-size arrays of the solvers in registers
+#define KF_UNROLL _Pragma("unroll")
 #else
 #define KF_HD inline
+#define KF_UNROLL
 #endif
 
 // ---------------------------------------------------------------------------
@@ -184,28 +187,28 @@ KF_HD int kf_m3_inverse(const float* in, float* out) {
 KF_HD int kf_ldlt6_solve(const double* A, const double* b, double* x) {
   double L[6][6];
   double d[6], rd[6];  // pivots and their reciprocals (6 divisions in total)
-  for (int i = 0; i < 6; ++i) {
-    for (int j = 0; j < i; ++j) {
+  KF_UNROLL for (int i = 0; i < 6; ++i) {
+    KF_UNROLL for (int j = 0; j < i; ++j) {
       double s = A[i * 6 + j];
-      for (int k = 0; k < j; ++k) s = s - L[i][k] * L[j][k] * d[k];
+      KF_UNROLL for (int k = 0; k < j; ++k) s = s - L[i][k] * L[j][k] * d[k];
       L[i][j] = s * rd[j];
     }
     double s = A[i * 6 + i];
-    for (int k = 0; k < i; ++k) s = s - L[i][k] * L[i][k] * d[k];
+    KF_UNROLL for (int k = 0; k < i; ++k) s = s - L[i][k] * L[i][k] * d[k];
     if (!(s >= KF_LDLT_EPS)) return 1;
     d[i] = s;
     rd[i] = 1.0 / s;
   }
   double y[6];
-  for (int i = 0; i < 6; ++i) {
+  KF_UNROLL for (int i = 0; i < 6; ++i) {
     double s = b[i];
-    for (int k = 0; k < i; ++k) s = s - L[i][k] * y[k];
+    KF_UNROLL for (int k = 0; k < i; ++k) s = s - L[i][k] * y[k];
     y[i] = s;
   }
-  for (int i = 0; i < 6; ++i) y[i] = y[i] * rd[i];
-  for (int i = 5; i >= 0; --i) {
+  KF_UNROLL for (int i = 0; i < 6; ++i) y[i] = y[i] * rd[i];
+  KF_UNROLL for (int i = 5; i >= 0; --i) {
     double s = y[i];
-    for (int k = i + 1; k < 6; ++k) s = s - L[k][i] * x[k];
+    KF_UNROLL for (int k = i + 1; k < 6; ++k) s = s - L[k][i] * x[k];
     x[i] = s;
   }
   return 0;
@@ -224,21 +227,21 @@ KF_HD int kf_sym_index(int i, int j) {  // i <= j
 // Returns the solver status (0 solved, 1 fallback to the right-hand side).
 KF_HD int kf_icp_increment(const double* sums, float* inc) {
   double A[36], b[6], xd[6];
-  for (int i = 0; i < 6; ++i)
-    for (int j = i; j < 6; ++j) {
+  KF_UNROLL for (int i = 0; i < 6; ++i)
+    KF_UNROLL for (int j = i; j < 6; ++j) {
       const double v = sums[kf_sym_index(i, j)];
       A[i * 6 + j] = v;
       A[j * 6 + i] = v;
     }
-  for (int i = 0; i < 6; ++i) b[i] = sums[21 + i];
+  KF_UNROLL for (int i = 0; i < 6; ++i) b[i] = sums[21 + i];
   const int status = kf_ldlt6_solve(A, b, xd);
   float x[6];
-  for (int i = 0; i < 6; ++i) x[i] = (float)(status ? b[i] : xd[i]);
+  KF_UNROLL for (int i = 0; i < 6; ++i) x[i] = (float)(status ? b[i] : xd[i]);
   const float m[16] = {1.0f, x[2], -x[1], x[3],  //
                        -x[2], 1.0f, x[0], x[4],  //
                        x[1], -x[0], 1.0f, x[5],  //
                        0.0f, 0.0f, 0.0f, 1.0f};
-  for (int i = 0; i < 16; ++i) inc[i] = m[i];
+  KF_UNROLL for (int i = 0; i < 16; ++i) inc[i] = m[i];
   return status;
 }
 

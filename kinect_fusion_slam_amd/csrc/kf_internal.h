@@ -107,7 +107,8 @@ struct kf_context {
 
   // ICP
   kf_dev_state* state = nullptr;
-  float* icp_partials = nullptr;   // 2 x [2 * KF_ICP_SUMS][blocks], by iteration parity
+  void* icp_partials = nullptr;    // 2 parities x [KF_ICP_SUMS][blocks] doubles (or
+                                   // [2 * KF_ICP_SUMS][blocks] floats, compensated)
   int icp_blocks = 0;
   int icp_px = 4;                  // pixels per thread
   uint8_t* icp_masks = nullptr;    // [KF_MAX_ICP_ITERS][H*W] when record_icp

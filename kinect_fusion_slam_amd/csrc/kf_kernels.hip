@@ -15,6 +15,8 @@
 //     frame (sensor_pose_estimation.cu:339-387).
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "kf_internal.h"
 
 namespace {
@@ -40,26 +42,6 @@ __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
   z = z * t;
 }
 
-// wave64 DPP reduction: after the call lane 63 holds the sum of all lanes.
-// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror, row_mirror give every
-// lane its row's sum; row_bcast:15 and row_bcast:31 then fold rows 0..3 into
-// row 3.  The order of additions is fixed, so the result is deterministic.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_src(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF, false));
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-  v = v + dpp_src<0xB1, 0xF>(v);
-  v = v + dpp_src<0x4E, 0xF>(v);
-  v = v + dpp_src<0x141, 0xF>(v);
-  v = v + dpp_src<0x140, 0xF>(v);
-  v = v + dpp_src<0x142, 0xA>(v);
-  v = v + dpp_src<0x143, 0xC>(v);
-  return v;
-}
-
 // Compensated (TwoSum) combination of (s, c) pairs: s + c carries the sum
 // with the rounding error of every addition folded into c.
 __device__ __forceinline__ void two_sum_add(float& s, float& c, float s2, float c2) {
@@ -70,20 +52,179 @@ __device__ __forceinline__ void two_sum_add(float& s, float& c, float s2, float 
   c = c + c2 + e;
 }
 
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ void dpp_pair_step(float& s, float& c) {
-  const float s2 = dpp_src<CTRL, ROW_MASK>(s);
-  const float c2 = dpp_src<CTRL, ROW_MASK>(c);
-  two_sum_add(s, c, s2, c2);
+// Cross-lane moves for float and double (a double moves as two dwords).
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __uint_as_float(dpp_u32<CTRL, ROW_MASK>(__float_as_uint(v)));
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)u);
+  const uint32_t hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(u >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// lane ^ 4 within each 32 lanes
+__device__ __forceinline__ float swz_xor4(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+}
+__device__ __forceinline__ double swz_xor4(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x101F);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x101F);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// v_permlane32_swap: the upper 32 lanes of a trade places with the lower 32
+// lanes of b.  v_permlane16_swap: rows 1 and 3 of a with rows 0 and 2 of b.
+template <bool W32>
+__device__ __forceinline__ void lane_swap_u32(uint32_t& a, uint32_t& b) {
+  if constexpr (W32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  }
+}
+template <bool W32>
+__device__ __forceinline__ void lane_swap(float& a, float& b) {
+  uint32_t x = __float_as_uint(a), y = __float_as_uint(b);
+  lane_swap_u32<W32>(x, y);
+  a = __uint_as_float(x);
+  b = __uint_as_float(y);
+}
+template <bool W32>
+__device__ __forceinline__ void lane_swap(double& a, double& b) {
+  const uint64_t ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  uint32_t alo = (uint32_t)ua, ahi = (uint32_t)(ua >> 32);
+  uint32_t blo = (uint32_t)ub, bhi = (uint32_t)(ub >> 32);
+  lane_swap_u32<W32>(alo, blo);
+  lane_swap_u32<W32>(ahi, bhi);
+  a = __longlong_as_double((long long)(((uint64_t)ahi << 32) | alo));
+  b = __longlong_as_double((long long)(((uint64_t)bhi << 32) | blo));
 }
 
-__device__ __forceinline__ void wave_sum_comp(float& s, float& c) {
-  dpp_pair_step<0xB1, 0xF>(s, c);
-  dpp_pair_step<0x4E, 0xF>(s, c);
-  dpp_pair_step<0x141, 0xF>(s, c);
-  dpp_pair_step<0x140, 0xF>(s, c);
-  dpp_pair_step<0x142, 0xA>(s, c);
-  dpp_pair_step<0x143, 0xC>(s, c);
+// Sum over the 64 lanes of a wave (DPP: quad_perm [1,0,3,2] and [2,3,0,1],
+// row_half_mirror, row_mirror give every lane its row's sum; row_bcast:15 and
+// row_bcast:31 fold rows 0..3 into row 3).  Lane 63 holds the total; the
+// order of additions is fixed.
+template <typename T>
+__device__ __forceinline__ T wave_total(T v) {
+  v = v + dpp_mov<0xB1>(v);
+  v = v + dpp_mov<0x4E>(v);
+  v = v + dpp_mov<0x141>(v);
+  v = v + dpp_mov<0x140>(v);
+  v = v + dpp_mov<0x142, 0xA>(v);
+  v = v + dpp_mov<0x143, 0xC>(v);
+  return v;
+}
+
+// Transposing wave reduction of 32 values per lane.  Each of the steps
+// o = 32, 16, 8, 4, 2 halves the values a lane keeps (lanes with bit o set
+// keep the upper half and take the partner's upper half, lane ^ o, the others
+// the lower), so the wave moves 16 + 8 + 4 + 2 + 1 + 1 values instead of
+// reducing 32 values separately.  Steps 32 and 16 are v_permlane32_swap /
+// v_permlane16_swap of the two halves (after the swap each lane adds its own
+// and its partner's element), steps 8 and 2 are DPP row_ror:8 and quad_perm,
+// step 4 is ds_swizzle.  On return s[0] (with its compensation c[0] when
+// KAHAN) in lane l holds the 64-lane total of value
+//   ((l >> 5) & 1) * 16 + ((l >> 4) & 1) * 8 + ((l >> 3) & 1) * 4
+//   + ((l >> 2) & 1) * 2 + ((l >> 1) & 1),
+// the same in lanes l and l ^ 1.  The additions form a fixed tree.
+__device__ __forceinline__ int wave_transpose_index(int l) {
+  return ((l >> 5) & 1) * 16 + ((l >> 4) & 1) * 8 + ((l >> 3) & 1) * 4 + ((l >> 2) & 1) * 2 +
+         ((l >> 1) & 1);
+}
+
+template <bool KAHAN, typename T>
+__device__ __forceinline__ void transpose_add(T& s, T* c, T gs, T gc) {
+  if constexpr (KAHAN)
+    two_sum_add(s, *c, gs, gc);
+  else
+    s = s + gs;
+}
+
+template <bool KAHAN, typename T>
+__device__ __forceinline__ void wave_transpose_sum(T (&s)[32], T* c) {
+  const int ln = lane_id();
+  T cdummy = T(0);
+  // steps 32 and 16: swap the halves across lanes, then add
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    T a = s[i], b = s[i + 16];
+    lane_swap<true>(a, b);
+    T ca = T(0), cb = T(0);
+    if constexpr (KAHAN) {
+      ca = c[i];
+      cb = c[i + 16];
+      lane_swap<true>(ca, cb);
+    }
+    transpose_add<KAHAN>(a, KAHAN ? &ca : &cdummy, b, cb);
+    s[i] = a;
+    if constexpr (KAHAN) c[i] = ca;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    T a = s[i], b = s[i + 8];
+    lane_swap<false>(a, b);
+    T ca = T(0), cb = T(0);
+    if constexpr (KAHAN) {
+      ca = c[i];
+      cb = c[i + 8];
+      lane_swap<false>(ca, cb);
+    }
+    transpose_add<KAHAN>(a, KAHAN ? &ca : &cdummy, b, cb);
+    s[i] = a;
+    if constexpr (KAHAN) c[i] = ca;
+  }
+  // steps 8, 4, 2: each lane keeps one half and takes the partner's copy of it
+  auto pull = [&](auto move, int o, int h) {
+    const bool up = (ln & o) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      // the empty asm keeps the two halves as values, so that the selects do
+      // not become a load from a selected address (which would put the array
+      // in scratch memory)
+      T lo = s[i], hi = s[i + h];
+      asm volatile("" : "+v"(lo), "+v"(hi));
+      T ks = up ? hi : lo;
+      const T gs = move(up ? lo : hi);
+      T kc = T(0), gc = T(0);
+      if constexpr (KAHAN) {
+        T clo = c[i], chi = c[i + h];
+        asm volatile("" : "+v"(clo), "+v"(chi));
+        kc = up ? chi : clo;
+        gc = move(up ? clo : chi);
+      }
+      transpose_add<KAHAN>(ks, KAHAN ? &kc : &cdummy, gs, gc);
+      s[i] = ks;
+      if constexpr (KAHAN) c[i] = kc;
+    }
+  };
+  pull([](T v) { return dpp_mov<0x128>(v); }, 8, 4);  // row_ror:8 = lane ^ 8
+  pull([](T v) { return swz_xor4(v); }, 4, 2);
+  pull([](T v) { return dpp_mov<0x4E>(v); }, 2, 1);    // quad_perm [2,3,0,1]
+  // step 1: both lanes of a pair add in lane order, so they agree
+  const T gs = dpp_mov<0xB1>(s[0]);                     // quad_perm [1,0,3,2]
+  if constexpr (KAHAN) {
+    const T gc = dpp_mov<0xB1>(c[0]);
+    if (ln & 1) {
+      T t = gs, tc = gc;
+      two_sum_add(t, tc, s[0], c[0]);
+      s[0] = t;
+      c[0] = tc;
+    } else {
+      two_sum_add(s[0], c[0], gs, gc);
+    }
+  } else {
+    s[0] = (ln & 1) ? gs + s[0] : s[0] + gs;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -258,74 +399,75 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_
 // from is read from the per-iteration record, so nothing a kernel reads is
 // written by that kernel.
 
+template <bool KAHAN>
+using IcpPartial = typename std::conditional<KAHAN, float, double>::type;
+
 template <int ROWS>
 struct IcpShared {
-  double lane_part[ROWS][64];
-  double quarter[ROWS][4];
+  double rowsum[ROWS];
   double sums[KF_ICP_SUMS];
   float inc[16], T0[16], Tp[16], Tn[16], F[16];
   int status;
 };
 
-// Sums the ROWS x nb partials into sh.sums (all threads take part).  Each
-// wave takes rows w, w + NW, ...; every lane issues all of its loads for
-// those rows at once (one memory round trip), adds its up to
-// KF_ICP_MAX_PART_PER_LANE entries of each row in FP64, and the 64 lane sums
-// of a row are then added in lane order by 4 threads of 16 lanes each.
-template <int ROWS, int NT>
-__device__ __forceinline__ void icp_reduce_partials(const float* __restrict__ P, int nb,
+// Sums the ROWS x nb partials into sh.sums (all threads take part).  Wave w
+// takes rows w, w + NW, ...; each lane loads its up to
+// KF_ICP_MAX_PART_PER_LANE entries of those rows with unconditional
+// (clamped) loads, so that they are all in flight together, adds them in
+// FP64, and the 64 lane sums of a row are added by a DPP tree (a fixed
+// order).
+template <int ROWS, int NT, typename PT>
+__device__ __forceinline__ void icp_reduce_partials(const PT* __restrict__ P, int nb,
                                                     IcpShared<ROWS>& sh) {
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
   constexpr int NW = NT / 64;
   constexpr int RPW = (ROWS + NW - 1) / NW;  // rows per wave
-  float v[RPW][KF_ICP_MAX_PART_PER_LANE];
+  PT v[RPW][KF_ICP_MAX_PART_PER_LANE];
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
-    const int row = wave + r * NW;
+    const int row = min(wave + r * NW, ROWS - 1);
 #pragma unroll
-    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j) {
-      const int idx = ln + 64 * j;
-      v[r][j] = (row < ROWS && idx < nb) ? P[(size_t)row * nb + idx] : 0.0f;
-    }
+    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j)
+      v[r][j] = P[(size_t)row * nb + min(ln + 64 * j, nb - 1)];
   }
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j)
+      t += ln + 64 * j < nb ? (double)v[r][j] : 0.0;
+    t = wave_total(t);
     const int row = wave + r * NW;
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j) s += (double)v[r][j];
-    if (row < ROWS) sh.lane_part[row][ln] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < 4 * ROWS) {
-    const int row = threadIdx.x >> 2, q = threadIdx.x & 3;
-    double s = 0.0;
-#pragma unroll
-    for (int l = 0; l < 16; ++l) s += sh.lane_part[row][q * 16 + l];
-    sh.quarter[row][q] = s;
+    if (ln == 63 && row < ROWS) sh.rowsum[row] = t;
   }
   __syncthreads();
   if (threadIdx.x < KF_ICP_SUMS) {
     const int k = threadIdx.x;
-    auto row_total = [&](int row) {
-      return ((sh.quarter[row][0] + sh.quarter[row][1]) + sh.quarter[row][2]) +
-             sh.quarter[row][3];
-    };
-    sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? row_total(2 * k) + row_total(2 * k + 1) : row_total(k);
+    sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? sh.rowsum[2 * k] + sh.rowsum[2 * k + 1] : sh.rowsum[k];
   }
   __syncthreads();
 }
 
-// Finishes an iteration: T_new = inc(sums) * T0 and F = Tp * T_new, into sh.
+// LDS visibility between the lanes of one wave.
+__device__ __forceinline__ void icp_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Finishes an iteration in wave 0: the solve (one lane), T_new = inc(sums) *
+// T0 and F = Tp * T_new (16 lanes each), then one block barrier.
 template <int ROWS>
 __device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh) {
-  if (threadIdx.x == 0) sh.status = kf_icp_increment(sh.sums, sh.inc);
-  __syncthreads();
-  if (threadIdx.x < 16)
-    sh.Tn[threadIdx.x] = kf_mat4_mul_elem(sh.inc, sh.T0, threadIdx.x >> 2, threadIdx.x & 3);
-  __syncthreads();
-  if (threadIdx.x < 16)
-    sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
+  if (threadIdx.x < 64) {
+    if (threadIdx.x == 0) sh.status = kf_icp_increment(sh.sums, sh.inc);
+    icp_wave_sync();
+    if (threadIdx.x < 16)
+      sh.Tn[threadIdx.x] = kf_mat4_mul_elem(sh.inc, sh.T0, threadIdx.x >> 2, threadIdx.x & 3);
+    icp_wave_sync();
+    if (threadIdx.x < 16)
+      sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
+  }
   __syncthreads();
 }
 
@@ -347,15 +489,31 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const float4* __restrict__ vmap, const float4* __restrict__ nmap,
     const float4* __restrict__ mv, const float4* __restrict__ mn, kf_cam cam, int H, int W,
     float dist_gate, float cos_lo, kf_dev_state* __restrict__ st,
-    const float* __restrict__ part_in, float* __restrict__ part_out,
+    const void* __restrict__ part_in_, void* __restrict__ part_out_,
     uint8_t* __restrict__ mask_out, int iter) {
   constexpr int NS = KF_ICP_SUMS;
   constexpr int ROWS = KAHAN ? 2 * NS : NS;
+  // block partials: FP64 sums, or FP32 (sum, compensation) pairs when KAHAN
+  using PT = IcpPartial<KAHAN>;
+  const PT* __restrict__ part_in = static_cast<const PT*>(part_in_);
+  PT* __restrict__ part_out = static_cast<PT*>(part_out_);
   __shared__ IcpShared<ROWS> sh;
-  __shared__ float red_s[KF_ICP_BLOCK / 64][NS];
-  __shared__ float red_c[KF_ICP_BLOCK / 64][NS];
+  __shared__ PT red_s[KF_ICP_BLOCK / 64][NS];
+  __shared__ PT red_c[KAHAN ? KF_ICP_BLOCK / 64 : 1][NS];
   const int nb = gridDim.x;
 
+  const int n = H * W;
+  const int base = blockIdx.x * (KF_ICP_BLOCK * PX) + threadIdx.x;
+  // phase 1: this thread's vertex and normal loads, issued before the
+  // previous iteration is finished (they do not depend on the pose); the
+  // index is clamped so that the loads are unconditional
+  float4 V[PX], Nc[PX];
+#pragma unroll
+  for (int k = 0; k < PX; ++k) {
+    const int x = min(base + k * KF_ICP_BLOCK, n - 1);
+    V[k] = vmap[x];
+    Nc[k] = nmap[x];
+  }
   if (iter == 0) {
     // frame start
     if (threadIdx.x == 0) {
@@ -380,7 +538,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       sh.T0[threadIdx.x] = iter == 1 ? st->Tprev[threadIdx.x] : st->iter_T[iter - 2][threadIdx.x];
       sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
     }
-    icp_reduce_partials<ROWS, KF_ICP_BLOCK>(part_in, nb, sh);
+    icp_reduce_partials<ROWS, KF_ICP_BLOCK, PT>(part_in, nb, sh);
     icp_update_pose<ROWS>(sh);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
   }
@@ -393,16 +551,9 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 #pragma unroll
   for (int k = 0; k < NS; ++k) acc[k] = 0.f, comp[k] = 0.f;
 
-  const int n = H * W;
-  const int base = blockIdx.x * (KF_ICP_BLOCK * PX) + threadIdx.x;
-  // phase 1: this thread's vertex and normal loads, issued together
-  float4 V[PX], Nc[PX];
 #pragma unroll
-  for (int k = 0; k < PX; ++k) {
-    const int x = base + k * KF_ICP_BLOCK;
-    V[k] = x < n ? vmap[x] : make_float4(0.f, 0.f, 0.f, 0.f);
-    Nc[k] = x < n ? nmap[x] : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  for (int k = 0; k < PX; ++k)
+    if (base + k * KF_ICP_BLOCK >= n) V[k] = Nc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
   // phase 2: projective association, then the model gathers, issued together
   int jx[PX];
 #pragma unroll
@@ -420,11 +571,13 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       if (u > 0 && u <= W && v > 0 && v <= H) jx[k] = (v - 1) * W + (u - 1);
     }
   }
+  // the model gathers, unconditional (pixel 0 for unassociated pixels, whose
+  // values are not used)
   float4 S[PX], Sn[PX];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
-    S[k] = jx[k] >= 0 ? mv[jx[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    Sn[k] = jx[k] >= 0 ? mn[jx[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    S[k] = mv[max(jx[k], 0)];
+    Sn[k] = mn[max(jx[k], 0)];
   }
   // phase 3: gates, Jacobian rows, accumulation
 #pragma unroll
@@ -497,26 +650,28 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   }
 
   // wave reduction -> LDS.  Kahan keeps s - c as the value (c is the running
-  // error with the opposite sign), so the pair (s, -c) is reduced.
+  // error with the opposite sign), so the pair (s, -c) is reduced in FP32
+  // TwoSum arithmetic; otherwise the lane sums (FP32 over this thread's
+  // pixels, as the per-pixel products) are added in FP64 from here on.
   const int wave = threadIdx.x >> 6, ln = lane_id();
+  {
+    PT rs[32], rc[KAHAN ? 32 : 1];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    if constexpr (KAHAN) {
-      float s = acc[k], c = -comp[k];
-      wave_sum_comp(s, c);
-      if (ln == 63) {
-        red_s[wave][k] = s;
-        red_c[wave][k] = c;
-      }
-    } else {
-      const float s = wave_sum(acc[k]);
-      if (ln == 63) red_s[wave][k] = s;
+    for (int k = 0; k < 32; ++k) {
+      rs[k] = k < NS ? (PT)acc[k] : (PT)0;
+      if constexpr (KAHAN) rc[k] = k < NS ? -comp[k] : 0.f;
+    }
+    wave_transpose_sum<KAHAN>(rs, rc);
+    const int k = wave_transpose_index(ln);
+    if ((ln & 1) == 0 && k < NS) {
+      red_s[wave][k] = rs[0];
+      if constexpr (KAHAN) red_c[wave][k] = rc[0];
     }
   }
   __syncthreads();
   if (threadIdx.x < NS) {
     const int k = threadIdx.x;
-    float s = red_s[0][k], c = KAHAN ? red_c[0][k] : 0.f;
+    PT s = red_s[0][k], c = KAHAN ? red_c[0][k] : (PT)0;
 #pragma unroll
     for (int w = 1; w < KF_ICP_BLOCK / 64; ++w) {
       if constexpr (KAHAN)
@@ -538,7 +693,7 @@ constexpr int kIcpFinishThreads = 1024;
 
 template <bool KAHAN>
 __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* __restrict__ st,
-                                                             const float* __restrict__ part_in,
+                                                             const void* __restrict__ part_in,
                                                              int nb, int last) {
   constexpr int ROWS = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
   __shared__ IcpShared<ROWS> sh;
@@ -546,7 +701,8 @@ __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* 
     sh.T0[threadIdx.x] = last == 0 ? st->Tprev[threadIdx.x] : st->iter_T[last - 1][threadIdx.x];
     sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
   }
-  icp_reduce_partials<ROWS, kIcpFinishThreads>(part_in, nb, sh);
+  icp_reduce_partials<ROWS, kIcpFinishThreads>(static_cast<const IcpPartial<KAHAN>*>(part_in),
+                                               nb, sh);
   icp_update_pose<ROWS>(sh);
   icp_record<ROWS>(st, last, sh);
   if (threadIdx.x < 16) {
@@ -1859,15 +2015,30 @@ __global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict
 // partials, the 6x6 solve (LDLT, FP64) and T <- exp(-x^) T in FP64, with
 // x = (omega, v) the solution of (J^T J) x = J^T r.  Records the mean squared
 // residual before the step, the pixel count and the solver status.
-__global__ __launch_bounds__(64) void k_refine_finish(const float* __restrict__ part,
-                                                      int n_part, kf_dev_state* st, int step) {
+__global__ __launch_bounds__(256) void k_refine_finish(const float* __restrict__ part,
+                                                       int n_part, kf_dev_state* st, int step) {
+  // FP64 sums of the partials in a fixed order: thread t adds blocks t,
+  // t + 256, ..., then a tree over the 256 threads
+  __shared__ double red[kGnSums][256];
   __shared__ double s[kGnSums];
-  const int q = threadIdx.x;
-  if (q < kGnSums) {
-    double v = 0.0;
-    for (int b = 0; b < n_part; ++b) v += (double)part[kGnSums * b + q];
-    s[q] = v;
+  {
+    double v[kGnSums];
+#pragma unroll
+    for (int q = 0; q < kGnSums; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < n_part; b += 256)
+#pragma unroll
+      for (int q = 0; q < kGnSums; ++q) v[q] += (double)part[kGnSums * b + q];
+#pragma unroll
+    for (int q = 0; q < kGnSums; ++q) red[q][threadIdx.x] = v[q];
   }
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < kGnSums; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < kGnSums) s[threadIdx.x] = red[threadIdx.x][0];
   __syncthreads();
   if (threadIdx.x != 0) return;
   double A[36], rhs[6], x[6] = {0, 0, 0, 0, 0, 0};
@@ -1993,11 +2164,14 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
   const int iters = c->prm.icp_iterations;
   const int n = c->H * c->W;
   const int nb = c->icp_blocks;
-  const size_t rows = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
+  // one parity of partials: KF_ICP_SUMS doubles, or 2 * KF_ICP_SUMS floats
+  // (compensated), per block
+  const size_t half = (size_t)KF_ICP_SUMS * 8 * nb;
+  char* const pbase = reinterpret_cast<char*>(c->icp_partials);
   for (int it = 0; it < iters; ++it) {
     uint8_t* m = (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
-    const float* pin = c->icp_partials + (size_t)((it + 1) & 1) * rows * nb;
-    float* pout = c->icp_partials + (size_t)(it & 1) * rows * nb;
+    const void* pin = pbase + (size_t)((it + 1) & 1) * half;
+    void* pout = pbase + (size_t)(it & 1) * half;
     k_icp_iter<KAHAN, PX><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
         c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
         c->icp_cos_lo, c->state, pin, pout, m, it);
@@ -2005,7 +2179,7 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
     if (e != hipSuccess) return e;
   }
   if (iters > 0) {
-    const float* pin = c->icp_partials + (size_t)((iters - 1) & 1) * rows * nb;
+    const void* pin = pbase + (size_t)((iters - 1) & 1) * half;
     k_icp_finish<KAHAN><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb, iters - 1);
   }
   return hipGetLastError();
@@ -2072,11 +2246,10 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   return io;
 }
 
-// Second-pass grids: kRestBlocks (one partial-sum slot each) for the adjoint
-// and refinement modes, kRestBlocksWide for the forward modes (8192 waves: one
-// queued ray per wave at the few thousand rays a 640x480 frame queues).
-constexpr int kRestBlocks = 256;
-constexpr int kRestBlocksWide = 2048;
+// Second-pass grid: 8192 waves, one queued ray per wave at the few thousand
+// rays a 640x480 frame queues.  The adjoint and refinement modes keep one
+// partial-sum slot per block.
+constexpr int kRestBlocks = 2048;
 
 template <int PREC, bool COUNT, int MODE>
 static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
@@ -2086,10 +2259,8 @@ static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb
                                                           tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
-  constexpr bool fixed = MODE == kModeAdjoint || MODE == kModeRefine;
-  k_raycast_rest<PREC, COUNT, MODE><<<fixed ? kRestBlocks : kRestBlocksWide, 256, 0,
-                                      c->stream>>>(c->tsdf, io2, c->state, c->model_v,
-                                                   c->model_n, c->hit, q);
+  k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(
+      c->tsdf, io2, c->state, c->model_v, c->model_n, c->hit, q);
 }
 
 template <int PREC>
@@ -2129,7 +2300,7 @@ hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps) {
   for (int it = 0; it < steps; ++it) {
     hipError_t e = launch_raycast_mode(c, cam, kModeRefine);
     if (e != hipSuccess) return e;
-    k_refine_finish<<<1, 64, 0, c->stream>>>(c->gn_partials, n_part, c->state, it);
+    k_refine_finish<<<1, 256, 0, c->stream>>>(c->gn_partials, n_part, c->state, it);
   }
   return hipGetLastError();
 }

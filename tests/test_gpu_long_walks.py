@@ -1,16 +1,17 @@
-"""The raycast's brick summary (one bit per 8^3 brick: "may hold F < 0") lets
-the second raycast pass skip TSDF loads where no test of the reference walk
-can fire.  These tests check that the skipping never changes a result:
+"""Long raycast walks.  The rays still walking after the first raycast pass
+(a few thousand per 640x480 frame, up to ~400 steps: bilateral-filtered hole
+pixels whose depth of a few 1e-5 mm starts them behind the camera) are
+finished by the second pass, one wave per ray, 64 steps per round.  These
+tests check that it walks exactly the reference's voxels:
 
-  * the summary that integration maintains (bits set, never cleared): after a
-    GPU pipeline run, the GPU's own model maps equal the oracle's raycast of
-    the GPU's own volume, bit for bit, in F32 and F16;
-  * the summary that a volume upload recomputes, on an adversarial volume:
-    free space of NaN, positive values, +0 and -0, isolated negative voxels in
-    a fraction of the bricks, and rays that start behind the camera (depth of
-    a few 1e-5 mm, as a bilateral-filtered hole has) so that they walk the
-    whole volume.
-Bar: bit-exact hit indices and maps, as in the main parity tests."""
+  * after a GPU pipeline run, the GPU's own model maps equal the oracle's
+    raycast of the GPU's own volume, bit for bit, with the same number of
+    visited voxels, in F32 and F16;
+  * adversarial volumes: free space of NaN, positive values, +0 and -0,
+    isolated negative voxels in a fraction of the bricks, and rays that start
+    behind the camera so that they walk the whole volume.
+Bar: bit-exact hit indices and maps and exact work counters, as in the main
+parity tests."""
 import numpy as np
 import pytest
 
@@ -31,7 +32,7 @@ def test_pipeline_maps_equal_oracle_raycast_of_own_volume(kf, oracle, room_strea
     vol, prm, K, H, W, frames, gt = room_stream
     vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
     vol.precision = kf.KF_TSDF_F32 if precision == "f32" else kf.KF_TSDF_F16
-    say(f"[gpu] brick summary: 10-frame pipeline, {precision}")
+    say(f"[gpu] long walks: 10-frame pipeline, {precision}")
     with kf.KinfuContext(vol, prm, H, W) as c:
         c.stream_upload(frames)
         c.pipeline_run(K, 0, len(frames))
@@ -73,7 +74,7 @@ def adversarial_volume(rng, dims, neg_brick_frac):
 
 
 @pytest.mark.parametrize("neg_brick_frac", [0.0, 0.05, 0.3])
-def test_upload_summary_adversarial_volume(kf, oracle, neg_brick_frac):
+def test_adversarial_volume_long_walks(kf, oracle, neg_brick_frac):
     rng = np.random.default_rng(int(neg_brick_frac * 1000) + 7)
     vol, prm, K, H, W = small_setup(kf, H=96, W=128, dims=(70, 66, 75), start=(-35, -33, -10),
                                     voxel=20.0)
@@ -83,7 +84,7 @@ def test_upload_summary_adversarial_volume(kf, oracle, neg_brick_frac):
     d[rng.random((H, W)) < 0.5] = np.float32(4.8e-5)  # rays from behind the camera
     d[rng.random((H, W)) < 0.05] = 0.0
     T = np.eye(4, dtype=np.float32)
-    say(f"[gpu] brick summary: adversarial volume, negative brick fraction {neg_brick_frac}")
+    say(f"[gpu] long walks: adversarial volume, negative brick fraction {neg_brick_frac}")
     with kf.KinfuContext(vol, prm, H, W) as c:
         c.upload_volume(t, w)
         c.upload_depth_f32(d)
