@@ -14,8 +14,9 @@ Runs, each on a fresh context over the same frames:
   3. FP32 with per-stage hipEvents               -> stage_ms, roofline
   4. FP32 with the raycast work counters on      -> algorithmic bytes
   5. FP32 with the trilinear raycast, timed      -> trilinear.value, ATE
-  6. the same with stage events, then 20 raycast + raycast-adjoint pairs on
-     the final state                             -> adjoint cost
+  6. the same with stage events, then on the final state, after one untimed
+     pair each, 20 raycast + pose-only adjoint pairs and 10 raycast +
+     pose-and-TSDF adjoint pairs                 -> adjoint cost
 and the CPU oracle on a bounded sample of the same workload.
 
 Prints one JSON line.  The roofline object is for the raycast stage (the
@@ -63,8 +64,9 @@ def ate_mm(poses, gt):
 def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, mode=0,
             adjoint_reps=0):
     """One pass over frames[0 .. warmup+steps]; times frames warmup+1 ...
-    With adjoint_reps, then times that many (raycast, raycast adjoint) pairs
-    on the final volume, pose and depth."""
+    With adjoint_reps, then times that many (raycast, pose-only raycast
+    adjoint) pairs and half as many (raycast, adjoint with dL/dTSDF) pairs on
+    the final volume, pose and depth, each after one untimed pair."""
     vol, prm, _, H, W = kf.preset(kf.PRESET_ROOM)
     vol.precision = precision
     prm.stage_timing = 1 if timing else 0
@@ -92,8 +94,12 @@ def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, 
             ones = np.ones((H, W), np.float32)
             work = {}
             for full in (False, True):
+                # one untimed pair first, then the timed pairs (a full adjoint
+                # also copies dL/dTSDF to the host, outside the timed stage)
+                ctx.raycast(K, T)
+                ctx.raycast_adjoint(K, T, ones, want_tsdf=full)
                 ctx.reset_stage_times()
-                for _ in range(adjoint_reps if not full else max(1, adjoint_reps // 4)):
+                for _ in range(adjoint_reps if not full else max(1, adjoint_reps // 2)):
                     ctx.raycast(K, T)
                     ctx.raycast_adjoint(K, T, ones, want_tsdf=full)
                 adj = ctx.stage_times()

@@ -367,7 +367,7 @@ int kf_destroy(kf_context* c) {
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
-                  c->adj_partials, c->gn_partials};
+                  c->adj_partials, c->gn_partials, c->adj_touched};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -436,11 +436,11 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   c->n_store = (int64_t)c->n_bricks * 512;
 
   const size_t n = (size_t)height * width;
-  for (c->icp_px = 1; c->icp_px < 4; c->icp_px *= 2)
-    if ((n + (size_t)KF_ICP_BLOCK * c->icp_px - 1) / ((size_t)KF_ICP_BLOCK * c->icp_px) <=
-        KF_ICP_MAX_BLOCKS)
-      break;
-  c->icp_blocks = (int)((n + (size_t)KF_ICP_BLOCK * c->icp_px - 1) / ((size_t)KF_ICP_BLOCK * c->icp_px));
+  auto icp_blocks_for = [&](int px) {
+    return (int)((n + (size_t)KF_ICP_BLOCK * px - 1) / ((size_t)KF_ICP_BLOCK * px));
+  };
+  c->icp_px = 4;
+  c->icp_blocks = icp_blocks_for(4);
   c->volume_bytes = (size_t)c->n_store * (vol->precision == KF_TSDF_F32 ? 8 : 4);
   if (c->icp_blocks > KF_ICP_MAX_BLOCKS) {
     delete c;
@@ -452,6 +452,22 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
     return kf_fail(code, m);
   };
   if (hipGetDevice(&c->device) != hipSuccess) return fail(KF_ERR_HIP, "no HIP device");
+  {
+    // ICP: the fewest pixels per thread (1 to 4) that keep the grid within
+    // one block per compute unit; a CU with two blocks would set the time of
+    // every iteration.  Larger images take 4 and up to KF_ICP_MAX_BLOCKS blocks.
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        n_cu <= 0)
+      n_cu = KF_ICP_MAX_BLOCKS;
+    for (int px = 1; px <= 4; ++px)
+      if (icp_blocks_for(px) <= n_cu) {
+        c->icp_px = px;
+        c->icp_blocks = icp_blocks_for(px);
+        break;
+      }
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(KF_ERR_HIP, "hipStreamCreate failed");
   struct Alloc {
@@ -956,6 +972,12 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
   const size_t n = (size_t)c->H * c->W;
   if (!c->grad_tsdf) {
     KF_HIP_CHECK(hipMalloc((void**)&c->grad_tsdf, (size_t)c->n_store * 4));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_touched, (size_t)((c->n_bricks + 31) / 32) * 4));
+    // all zero once; afterwards each adjoint zeroes the bricks the previous
+    // one wrote
+    KF_HIP_CHECK(hipMemsetAsync(c->grad_tsdf, 0, (size_t)c->n_store * 4, c->stream));
+    KF_HIP_CHECK(hipMemsetAsync(c->adj_touched, 0, (size_t)((c->n_bricks + 31) / 32) * 4,
+                                c->stream));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_depth, n * 4));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_normal, n * 16));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_partials, kf_adjoint_partials(c) * 8));

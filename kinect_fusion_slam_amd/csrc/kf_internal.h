@@ -69,8 +69,8 @@ __host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int
 #define KF_ICP_BLOCK 512
 #endif
 // Block partials one lane reads per row when an iteration is finished: the
-// pixels per thread (1, 2 or 4) are chosen so that at most 64 * 8 blocks
-// cover the image (up to 2^21 pixels).
+// pixels per thread (1 to 4) are chosen so that at most 64 * 8 blocks cover
+// the image (up to 2^21 pixels).
 #define KF_ICP_MAX_PART_PER_LANE 8
 #define KF_ICP_MAX_BLOCKS (64 * KF_ICP_MAX_PART_PER_LANE)
 
@@ -128,6 +128,8 @@ struct kf_context {
   int ray_slot = 0;         // ray_queue slot of the next raycast
   // raycast adjoint buffers (allocated on first use)
   float* grad_tsdf = nullptr;   // dL/dF, n_store floats, native order
+  uint32_t* adj_touched = nullptr; // one bit per brick that holds non-zero dL/dF
+                                   // (zeroed before the next adjoint)
   float* adj_depth = nullptr;   // dL/d(rendered depth), H*W
   float4* adj_normal = nullptr; // dL/d(normal), H*W
   bool adj_normal_set = false;

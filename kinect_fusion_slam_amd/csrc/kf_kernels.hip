@@ -1133,6 +1133,7 @@ struct RayIO {
   const float* gD;     // dL/d(depth), H*W
   const float4* gN;    // dL/d(normal), H*W (nullable)
   float* gF;           // dL/d(TSDF), native store order (null: not wanted)
+  uint32_t* touched;   // bricks that receive dL/dF (one bit each)
   double* xi_part;     // dL/dxi partials, 6 per block
   int part_base;       // first partial slot of this launch
   float* gn_part;      // refinement partials, kGnSums per block
@@ -1348,7 +1349,14 @@ struct Sink {
   float* gF;   // global dL/dF (native order); null: no TSDF gradient
   int* key;    // LDS table keys (-1 = empty); null: global atomics only
   float* val;  // LDS table values
+  uint32_t* touched;  // one bit per brick that received dL/dF (for zeroing)
 };
+
+// Marks brick b as holding dL/dF (a bit already seen set is not set again).
+__device__ __forceinline__ void mark_brick(uint32_t* touched, int b) {
+  const uint32_t m = 1u << (b & 31);
+  if (!(touched[b >> 5] & m)) atomicOr(&touched[b >> 5], m);
+}
 
 __device__ __forceinline__ void sink_add(const Sink& k, int idx, float v) {
   if (k.key) {
@@ -1362,6 +1370,7 @@ __device__ __forceinline__ void sink_add(const Sink& k, int idx, float v) {
       h = (h + 1) & (kHash - 1);
     }
   }
+  mark_brick(k.touched, idx >> 9);
   atomicAdd(&k.gF[idx], v);
 }
 
@@ -1386,7 +1395,7 @@ __device__ __forceinline__ void tri_scatter(const Sink& sink, const kf_vol& vg, 
 // te.  MODE 1 writes vertex (w = depth) and normal; MODE 2 adds the adjoint
 // contributions of this pixel (dL/dF scattered, dL/dxi into xi).
 template <int PREC, int MODE>
-__device__ __noinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, const float* T,
+__device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, const float* T,
                                      int pix, float te, float dx, float dy, float dz,
                                      float4& ov, float4& on, double* xi, const Sink& sink,
                                      float* acc) {
@@ -1850,7 +1859,7 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 // to the 8 XCDs; the id is remapped so that each XCD gets a contiguous band
 // of the image and its L2 sees the band's voxels.
 #ifndef KF_PASS1_BATCHES
-#define KF_PASS1_BATCHES 2
+#define KF_PASS1_BATCHES 1
 #endif
 constexpr int kPass1Batches = KF_PASS1_BATCHES;  // of 16 steps
 
@@ -1880,15 +1889,19 @@ __global__ __launch_bounds__(256) void k_raycast(
   constexpr int kTable = MODE == kModeAdjoint ? kHash : 1;
   __shared__ int s_key[kTable];
   __shared__ float s_val[kTable];
+  // bricks this block has marked in io.touched (a small open-addressing set)
+  constexpr int kBrickSet = MODE == kModeAdjoint ? 64 : 1;
+  __shared__ int s_brick[kBrickSet];
   const bool use_table = MODE == kModeAdjoint && io.gF != nullptr;
   if (use_table) {
     for (int e = threadIdx.x; e < kTable; e += 256) {
       s_key[e] = -1;
       s_val[e] = 0.0f;
     }
+    if (threadIdx.x < kBrickSet) s_brick[threadIdx.x] = -1;
     __syncthreads();
   }
-  const Sink sink{io.gF, use_table ? s_key : nullptr, s_val};
+  const Sink sink{io.gF, use_table ? s_key : nullptr, s_val, io.touched};
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
   int hit = -1;
   bool pending = false;
@@ -1920,8 +1933,22 @@ __global__ __launch_bounds__(256) void k_raycast(
   }
   if (use_table) {
     __syncthreads();
-    for (int e = threadIdx.x; e < kTable; e += 256)
-      if (s_key[e] >= 0 && s_val[e] != 0.0f) atomicAdd(&io.gF[s_key[e]], s_val[e]);
+    for (int e = threadIdx.x; e < kTable; e += 256) {
+      const int key = s_key[e];
+      if (key < 0 || s_val[e] == 0.0f) continue;
+      // the first thread of the block to meet a brick marks it globally
+      const int b = key >> 9;
+      uint32_t h = (((uint32_t)b * 2654435761u) >> 26) & (kBrickSet - 1);
+      bool seen = false, placed = false;
+      for (int probe = 0; probe < kBrickSet && !seen && !placed; ++probe) {
+        const int old = atomicCAS(&s_brick[h], -1, b);
+        seen = old == b;
+        placed = old == -1;
+        h = (h + 1) & (kBrickSet - 1);
+      }
+      if (!seen) mark_brick(io.touched, b);
+      atomicAdd(&io.gF[key], s_val[e]);
+    }
   }
   if (!active || MODE == kModeAdjoint || MODE == kModeRefine) return;
   out_v[p] = ov;
@@ -1969,7 +1996,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     uniform_ray(r);
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
-    const Sink sink{io.gF, nullptr, nullptr};
+    const Sink sink{io.gF, nullptr, nullptr, io.touched};
     const int owner =
         wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, gn);
     if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
@@ -1986,28 +2013,44 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     block_gn(gn, io.gn_part + kGnSums * (io.part_base + blockIdx.x));
 }
 
-// Zeroes n4 float4 of dL/dF.
-__global__ __launch_bounds__(256) void k_zero_adjoint(float4* gF, int64_t n4) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n4) gF[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+// Zeroes dL/dF in the bricks the previous adjoint wrote and clears their
+// bits: one wave per 32-brick word, 2 float4 per lane per brick.
+__global__ __launch_bounds__(256) void k_zero_touched(float4* __restrict__ gF,
+                                                      uint32_t* __restrict__ touched,
+                                                      int n_words) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= n_words) return;
+  uint32_t bits = touched[w];
+  if (!bits) return;
+  while (bits) {
+    const int b = w * 32 + __ffs(bits) - 1;
+    bits &= bits - 1;
+    float4* p = gF + (size_t)b * 128;
+    p[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    p[lane + 64] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (lane == 0) touched[w] = 0u;
 }
 
-// dL/dxi = the sum of the n_part block partials, in a fixed order.
+// dL/dxi: the FP64 sums of the block partials in a fixed order (thread t adds
+// blocks t, t + 256, ..., then a tree over the 256 threads).
 __global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict__ part,
                                                         int n_part, kf_dev_state* st) {
-  __shared__ double s[256];
-  for (int q = 0; q < 6; ++q) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < n_part; b += 256) v += part[6 * b + q];
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) st->xi_grad[q] = s[0];
+  __shared__ double red[6][256];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < n_part; b += 256)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] += part[6 * b + q];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x < 6) st->xi_grad[threadIdx.x] = red[threadIdx.x][0];
 }
 
 
@@ -2192,6 +2235,8 @@ hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
       return kahan ? launch_icp_t<true, 1>(c, cam) : launch_icp_t<false, 1>(c, cam);
     case 2:
       return kahan ? launch_icp_t<true, 2>(c, cam) : launch_icp_t<false, 2>(c, cam);
+    case 3:
+      return kahan ? launch_icp_t<true, 3>(c, cam) : launch_icp_t<false, 3>(c, cam);
     default:
       return kahan ? launch_icp_t<true, 4>(c, cam) : launch_icp_t<false, 4>(c, cam);
   }
@@ -2239,6 +2284,7 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   io.gD = c->adj_depth;
   io.gN = c->adj_normal_set ? c->adj_normal : nullptr;
   io.gF = c->adj_want_tsdf ? c->grad_tsdf : nullptr;
+  io.touched = c->adj_touched;
   io.xi_part = c->adj_partials;
   io.part_base = 0;
   io.gn_part = c->gn_partials;
@@ -2315,10 +2361,9 @@ size_t kf_adjoint_partials(const kf_context* c) {
 
 hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam) {
   if (c->adj_want_tsdf) {
-    // n_store is a whole number of 512-voxel bricks
-    const int64_t n4 = c->n_store / 4;
-    k_zero_adjoint<<<(unsigned)((n4 + 255) / 256), 256, 0, c->stream>>>(
-        reinterpret_cast<float4*>(c->grad_tsdf), n4);
+    const int n_words = (c->n_bricks + 31) / 32;
+    k_zero_touched<<<(unsigned)((n_words + 3) / 4), 256, 0, c->stream>>>(
+        reinterpret_cast<float4*>(c->grad_tsdf), c->adj_touched, n_words);
   }
   hipError_t e = launch_raycast_mode(c, cam, kModeAdjoint);
   if (e != hipSuccess) return e;
