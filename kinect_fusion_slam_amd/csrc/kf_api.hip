@@ -365,6 +365,7 @@ int kf_destroy(kf_context* c) {
   timing_drop(c);
   void* bufs[] = {c->ray_queue, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
+                  c->integ_tiles,
                   c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
                   c->adj_partials, c->gn_partials, c->adj_touched};
@@ -479,6 +480,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->nmap, n * 16},       {(void**)&c->mask, n},
       {(void**)&c->model_v, n * 16},    {(void**)&c->model_n, n * 16},
       {(void**)&c->hit, n * 4},         {(void**)&c->integ_tab, n * 8},
+      {(void**)&c->integ_tiles, (size_t)((width + 15) / 16) * ((height + 15) / 16) * 8},
       {&c->ray_queue, n * kf_ray_state_bytes()},
       {(void**)&c->spatial, 4 * (2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)},
       {(void**)&c->state, sizeof(kf_dev_state)},

@@ -94,6 +94,8 @@ struct kf_context {
   int32_t* hit = nullptr;       // raycast hit voxel index
   void* ray_queue = nullptr;    // H*W ray states between the two raycast passes
   float2* integ_tab = nullptr;  // per-pixel {predicted depth, 1/|R lambda|}
+  float2* integ_tiles = nullptr;// per 16x16 tile: {min, max} camera distance a voxel
+                                // projecting there can be updated at (NaN-free)
   float* spatial = nullptr;     // bilateral spatial weights
 
   // volume (one allocation: TSDF array, then weight array, brick-tiled)

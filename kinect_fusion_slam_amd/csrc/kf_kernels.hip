@@ -726,15 +726,27 @@ __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* 
 // with the reference's expressions (:39-52, :78-94).  Depth 0 stores NaN.
 // Thread 0 also stores the front-test direction (:57-71), which depends only
 // on K and T.
+//
+// A 256-thread block is a 16x16 pixel tile.  It also stores the tile's band
+// bounds: a voxel at camera distance tm projecting to pixel p is updated only
+// when sdf_min <= frk = tab.x - tab.y tm <= sdf_max, i.e. (tab.y > 0) when
+// tm lies in [(tab.x - sdf_max) / tab.y, (tab.x - sdf_min) / tab.y].  The
+// tile keeps the smallest lower and the largest upper end over its pixels
+// with depth (FP64, then rounded outwards to float); a tile without depth
+// stores {+inf, -inf}.  Integration culls bricks against them.
 __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ depth, kf_cam cam,
                                                      int H, int W, kf_dev_state* __restrict__ st,
-                                                     float2* __restrict__ tab) {
+                                                     float2* __restrict__ tab,
+                                                     float2* __restrict__ tiles, float smin,
+                                                     float smax) {
   float T[12];
   load_uniform<12>(st->T, T);
   const float* Ki = cam.Kinv;
   const float* K = cam.K;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i == 0) {
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const bool inside = px < W && py < H;
+  const int i = inside ? py * W + px : -1;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     const float cax = Ki[0] * K[5] + Ki[1] * K[2] + Ki[2] * 1.f;
     const float cay = Ki[3] * K[5] + Ki[4] * K[2] + Ki[5] * 1.f;
     const float caz = Ki[6] * K[5] + Ki[7] * K[2] + Ki[8] * 1.f;
@@ -752,33 +764,54 @@ __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ d
     st->center_dir[2] = gbz - gaz;
     st->center_dir[3] = 0.f;
   }
-  if (i >= H * W) return;
-  const float dv = depth[i];
-  float2 out = make_float2(__builtin_nanf(""), __builtin_nanf(""));
-  if (dv != 0) {
-    const int y = i / W, x = i - y * W;
-    const float U = (float)(x + 1), V = (float)(y + 1);
-    const float lamx = Ki[0] * U + Ki[1] * V + Ki[2] * 1.f;
-    const float lamy = Ki[3] * U + Ki[4] * V + Ki[5] * 1.f;
-    const float lamz = Ki[6] * U + Ki[7] * V + Ki[8] * 1.f;
-    const float rsx = T[0] * lamx + T[1] * lamy + T[2] * lamz + T[3];
-    const float rsy = T[4] * lamx + T[5] * lamy + T[6] * lamz + T[7];
-    const float rsz = T[8] * lamx + T[9] * lamy + T[10] * lamz + T[11];
-    const float rdx = T[0] * 2 * lamx + T[1] * 2 * lamy + T[2] * 2 * lamz + T[3];
-    const float rdy = T[4] * 2 * lamx + T[5] * 2 * lamy + T[6] * 2 * lamz + T[7];
-    const float rdz = T[8] * 2 * lamx + T[9] * 2 * lamy + T[10] * 2 * lamz + T[11];
-    const float dx = rdx - rsx, dy = rdy - rsy, dz = rdz - rsz;
-    const float rm = sqrtf(dx * dx + dy * dy + dz * dz);
-    const float p3x = dv * lamx, p3y = dv * lamy, p3z = dv * lamz;
-    const float pgx = T[0] * p3x + T[1] * p3y + T[2] * p3z + T[3];
-    const float pgy = T[4] * p3x + T[5] * p3y + T[6] * p3z + T[7];
-    const float pgz = T[8] * p3x + T[9] * p3y + T[10] * p3z + T[11];
-    const float rx = pgx - T[3], ry = pgy - T[7], rz = pgz - T[11];
-    const float rpd = sqrtf(rx * rx + ry * ry + rz * rz);
-    out.x = rpd / rm;
-    out.y = 1.f / rm;
+  double lo = __builtin_inf(), hi = -__builtin_inf();
+  if (inside) {
+    const float dv = depth[i];
+    float2 out = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+    if (dv != 0) {
+      const int y = i / W, x = i - y * W;
+      const float U = (float)(x + 1), V = (float)(y + 1);
+      const float lamx = Ki[0] * U + Ki[1] * V + Ki[2] * 1.f;
+      const float lamy = Ki[3] * U + Ki[4] * V + Ki[5] * 1.f;
+      const float lamz = Ki[6] * U + Ki[7] * V + Ki[8] * 1.f;
+      const float rsx = T[0] * lamx + T[1] * lamy + T[2] * lamz + T[3];
+      const float rsy = T[4] * lamx + T[5] * lamy + T[6] * lamz + T[7];
+      const float rsz = T[8] * lamx + T[9] * lamy + T[10] * lamz + T[11];
+      const float rdx = T[0] * 2 * lamx + T[1] * 2 * lamy + T[2] * 2 * lamz + T[3];
+      const float rdy = T[4] * 2 * lamx + T[5] * 2 * lamy + T[6] * 2 * lamz + T[7];
+      const float rdz = T[8] * 2 * lamx + T[9] * 2 * lamy + T[10] * 2 * lamz + T[11];
+      const float dx = rdx - rsx, dy = rdy - rsy, dz = rdz - rsz;
+      const float rm = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float p3x = dv * lamx, p3y = dv * lamy, p3z = dv * lamz;
+      const float pgx = T[0] * p3x + T[1] * p3y + T[2] * p3z + T[3];
+      const float pgy = T[4] * p3x + T[5] * p3y + T[6] * p3z + T[7];
+      const float pgz = T[8] * p3x + T[9] * p3y + T[10] * p3z + T[11];
+      const float rx = pgx - T[3], ry = pgy - T[7], rz = pgz - T[11];
+      const float rpd = sqrtf(rx * rx + ry * ry + rz * rz);
+      out.x = rpd / rm;
+      out.y = 1.f / rm;
+      lo = ((double)out.x - (double)smax) / (double)out.y;
+      hi = ((double)out.x - (double)smin) / (double)out.y;
+    }
+    tab[i] = out;
   }
-  tab[i] = out;
+  // tile bounds: wave min / max, then over the 4 waves
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o));
+    hi = fmax(hi, __shfl_xor(hi, o));
+  }
+  __shared__ double s_lo[4], s_hi[4];
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double l = fmin(fmin(s_lo[0], s_lo[1]), fmin(s_lo[2], s_lo[3]));
+    const double h = fmax(fmax(s_hi[0], s_hi[1]), fmax(s_hi[2], s_hi[3]));
+    tiles[blockIdx.y * gridDim.x + blockIdx.x] =
+        make_float2(__double2float_rd(l), __double2float_ru(h));
+  }
 }
 
 // Per-voxel part.  A 256-thread block owns 32 consecutive bricks (8x8x8
@@ -812,7 +845,8 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
                                                    kf_vol vg, int H, int W, float trunc,
                                                    float smin, float smax,
                                                    const kf_dev_state* __restrict__ st,
-                                                   int nbricks) {
+                                                   int nbricks, const float2* __restrict__ tiles,
+                                                   int tiles_x, int tiles_y) {
   __shared__ int keep_list[kBricksPerBlock];
   __shared__ uint32_t keep_mask[4];
   __shared__ int n_keep;
@@ -830,7 +864,7 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
   {  // brick cull: thread = (brick of the block, corner)
     const int brick = blockIdx.x * kBricksPerBlock + (int)(threadIdx.x >> 3);
     const int corner = threadIdx.x & 7;
-    bool r1 = true, r2 = true, r3 = true, r4 = true, r5 = true;
+    bool r1 = true, r2 = true, r3 = true, r4 = true, r5 = true, r6 = false;
     if (brick < nbricks) {
       const int bz = brick % vg.bz;
       const int q = brick / vg.bz;
@@ -862,16 +896,70 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
       r3 = front && px > ((double)W + 0.55) * pz + M;
       r4 = front && py < 0.45 * pz - M;
       r5 = front && py > ((double)H + 0.55) * pz + M;
+      // band cull: every voxel of the brick is nearer than the updatable
+      // distances of every pixel it can project to, or farther
+      double u = front ? px / pz : 0.0, v = front ? py / pz : 0.0;
+      double umin = u, umax = u, vmin = v, vmax = v;
+      double dmax = sqrt(tx * tx + ty * ty + tz * tz);
+      double xmin = cx, xmax = cx, ymin = cy, ymax = cy, zmin = cz, zmax = cz;
+      int all_front = front ? 1 : 0;
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        umin = fmin(umin, __shfl_xor(umin, o, 8));
+        umax = fmax(umax, __shfl_xor(umax, o, 8));
+        vmin = fmin(vmin, __shfl_xor(vmin, o, 8));
+        vmax = fmax(vmax, __shfl_xor(vmax, o, 8));
+        dmax = fmax(dmax, __shfl_xor(dmax, o, 8));
+        xmin = fmin(xmin, __shfl_xor(xmin, o, 8));
+        xmax = fmax(xmax, __shfl_xor(xmax, o, 8));
+        ymin = fmin(ymin, __shfl_xor(ymin, o, 8));
+        ymax = fmax(ymax, __shfl_xor(ymax, o, 8));
+        zmin = fmin(zmin, __shfl_xor(zmin, o, 8));
+        zmax = fmax(zmax, __shfl_xor(zmax, o, 8));
+        all_front = all_front & __shfl_xor(all_front, o, 8);
+      }
+      if (all_front) {
+        // footprint: the voxels project inside the hull of the corner
+        // projections; one pixel more on each side covers roundf and the
+        // float evaluation
+        const int u0 = (int)fmax(floor(umin) - 1.0, 1.0), u1 = (int)fmin(ceil(umax) + 1.0, (double)W);
+        const int v0 = (int)fmax(floor(vmin) - 1.0, 1.0), v1 = (int)fmin(ceil(vmax) + 1.0, (double)H);
+        if (u0 <= u1 && v0 <= v1) {
+          const int tx0 = (u0 - 1) >> 4, tx1 = (u1 - 1) >> 4;
+          const int ty0 = (v0 - 1) >> 4, ty1 = (v1 - 1) >> 4;
+          const int ntx = tx1 - tx0 + 1, nt = ntx * (ty1 - ty0 + 1);
+          if (nt <= 64) {
+            float lo = __builtin_inff(), hi = -__builtin_inff();
+            for (int k = corner; k < nt; k += 8) {
+              const float2 b = tiles[(ty0 + k / ntx) * tiles_x + tx0 + k % ntx];
+              lo = fminf(lo, b.x);
+              hi = fmaxf(hi, b.y);
+            }
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1) {
+              lo = fminf(lo, __shfl_xor(lo, o, 8));
+              hi = fmaxf(hi, __shfl_xor(hi, o, 8));
+            }
+            // nearest voxel centre of the brick
+            const double ex = fmax(fmax(xmin - (double)t[0], (double)t[0] - xmax), 0.0);
+            const double ey = fmax(fmax(ymin - (double)t[1], (double)t[1] - ymax), 0.0);
+            const double ez = fmax(fmax(zmin - (double)t[2], (double)t[2] - zmax), 0.0);
+            const double dmin = sqrt(ex * ex + ey * ey + ez * ez);
+            const double margin = 1.0 + 1e-5 * dmax;
+            r6 = dmax + margin < (double)lo || dmin - margin > (double)hi;
+          }
+        }
+      }
     }
     const uint64_t m1 = __ballot(r1), m2 = __ballot(r2), m3 = __ballot(r3), m4 = __ballot(r4),
-                   m5 = __ballot(r5);
+                   m5 = __ballot(r5), m6 = __ballot(r6);
     if (lane == 0) {
       uint32_t keep = 0;
       for (int j = 0; j < 8; ++j) {
         const int sh = 8 * j;
         const bool rej = ((m1 >> sh) & 0xff) == 0xff || ((m2 >> sh) & 0xff) == 0xff ||
                          ((m3 >> sh) & 0xff) == 0xff || ((m4 >> sh) & 0xff) == 0xff ||
-                         ((m5 >> sh) & 0xff) == 0xff;
+                         ((m5 >> sh) & 0xff) == 0xff || ((m6 >> sh) & 0xff) == 0xff;
         if (!rej) keep |= 1u << j;
       }
       keep_mask[wave] = keep;
@@ -2243,9 +2331,10 @@ hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
 }
 
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam) {
-  const int n = c->H * c->W;
-  k_integ_table<<<(n + 255) / 256, 256, 0, c->stream>>>(c->depth, cam, c->H, c->W, c->state,
-                                                        c->integ_tab);
+  const dim3 grid((c->W + 15) / 16, (c->H + 15) / 16);
+  k_integ_table<<<grid, 256, 0, c->stream>>>(c->depth, cam, c->H, c->W, c->state, c->integ_tab,
+                                             c->integ_tiles, c->prm.sdf_min_mm,
+                                             c->prm.sdf_max_mm);
   return hipGetLastError();
 }
 
@@ -2256,7 +2345,8 @@ hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
   k_integrate<P, FI><<<blocks, 256, 0, c->stream>>>(c->tsdf, c->weight, c->integ_tab, cam, \
                                                     c->vg, c->H, c->W, p.truncation_mm,    \
                                                     p.sdf_min_mm, p.sdf_max_mm, c->state,  \
-                                                    c->n_bricks)
+                                                    c->n_bricks, c->integ_tiles,           \
+                                                    (c->W + 15) / 16, (c->H + 15) / 16)
   if (c->vol.precision == KF_TSDF_F32) {
     if (first)
       KF_INTEG(KF_TSDF_F32, true);
