@@ -20,9 +20,6 @@ struct kf_dev_state {
   float F[16];          // Tprev_inv * T (frame_to_frame, sensor_pose_estimation.cu:343)
   float center_dir[4];  // integration front-test direction (projective_tsdf.cu:57-71)
   int32_t icp_iters_done;
-  uint32_t ray_queue[2];  // rays queued by the first raycast pass, two slots used
-                          // by alternate raycasts (each clears the other's)
-  int32_t pad;
   unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
   unsigned long long ray_hits;
   double xi_grad[6];             // raycast adjoint: dL/dxi (omega, v)
@@ -59,6 +56,17 @@ struct kf_vol {
 
 #define KF_BRICK 8
 
+// Ray queue between the two raycast passes: KF_RAY_PARTS partitions, each
+// with its own append counter on its own 256-byte line (a single counter
+// serialises the appends of every wave), two counter slots used by alternate
+// raycasts (each clears the other's).  First-pass block b appends to
+// partition b % KF_RAY_PARTS, whose region holds the rays of all its blocks.
+#define KF_RAY_PARTS 64
+#define KF_RAY_COUNT_STRIDE 64  // uint32 per counter line
+__host__ __device__ inline int kf_ray_part_capacity(int n_blocks) {
+  return (n_blocks + KF_RAY_PARTS - 1) / KF_RAY_PARTS * 256;
+}
+
 __host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int z) {
   return ((((x >> 3) * v.by + (y >> 3)) * v.bz + (z >> 3)) << 9) | ((x & 7) << 6) |
          ((y & 7) << 3) | (z & 7);
@@ -92,7 +100,8 @@ struct kf_context {
   float4* model_v = nullptr;    // raycast vertices {x, y, z, 0}
   float4* model_n = nullptr;    // raycast normals
   int32_t* hit = nullptr;       // raycast hit voxel index
-  void* ray_queue = nullptr;    // H*W ray states between the two raycast passes
+  void* ray_queue = nullptr;    // KF_RAY_PARTS x kf_ray_part_capacity ray states
+  uint32_t* ray_counts = nullptr; // [2][KF_RAY_PARTS * KF_RAY_COUNT_STRIDE] append counters
   float2* integ_tab = nullptr;  // per-pixel {predicted depth, 1/|R lambda|}
   float2* integ_tiles = nullptr;// per 16x16 tile: {min, max} camera distance a voxel
                                 // projecting there can be updated at (NaN-free)
@@ -127,7 +136,7 @@ struct kf_context {
   int64_t stage_count[KF_STAGE_COUNT] = {};
   bool have_depth = false;
   bool count_rays = false;  // raycast work counters on
-  int ray_slot = 0;         // ray_queue slot of the next raycast
+  int ray_slot = 0;         // ray_counts slot of the next raycast
   // raycast adjoint buffers (allocated on first use)
   float* grad_tsdf = nullptr;   // dL/dF, n_store floats, native order
   uint32_t* adj_touched = nullptr; // one bit per brick that holds non-zero dL/dF
@@ -169,6 +178,7 @@ hipError_t kf_launch_set_pose(kf_context* c, const float* T_host);
 hipError_t kf_launch_record_pose(kf_context* c, int frame);
 
 size_t kf_ray_state_bytes();
+size_t kf_ray_queue_entries(int H, int W);
 
 // shared host helpers
 void kf_make_cam(const float* K, kf_cam* cam);

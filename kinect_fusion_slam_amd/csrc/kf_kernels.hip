@@ -1226,7 +1226,9 @@ struct RayIO {
   int part_base;       // first partial slot of this launch
   float* gn_part;      // refinement partials, kGnSums per block
   float max_res;       // refinement: residuals beyond this (mm) are ignored
-  int qslot;           // ray_queue slot of this raycast
+  uint32_t* qcount;    // this raycast's partition append counters
+  uint32_t* qnext;     // the next raycast's counters (cleared by the second pass)
+  int qcap;            // rays per queue partition
 };
 
 // Raycast modes: 0 = the reference's hit handling (voxel_traversal.cu:222-
@@ -1949,7 +1951,11 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 #ifndef KF_PASS1_BATCHES
 #define KF_PASS1_BATCHES 1
 #endif
-constexpr int kPass1Batches = KF_PASS1_BATCHES;  // of 16 steps
+#ifndef KF_PASS1_STEPS
+#define KF_PASS1_STEPS 16
+#endif
+constexpr int kPass1Batches = KF_PASS1_BATCHES;
+constexpr int kPass1Steps = KF_PASS1_STEPS;  // steps per batch
 
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast(
@@ -2003,21 +2009,24 @@ __global__ __launch_bounds__(256) void k_raycast(
   if (dval != 0) {
     if (ray_setup(x, y, dval, io.Ki, T, io.vg, io.trunc, r)) {
       r.pix = p;
-      pending = !ray_walk<PREC, 16, COUNT, MODE>(F, r, io, T, kPass1Batches, ov, on, hit,
+      pending = !ray_walk<PREC, kPass1Steps, COUNT, MODE>(F, r, io, T, kPass1Batches, ov, on, hit,
                                                  visited, xi, sink, gn);
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * blockIdx.x);
   if constexpr (MODE == kModeRefine) block_gn(gn, io.gn_part + kGnSums * blockIdx.x);
-  // wave-aggregated append of the unfinished rays
+  // wave-aggregated append of the unfinished rays to this block's partition
   const uint64_t mask = __ballot(pending);
   if (mask) {
     const int leader = __ffsll((unsigned long long)mask) - 1;
+    const int part = blockIdx.x & (KF_RAY_PARTS - 1);
     uint32_t base = 0;
-    if (ln == leader) base = atomicAdd(&st->ray_queue[io.qslot], (uint32_t)__popcll(mask));
+    if (ln == leader)
+      base = atomicAdd(&io.qcount[part * KF_RAY_COUNT_STRIDE], (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
-    if (pending) queue[base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
+    if (pending)
+      queue[(size_t)part * io.qcap + base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
   if (use_table) {
     __syncthreads();
@@ -2059,18 +2068,30 @@ __device__ __forceinline__ void uniform_ray(RayState& r) {
 }
 
 // Pass 2: finishes the queued rays, one ray per wave (wave_walk).  Rays are
-// dealt to waves in a fixed order (wave w takes rays w, w + waves, ...), so
-// that each block's gradient partial sums do not depend on timing and the
-// ray loop has no lane-dependent branch ahead of its cross-lane steps.
+// numbered over the concatenated partitions and dealt to waves in a fixed
+// order (wave w takes rays w, w + waves, ...), so that the ray loop has no
+// lane-dependent branch ahead of its cross-lane steps.  Lane l holds
+// partition l's count and its inclusive prefix sum; ray q lies in the
+// partition p = #{l : incl_l <= q}, at offset q - (incl_p - count_p).
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
     float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
     const RayState* __restrict__ queue) {
   __shared__ WaveWalkLds lds[4];
-  const uint32_t n = __builtin_amdgcn_readfirstlane(st->ray_queue[io.qslot]);
+  static_assert(KF_RAY_PARTS == 64, "one queue partition per lane");
+  const int ln = threadIdx.x & 63;
+  const uint32_t pc = io.qcount[ln * KF_RAY_COUNT_STRIDE];
+  uint32_t incl = pc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o);
+    incl += ln >= o ? t : 0u;
+  }
+  const uint32_t excl = incl - pc;
+  const uint32_t n = __builtin_amdgcn_readlane(incl, 63);
   // the next raycast appends to the other slot, which nothing reads now
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->ray_queue[io.qslot ^ 1] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < 64) io.qnext[ln * KF_RAY_COUNT_STRIDE] = 0u;
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
   float T[12];
@@ -2080,7 +2101,9 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   double xi[6] = {0, 0, 0, 0, 0, 0};
   float gn[MODE == kModeRefine ? kGnSums : 1] = {};
   for (uint32_t q = blockIdx.x * 4 + wave; q < n; q += n_waves) {
-    RayState r = queue[q];
+    const int part = uni((int)__popcll(__ballot(incl <= q)));
+    const uint32_t off = q - (uint32_t)__builtin_amdgcn_readlane((int)excl, part);
+    RayState r = queue[(size_t)part * io.qcap + off];
     uniform_ray(r);
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
@@ -2418,7 +2441,9 @@ static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
   RayIO io = make_ray_io(c, cam);
-  io.qslot = c->ray_slot;
+  io.qcount = c->ray_counts + (size_t)c->ray_slot * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
+  io.qnext = c->ray_counts + (size_t)(c->ray_slot ^ 1) * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
+  io.qcap = kf_ray_part_capacity(nb);
   c->ray_slot ^= 1;
   if (c->vol.precision == KF_TSDF_F32)
     launch_raycast_p<KF_TSDF_F32>(c, io, tiles_x, nb, c->count_rays, mode);
@@ -2463,6 +2488,10 @@ hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam) {
 }
 
 size_t kf_ray_state_bytes() { return sizeof(RayState); }
+
+size_t kf_ray_queue_entries(int H, int W) {
+  return (size_t)KF_RAY_PARTS * kf_ray_part_capacity(((W + 15) / 16) * ((H + 15) / 16));
+}
 
 hipError_t kf_launch_volume_fill(kf_context* c) {
   const int64_t n = c->n_store;
