@@ -834,8 +834,11 @@ __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ d
 // * Division pre-check.  When pz > 0 and px < 0.25 pz (or > (W+1) pz, and
 //   the same for y, mirrored for pz < 0), the rounded quotient cannot land in
 //   [1, W]; the voxel is rejected before the two correctly rounded divisions.
-// The voxel is read and written only when its signed distance falls in
-// [sdf_min, sdf_max], as in the reference.
+// * A kept brick costs one memory round trip: each lane loads its column's 8
+//   TSDF and weight values up front (vector loads; bricks are padded), then
+//   computes the 8 projections and issues the 8 table gathers together.
+// A voxel is written only when its signed distance falls in [sdf_min,
+// sdf_max], as in the reference.
 constexpr int kBricksPerBlock = 32;
 
 template <int PREC, bool FIRST>
@@ -994,7 +997,37 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
     const float Pt = tdx * tdx + tdy * tdy;
     const int base = (brick << 9) | (lane << 3);
     const int zn = min(8, vg.nz - bz * 8);
-    for (int lz = 0; lz < zn; ++lz) {
+    // The column's 8 stored values (bricks are padded, so all 8 exist), loaded
+    // before anything else so that the brick costs one memory round trip:
+    // projections first, then the 8 table gathers together.
+    float F0[8], W0[8];  // F32 weight
+    uint32_t Wc[8];      // u16 observation count (F16 volume)
+    if constexpr (!FIRST) {
+      if constexpr (PREC == KF_TSDF_F32) {
+        const float4* T4 = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(tsdf_) + base);
+        const float4* W4 = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(weight_) + base);
+        const float4 a = T4[0], b = T4[1], c = W4[0], e = W4[1];
+        F0[0] = a.x, F0[1] = a.y, F0[2] = a.z, F0[3] = a.w;
+        F0[4] = b.x, F0[5] = b.y, F0[6] = b.z, F0[7] = b.w;
+        W0[0] = c.x, W0[1] = c.y, W0[2] = c.z, W0[3] = c.w;
+        W0[4] = e.x, W0[5] = e.y, W0[6] = e.z, W0[7] = e.w;
+      } else {
+        const uint4 a = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(tsdf_) + base);
+        const uint4 c = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(weight_) + base);
+        const uint32_t ta[4] = {a.x, a.y, a.z, a.w}, wa[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const uint32_t h = (ta[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+          F0[k] = __half2float(__ushort_as_half((uint16_t)h));
+          Wc[k] = (wa[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+        }
+      }
+    }
+    int tix[8];
+#pragma unroll
+    for (int lz = 0; lz < 8; ++lz) {
+      tix[lz] = -1;
+      if (lz >= zn) continue;
       const float cz = (float)(bz * 8 + lz + vg.sz) * d + hd;
       const float lx = (Px + Ti[2] * cz) + Ti[3];
       const float ly = (Py + Ti[6] * cz) + Ti[7];
@@ -1010,13 +1043,21 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
       const int u = kf_round_to_int(px / pz);
       const int pv = kf_round_to_int(py / pz);
       if (!(u > 0 && u <= W && pv > 0 && pv <= H)) continue;
-      const float2 tb = tab[(pv - 1) * W + (u - 1)];
-      if (tb.x != tb.x) continue;  // depth 0 at that pixel
+      tix[lz] = (pv - 1) * W + (u - 1);
+    }
+    float2 tb[8];
+#pragma unroll
+    for (int lz = 0; lz < 8; ++lz) tb[lz] = tab[max(tix[lz], 0)];
+#pragma unroll
+    for (int lz = 0; lz < 8; ++lz) {
+      if (tix[lz] < 0) continue;
+      if (tb[lz].x != tb[lz].x) continue;  // depth 0 at that pixel
+      const float cz = (float)(bz * 8 + lz + vg.sz) * d + hd;
       const float tdz = cz - t[2];
       const float mag = Pm + cd[2] * tdz;
       if (!(mag > 0)) continue;
       const float tm = sqrtf(Pt + tdz * tdz);
-      const float frk = tb.x - (tb.y * tm);
+      const float frk = tb[lz].x - (tb[lz].y * tm);
       const bool pos = frk > 0 && frk <= smax;
       const bool neg = frk <= 0 && frk >= smin;
       if (!pos && !neg) continue;
@@ -1029,11 +1070,11 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
           T[v] = f;
           Wt[v] = 1.0f;
         } else {
-          float F0 = T[v], W0 = Wt[v];
-          if (F0 != F0) F0 = 0.0f;
-          if (W0 != W0) W0 = 0.0f;
-          T[v] = (W0 * F0 + 1.0f * f) / (W0 + 1.0f);
-          Wt[v] = W0 + 1.0f;
+          float Fo = F0[lz], Wo = W0[lz];
+          if (Fo != Fo) Fo = 0.0f;
+          if (Wo != Wo) Wo = 0.0f;
+          T[v] = (Wo * Fo + 1.0f * f) / (Wo + 1.0f);
+          Wt[v] = Wo + 1.0f;
         }
       } else {
         uint16_t* T = reinterpret_cast<uint16_t*>(tsdf_);
@@ -1042,11 +1083,11 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
           T[v] = __half_as_ushort(__float2half_rn(f));
           Wt[v] = 1;
         } else {
-          float F0 = __half2float(__ushort_as_half(T[v]));
-          if (F0 != F0) F0 = 0.0f;
-          const uint32_t w = Wt[v];
-          const float W0 = (float)w;
-          const float Fn = (W0 * F0 + 1.0f * f) / (W0 + 1.0f);
+          float Fo = F0[lz];
+          if (Fo != Fo) Fo = 0.0f;
+          const uint32_t w = Wc[lz];
+          const float Wo = (float)w;
+          const float Fn = (Wo * Fo + 1.0f * f) / (Wo + 1.0f);
           T[v] = __half_as_ushort(__float2half_rn(Fn));
           Wt[v] = (uint16_t)(w == 65535u ? 65535u : w + 1u);
         }
