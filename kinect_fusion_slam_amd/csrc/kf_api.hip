@@ -366,7 +366,7 @@ int kf_destroy(kf_context* c) {
   void* bufs[] = {c->ray_queue, c->ray_counts, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->integ_tiles,
-                  c->spatial, c->volume,    c->state,     c->icp_partials, c->icp_masks,
+                  c->spatial, c->range_lut, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
                   c->adj_partials, c->gn_partials, c->adj_touched};
   for (void* b : bufs)
@@ -384,6 +384,25 @@ static int apply_params(kf_context* c, const kf_params* prm) {
   host_spatial_kernel(prm->bilateral_radius, prm->sigma_s_px, w);
   const int L = 2 * prm->bilateral_radius + 1;
   KF_HIP_CHECK(hipMemcpy(c->spatial, w, sizeof(float) * L * L, hipMemcpyHostToDevice));
+  {
+    // range weights of integer depth differences, with the filter's float
+    // operations (the argument -(t*t)/den, then kf_expf); kf_expf returns
+    // exactly 0 below its underflow bound, and the argument only decreases
+    // with j, so the table ends where the bound is crossed
+    std::vector<float> lut(KF_RANGE_LUT_MAX);
+    const float sr = prm->sigma_r_mm;
+    const float den = 2 * sr * sr;
+    int n = 0;
+    for (; n < KF_RANGE_LUT_MAX; ++n) {
+      const float t = (float)n;
+      const float x = -(t * t) / den;
+      if (x < -103.972084045410156f) break;
+      lut[n] = kf_expf(x);
+    }
+    c->range_lut_n = n < KF_RANGE_LUT_MAX ? n : 0;
+    if (c->range_lut_n > 0)
+      KF_HIP_CHECK(hipMemcpy(c->range_lut, lut.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  }
   c->icp_cos_lo = icp_cos_threshold(prm->icp_angle_deg);
   c->flip_cos_hi = flip_cos_threshold();
   if (prm->record_icp && !c->icp_masks) {
@@ -484,6 +503,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {&c->ray_queue, kf_ray_queue_entries(height, width) * kf_ray_state_bytes()},
       {(void**)&c->ray_counts, (size_t)2 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4},
       {(void**)&c->spatial, 4 * (2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)},
+      {(void**)&c->range_lut, 4 * KF_RANGE_LUT_MAX},
       {(void**)&c->state, sizeof(kf_dev_state)},
       {(void**)&c->icp_partials, (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 2 * 4},
       {(void**)&c->gn_partials, kf_refine_partials(c) * 4},

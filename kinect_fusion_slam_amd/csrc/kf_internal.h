@@ -61,6 +61,12 @@ struct kf_vol {
 // serialises the appends of every wave), two counter slots used by alternate
 // raycasts (each clears the other's).  First-pass block b appends to
 // partition b % KF_RAY_PARTS, whose region holds the rays of all its blocks.
+// Bilateral range weights of integer depth differences j (u16 frames):
+// entry j = expf(-(j * j) / (2 sigma_r^2)) with the filter's own float
+// operations; every j >= the table length gives exactly 0.  Used when the
+// zero tail starts within this many entries.
+#define KF_RANGE_LUT_MAX 2048
+
 #define KF_RAY_PARTS 64
 #define KF_RAY_COUNT_STRIDE 64  // uint32 per counter line
 __host__ __device__ inline int kf_ray_part_capacity(int n_blocks) {
@@ -106,6 +112,8 @@ struct kf_context {
   float2* integ_tiles = nullptr;// per 16x16 tile: {min, max} camera distance a voxel
                                 // projecting there can be updated at (NaN-free)
   float* spatial = nullptr;     // bilateral spatial weights
+  float* range_lut = nullptr;   // bilateral range weights by integer |depth difference|
+  int range_lut_n = 0;          // entries (0: not usable, the filter evaluates expf)
 
   // volume (one allocation: TSDF array, then weight array, brick-tiled)
   void* volume = nullptr;

@@ -267,14 +267,20 @@ __global__ __launch_bounds__(256) void k_depth_u16(const uint16_t* __restrict__ 
 // as 0 (the reference's zero-initialised output replaces the depth).
 // U16: the input is the raw uint16 frame and the range gate of k_depth_u16
 // is applied as the tile is loaded (the gated frame itself is not stored).
+// Its depths are whole millimetres, so a tap's range weight
+// expf(-(t * t) / den) depends only on the integer |t|: with lut_n > 0 it is
+// read from the table of those exact float results (staged in LDS), entries
+// past lut_n being exactly 0.
 template <int R, bool U16>
 __global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
                                                    float* __restrict__ out,
                                                    const float* __restrict__ ws, int H, int W,
-                                                   float sigma_r, float dmin, float dmax) {
+                                                   float sigma_r, float dmin, float dmax,
+                                                   const float* __restrict__ lut, int lut_n) {
   constexpr int TS = 16, L = 2 * R + 1, SW = TS + 2 * R;
   __shared__ float tile[SW][SW + 1];
   __shared__ float sw[L * L];
+  __shared__ float s_lut[U16 ? KF_RANGE_LUT_MAX : 1];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
   const int n = H * W;
@@ -293,6 +299,8 @@ __global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
     tile[r][cc] = v;
   }
   for (int i = tid; i < L * L; i += 256) sw[i] = ws[i];
+  if constexpr (U16)
+    for (int i = tid; i < lut_n; i += 256) s_lut[i] = lut[i];
   __syncthreads();
   const int tx = tid & (TS - 1), ty = tid / TS;
   const int x = x0 + tx, y = y0 + ty;
@@ -302,7 +310,21 @@ __global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
     const float dc = tile[ty + R][tx + R];
     const float den = 2 * sigma_r * sigma_r;
     float sum = 0.f, fin = 0.f;
-    if constexpr (L * L <= 49) {
+    if (U16 && lut_n > 0) {
+      float w[L * L];
+#pragma unroll
+      for (int k = 0; k < L * L; ++k) {
+        const int j = (int)fabsf(tile[ty + k / L][tx + k % L] - dc);
+        const float t = j < lut_n ? s_lut[j] : 0.0f;
+        w[k] = sw[k] * t + 0.000000001f;
+        sum = sum + w[k];
+      }
+#pragma unroll
+      for (int k = 0; k < L * L; ++k) {
+        const float wr = w[k] / sum;
+        fin = fin + wr * tile[ty + k / L][tx + k % L];
+      }
+    } else if constexpr (L * L <= 49) {
       // small windows: keep the weights in registers (fully unrolled)
       float w[L * L];
 #pragma unroll
@@ -2316,10 +2338,11 @@ static hipError_t launch_bilateral(kf_context* c, const uint16_t* src) {
   case R:                                                                                      \
     if (src)                                                                                   \
       k_bilateral<R, true><<<grid, 256, 0, c->stream>>>(in, c->depth_tmp, c->spatial, c->H,    \
-                                                        c->W, sr, lo, hi);                     \
+                                                        c->W, sr, lo, hi, c->range_lut,        \
+                                                        c->range_lut_n);                       \
     else                                                                                       \
       k_bilateral<R, false><<<grid, 256, 0, c->stream>>>(in, c->depth_tmp, c->spatial, c->H,   \
-                                                         c->W, sr, lo, hi);                    \
+                                                         c->W, sr, lo, hi, nullptr, 0);        \
     break;
     KF_BIL(1)
     KF_BIL(2)
