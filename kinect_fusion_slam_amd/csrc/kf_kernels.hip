@@ -1777,8 +1777,9 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
 // Visits, decisions and the crossing state are those of the serial walk.
 constexpr int kWaveSteps = 64;
 
-struct WaveWalkLds {
-  float seq[4][kWaveSteps + 1];  // t_a(j); row 3 takes the writes of lanes >= 3
+struct alignas(16) WaveWalkLds {
+  float seq[4][kWaveSteps + 4];  // t_a(j), j <= kWaveSteps (rows padded to 16 bytes);
+                                 // row 3 takes the writes of lanes >= 3
   int2 rec[3][kWaveSteps];       // other two indices at the j-th step along a
   uint32_t cnt[kWaveSteps];      // step counts after step r: cx | cy << 8 | cz << 16
   float key[kWaveSteps];         // boundary distance of step r
@@ -1788,9 +1789,17 @@ struct WaveWalkLds {
 // s with s[j] close to s0 + j td: estimated from that, then corrected against
 // the stored values.
 template <bool LE>
-__device__ __forceinline__ int seq_count(const float* s, float s0, float inv_td, float v) {
+__device__ __forceinline__ int seq_guess(float s0, float inv_td, float v) {
   const float q = floorf((v - s0) * inv_td) + 1.0f;
-  int c = q <= 0.f ? 0 : (q >= (float)kWaveSteps ? kWaveSteps : (int)q);
+  return q <= 0.f ? 0 : (q >= (float)kWaveSteps ? kWaveSteps : (int)q);
+}
+// Whether guess c needs no correction, given lo = s[c - 1] and hi = s[c].
+template <bool LE>
+__device__ __forceinline__ bool seq_exact(int c, float lo, float hi, float v) {
+  return (c == 0 || !(LE ? lo > v : lo >= v)) && (c == kWaveSteps || !(LE ? hi <= v : hi < v));
+}
+template <bool LE>
+__device__ __forceinline__ int seq_fix(const float* s, int c, float v) {
   while (c > 0 && (LE ? s[c - 1] > v : s[c - 1] >= v)) --c;
   while (c < kWaveSteps && (LE ? s[c] <= v : s[c] < v)) ++c;
   return c;
@@ -1842,21 +1851,53 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     {
       float t = lane == 0 ? c0.tmx : (lane == 1 ? c0.tmy : c0.tmz);
       const float td = lane == 0 ? p.tdx : (lane == 1 ? p.tdy : p.tdz);
-      float* sq = L.seq[min(lane, 3)];
-      for (int j = 0; j <= kWaveSteps; ++j) {
-        sq[j] = t;
+      float4* sq = reinterpret_cast<float4*>(L.seq[min(lane, 3)]);
+#pragma unroll
+      for (int j = 0; j < kWaveSteps / 4; ++j) {
+        float4 q;
+        q.x = t;
         t = t + td;
+        q.y = t;
+        t = t + td;
+        q.z = t;
+        t = t + td;
+        q.w = t;
+        t = t + td;
+        sq[j] = q;
       }
+      L.seq[min(lane, 3)][kWaveSteps] = t;
     }
     wave_lds_sync();
-    // 2. ranks in the merge; ties: y and z before x, z before y
+    // 2. ranks in the merge; ties: y and z before x, z before y.  Each rank
+    //    is #{j : s[j] <= v} (or < v) over one sequence: a guess from s0 + j
+    //    td, checked against the two stored values around it (all 12 loaded
+    //    together), and corrected by a walk where the check fails.
     const float ex = L.seq[0][lane], ey = L.seq[1][lane], ez = L.seq[2][lane];
-    const int y_x = seq_count<true>(L.seq[1], c0.tmy, ity, ex);
-    const int z_x = seq_count<true>(L.seq[2], c0.tmz, itz, ex);
-    const int x_y = seq_count<false>(L.seq[0], c0.tmx, itx, ey);
-    const int z_y = seq_count<true>(L.seq[2], c0.tmz, itz, ey);
-    const int x_z = seq_count<false>(L.seq[0], c0.tmx, itx, ez);
-    const int y_z = seq_count<false>(L.seq[1], c0.tmy, ity, ez);
+    int y_x = seq_guess<true>(c0.tmy, ity, ex);
+    int z_x = seq_guess<true>(c0.tmz, itz, ex);
+    int x_y = seq_guess<false>(c0.tmx, itx, ey);
+    int z_y = seq_guess<true>(c0.tmz, itz, ey);
+    int x_z = seq_guess<false>(c0.tmx, itx, ez);
+    int y_z = seq_guess<false>(c0.tmy, ity, ez);
+    {
+      const float* s0 = L.seq[0];
+      const float* s1 = L.seq[1];
+      const float* s2 = L.seq[2];
+      const bool e0 = seq_exact<true>(y_x, s1[max(y_x - 1, 0)], s1[y_x], ex);
+      const bool e1 = seq_exact<true>(z_x, s2[max(z_x - 1, 0)], s2[z_x], ex);
+      const bool e2 = seq_exact<false>(x_y, s0[max(x_y - 1, 0)], s0[x_y], ey);
+      const bool e3 = seq_exact<true>(z_y, s2[max(z_y - 1, 0)], s2[z_y], ey);
+      const bool e4 = seq_exact<false>(x_z, s0[max(x_z - 1, 0)], s0[x_z], ez);
+      const bool e5 = seq_exact<false>(y_z, s1[max(y_z - 1, 0)], s1[y_z], ez);
+      if (!(e0 & e1 & e2 & e3 & e4 & e5)) {
+        y_x = seq_fix<true>(s1, y_x, ex);
+        z_x = seq_fix<true>(s2, z_x, ex);
+        x_y = seq_fix<false>(s0, x_y, ey);
+        z_y = seq_fix<true>(s2, z_y, ey);
+        x_z = seq_fix<false>(s0, x_z, ez);
+        y_z = seq_fix<false>(s1, y_z, ez);
+      }
+    }
     L.rec[0][lane] = make_int2(c0.iy + p.sy * y_x, c0.iz + p.sz * z_x);
     L.rec[1][lane] = make_int2(c0.ix + p.sx * x_y, c0.iz + p.sz * z_y);
     L.rec[2][lane] = make_int2(c0.ix + p.sx * x_z, c0.iy + p.sy * y_z);
