@@ -363,7 +363,7 @@ int kf_destroy(kf_context* c) {
   if (!c) return KF_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   timing_drop(c);
-  void* bufs[] = {c->ray_queue, c->ray_counts, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
+  void* bufs[] = {c->ray_queue, c->ray_counts, c->integ_groups, c->integ_counts, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
                   c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
                   c->integ_tiles,
                   c->spatial, c->range_lut, c->volume,    c->state,     c->icp_partials, c->icp_masks,
@@ -502,6 +502,8 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->integ_tiles, (size_t)((width + 15) / 16) * ((height + 15) / 16) * 8},
       {&c->ray_queue, kf_ray_queue_entries(height, width) * kf_ray_state_bytes()},
       {(void**)&c->ray_counts, (size_t)2 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4},
+      {(void**)&c->integ_groups, kf_integ_group_entries(c->vg) * 4},
+      {(void**)&c->integ_counts, (size_t)4 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4},
       {(void**)&c->spatial, 4 * (2 * KF_MAX_BILATERAL_RADIUS + 1) * (2 * KF_MAX_BILATERAL_RADIUS + 1)},
       {(void**)&c->range_lut, 4 * KF_RANGE_LUT_MAX},
       {(void**)&c->state, sizeof(kf_dev_state)},
@@ -530,6 +532,8 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   if (hipMemsetAsync(c->hit, 0xff, n * 4, c->stream) != hipSuccess ||
       hipMemsetAsync(c->state, 0, sizeof(kf_dev_state), c->stream) != hipSuccess ||
       hipMemsetAsync(c->ray_counts, 0, (size_t)2 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4,
+                     c->stream) != hipSuccess ||
+      hipMemsetAsync(c->integ_counts, 0, (size_t)4 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4,
                      c->stream) != hipSuccess)
     return fail(KF_ERR_HIP, "hipMemsetAsync failed");
   *out = c;
@@ -583,7 +587,8 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
   size_t b = c->volume_bytes + n * (4 + 4 + 2 + 16 + 16 + 1 + 16 + 16 + 4 + 8) + sizeof(kf_dev_state) +
              (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 8 +
              kf_ray_queue_entries(c->H, c->W) * kf_ray_state_bytes() +
-             (size_t)2 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4;
+             kf_integ_group_entries(c->vg) * 4 +
+             (size_t)6 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4;
   if (c->icp_masks) b += (size_t)KF_MAX_ICP_ITERS * n;
   if (c->frames) b += (size_t)c->n_frames * n * 2 + (size_t)c->n_frames * 64;
   *bytes = b;

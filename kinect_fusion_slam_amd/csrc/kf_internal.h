@@ -111,6 +111,9 @@ struct kf_context {
   float2* integ_tab = nullptr;  // per-pixel {predicted depth, 1/|R lambda|}
   float2* integ_tiles = nullptr;// per 16x16 tile: {min, max} camera distance a voxel
                                 // projecting there can be updated at (NaN-free)
+  int* integ_groups = nullptr;  // groups, then bricks, that pass the culls (partitioned lists)
+  uint32_t* integ_counts = nullptr; // [group, brick][2 slots][KF_RAY_PARTS * KF_RAY_COUNT_STRIDE]
+  int integ_slot = 0;           // integ_counts slot of the next integration
   float* spatial = nullptr;     // bilateral spatial weights
   float* range_lut = nullptr;   // bilateral range weights by integer |depth difference|
   int range_lut_n = 0;          // entries (0: not usable, the filter evaluates expf)
@@ -187,6 +190,7 @@ hipError_t kf_launch_record_pose(kf_context* c, int frame);
 
 size_t kf_ray_state_bytes();
 size_t kf_ray_queue_entries(int H, int W);
+size_t kf_integ_group_entries(const kf_vol& vg);
 
 // shared host helpers
 void kf_make_cam(const float* K, kf_cam* cam);

@@ -742,6 +742,145 @@ __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* 
 // ---------------------------------------------------------------------------
 // A8  projective TSDF integration (projective_tsdf.cu:5-140).
 
+// Front-test direction of integration (projective_tsdf.cu:57-71): the ray
+// through the pixel at the swapped principal point (K[5], K[2]), through T.
+__device__ __forceinline__ void integ_center_dir(const float* K, const float* Ki,
+                                                 const float* T, float* cd) {
+  const float cax = Ki[0] * K[5] + Ki[1] * K[2] + Ki[2] * 1.f;
+  const float cay = Ki[3] * K[5] + Ki[4] * K[2] + Ki[5] * 1.f;
+  const float caz = Ki[6] * K[5] + Ki[7] * K[2] + Ki[8] * 1.f;
+  const float gax = T[0] * cax + T[1] * cay + T[2] * caz + T[3];
+  const float gay = T[4] * cax + T[5] * cay + T[6] * caz + T[7];
+  const float gaz = T[8] * cax + T[9] * cay + T[10] * caz + T[11];
+  const float cbx = 2 * (Ki[0] * K[5] + Ki[1] * K[2] + Ki[2] * 1.f);
+  const float cby = 2 * (Ki[3] * K[5] + Ki[4] * K[2] + Ki[5] * 1.f);
+  const float cbz = 2 * (Ki[6] * K[5] + Ki[7] * K[2] + Ki[8] * 1.f);
+  const float gbx = T[0] * cbx + T[1] * cby + T[2] * cbz + T[3];
+  const float gby = T[4] * cbx + T[5] * cby + T[6] * cbz + T[7];
+  const float gbz = T[8] * cbx + T[9] * cby + T[10] * cbz + T[11];
+  cd[0] = gbx - gax;
+  cd[1] = gby - gay;
+  cd[2] = gbz - gaz;
+}
+
+// Integration's box culls (see k_integrate): the FP64 affine quantities the
+// per-voxel tests depend on, at the voxel centre (x, y, z) of a box corner,
+// and which of the five rejections hold there.  Each rejection set is convex
+// (a linear form against a convex margin), so one that holds at all 8
+// corners of a box holds on the whole box.
+//
+// any_side: the four image-side tests hold without "pz > M".  A voxel with
+// camera coordinates l and p = K l lies on the ray l = pz K^-1 (u, v, 1)
+// (u = px / pz, v = py / pz), so the front test's dot is
+// mag = lc . l = pz (lc . K^-1 (u, v, 1)), lc = K^-1 (K[5], K[2], 1).  When
+// lc . K^-1 (u, v, 1) > 0 over the image (checked at its corners on the
+// host), a voxel behind the camera plane that projects into the image has
+// mag < 0 and fails the front test, so the side tests alone decide; and a
+// voxel near the plane projects far outside the image (|px| or |py| is at
+// least M while pz is near 0).
+struct BoxCorner {
+  double cx, cy, cz;  // voxel centre
+  double px, py, pz;  // K T^-1 c
+  double tx, ty, tz;  // c - t
+  bool front, r1, r2, r3, r4, r5;
+};
+
+__device__ __forceinline__ BoxCorner integ_corner(int x, int y, int z, const kf_vol& vg,
+                                                  const float* Ti, const float* t,
+                                                  const float* cd, const float* K, int H,
+                                                  int W, bool any_side) {
+  BoxCorner o;
+  const float d = vg.d, hd = d / 2.f;
+  o.cx = (double)((float)(x + vg.sx) * d + hd);
+  o.cy = (double)((float)(y + vg.sy) * d + hd);
+  o.cz = (double)((float)(z + vg.sz) * d + hd);
+  const double lx = (double)Ti[0] * o.cx + (double)Ti[1] * o.cy + (double)Ti[2] * o.cz + (double)Ti[3];
+  const double ly = (double)Ti[4] * o.cx + (double)Ti[5] * o.cy + (double)Ti[6] * o.cz + (double)Ti[7];
+  const double lz = (double)Ti[8] * o.cx + (double)Ti[9] * o.cy + (double)Ti[10] * o.cz + (double)Ti[11];
+  o.px = (double)K[0] * lx + (double)K[1] * ly + (double)K[2] * lz;
+  o.py = (double)K[3] * lx + (double)K[4] * ly + (double)K[5] * lz;
+  o.pz = (double)K[6] * lx + (double)K[7] * ly + (double)K[8] * lz;
+  double kabs = 0.0;
+  for (int i = 0; i < 9; ++i) kabs = fmax(kabs, fabs((double)K[i]));
+  const double M = 16.0 + 3e-5 * kabs * (fabs(lx) + fabs(ly) + fabs(lz) + 1.0);
+  o.tx = o.cx - (double)t[0], o.ty = o.cy - (double)t[1], o.tz = o.cz - (double)t[2];
+  const double mag = (double)cd[0] * o.tx + (double)cd[1] * o.ty + (double)cd[2] * o.tz;
+  const double Mm = 1.0 + 1e-5 * (fabs((double)cd[0]) + fabs((double)cd[1]) + fabs((double)cd[2])) *
+                              (fabs(o.tx) + fabs(o.ty) + fabs(o.tz));
+  o.front = o.pz > M;
+  const bool side = any_side || o.front;
+  o.r1 = mag < -Mm;
+  o.r2 = side && o.px < 0.45 * o.pz - M;
+  o.r3 = side && o.px > ((double)W + 0.55) * o.pz + M;
+  o.r4 = side && o.py < 0.45 * o.pz - M;
+  o.r5 = side && o.py > ((double)H + 0.55) * o.pz + M;
+  return o;
+}
+
+// The 8 boxes of a wave (box j = lanes 8j .. 8j+7, one corner each): bit j
+// is set when no rejection holds at all 8 corners of box j.
+__device__ __forceinline__ uint32_t wave_box_keep(bool r1, bool r2, bool r3, bool r4, bool r5,
+                                                  bool r6) {
+  const uint64_t m1 = __ballot(r1), m2 = __ballot(r2), m3 = __ballot(r3), m4 = __ballot(r4),
+                 m5 = __ballot(r5), m6 = __ballot(r6);
+  uint32_t keep = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int sh = 8 * j;
+    const bool rej = ((m1 >> sh) & 0xff) == 0xff || ((m2 >> sh) & 0xff) == 0xff ||
+                     ((m3 >> sh) & 0xff) == 0xff || ((m4 >> sh) & 0xff) == 0xff ||
+                     ((m5 >> sh) & 0xff) == 0xff || ((m6 >> sh) & 0xff) == 0xff;
+    keep |= rej ? 0u : 1u << j;
+  }
+  return keep;
+}
+
+// Integration culls in two levels.  Groups of 2x2x2 bricks are tested first
+// (by extra blocks of k_integ_table, with the frustum and front tests), and
+// the surviving groups are appended to a list; k_integrate then tests the
+// bricks of the listed groups only.  The list has KF_RAY_PARTS partitions,
+// each with its own counter line (two counter slots used by alternate
+// integrations, the integration clearing the other slot); the table block
+// of group-block index gb appends to partition gb % KF_RAY_PARTS.
+struct GroupList {
+  int* glist;         // surviving groups: KF_RAY_PARTS x gcap ids
+  uint32_t* gcount;   // their counters (this integration's slot)
+  uint32_t* gnext;    // the next integration's slot
+  int gcap;           // groups per partition
+  int* blist;         // surviving bricks: KF_RAY_PARTS x bcap ids
+  uint32_t* bcount;   // their counters (this integration's slot)
+  uint32_t* bnext;    // the next integration's slot
+  int bcap;           // bricks per partition
+  int gx, gy, gz;     // groups per axis
+  int any_side;       // side tests without the front requirement (see integ_corner)
+};
+
+// Lane l of the calling wave: partition l's count (KF_RAY_PARTS == 64) and
+// its exclusive prefix sum; returns the list length.  Entry e then lies in
+// partition p = #{l : incl_l <= e}, at offset e - excl_p.
+__device__ __forceinline__ uint32_t list_prefix(const uint32_t* count, uint32_t& incl,
+                                                uint32_t& excl) {
+  static_assert(KF_RAY_PARTS == 64, "one list partition per lane");
+  const int lane = threadIdx.x & 63;
+  const uint32_t pc = count[lane * KF_RAY_COUNT_STRIDE];
+  incl = pc;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o);
+    incl += lane >= o ? u : 0u;
+  }
+  excl = incl - pc;
+  return __builtin_amdgcn_readlane(incl, 63);
+}
+
+// Entry e (wave-uniform, below the list length) of a partitioned list.
+__device__ __forceinline__ int list_entry(const int* list, int cap, uint32_t incl,
+                                          uint32_t excl, uint32_t e) {
+  const int part = __builtin_amdgcn_readfirstlane(__popcll(__ballot(incl <= e)));
+  const uint32_t off = e - (uint32_t)__builtin_amdgcn_readlane((int)excl, part);
+  return __builtin_amdgcn_readfirstlane(list[part * cap + off]);
+}
+
 // Per-pixel part: for a pixel (u, v) with depth d != 0, the ray magnitude
 // |T(2 lambda) - T(lambda)|, the predicted depth |T(d lambda) - t| / |.| and
 // its reciprocal depend only on the pixel, so they are computed here once
@@ -760,30 +899,53 @@ __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ d
                                                      int H, int W, kf_dev_state* __restrict__ st,
                                                      float2* __restrict__ tab,
                                                      float2* __restrict__ tiles, float smin,
-                                                     float smax) {
+                                                     float smax, int tiles_y, kf_vol vg,
+                                                     GroupList gl) {
   float T[12];
   load_uniform<12>(st->T, T);
   const float* Ki = cam.Kinv;
   const float* K = cam.K;
+  if (blockIdx.y >= (unsigned)tiles_y) {
+    // group cull: lane = (group of the block, corner), 32 groups per block;
+    // the surviving groups are appended to the group list
+    const int gb = (blockIdx.y - tiles_y) * gridDim.x + blockIdx.x;
+    const int lane = threadIdx.x & 63, corner = lane & 7;
+    const int g0 = gb * 32 + (int)(threadIdx.x >> 6) * 8;  // this wave's first group
+    const int g = g0 + (lane >> 3);
+    bool r1 = true, r2 = true, r3 = true, r4 = true, r5 = true;
+    if (g < gl.gx * gl.gy * gl.gz) {
+      float Ti[12], t[3], cd[3];
+      load_uniform<12>(st->Tinv, Ti);
+      t[0] = T[3], t[1] = T[7], t[2] = T[11];
+      integ_center_dir(K, Ki, T, cd);
+      const int gz = g % gl.gz, q = g / gl.gz;
+      const int gy = q % gl.gy, gx = q / gl.gy;
+      const int x = min(gx * 16 + ((corner & 4) ? 15 : 0), vg.nx - 1);
+      const int y = min(gy * 16 + ((corner & 2) ? 15 : 0), vg.ny - 1);
+      const int z = min(gz * 16 + ((corner & 1) ? 15 : 0), vg.nz - 1);
+      const BoxCorner c = integ_corner(x, y, z, vg, Ti, t, cd, K, H, W, gl.any_side != 0);
+      r1 = c.r1, r2 = c.r2, r3 = c.r3, r4 = c.r4, r5 = c.r5;
+    }
+    const uint32_t keep = wave_box_keep(r1, r2, r3, r4, r5, false);
+    if (keep) {
+      const int part = gb & (KF_RAY_PARTS - 1);
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&gl.gcount[part * KF_RAY_COUNT_STRIDE], (uint32_t)__popc(keep));
+      base = __shfl(base, 0);
+      if (lane < 8 && ((keep >> lane) & 1u))
+        gl.glist[part * gl.gcap + base + __popc(keep & ((1u << lane) - 1u))] = g0 + lane;
+    }
+    return;
+  }
   const int px = blockIdx.x * 16 + (threadIdx.x & 15), py = blockIdx.y * 16 + (threadIdx.x >> 4);
   const bool inside = px < W && py < H;
   const int i = inside ? py * W + px : -1;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-    const float cax = Ki[0] * K[5] + Ki[1] * K[2] + Ki[2] * 1.f;
-    const float cay = Ki[3] * K[5] + Ki[4] * K[2] + Ki[5] * 1.f;
-    const float caz = Ki[6] * K[5] + Ki[7] * K[2] + Ki[8] * 1.f;
-    const float gax = T[0] * cax + T[1] * cay + T[2] * caz + T[3];
-    const float gay = T[4] * cax + T[5] * cay + T[6] * caz + T[7];
-    const float gaz = T[8] * cax + T[9] * cay + T[10] * caz + T[11];
-    const float cbx = 2 * (Ki[0] * K[5] + Ki[1] * K[2] + Ki[2] * 1.f);
-    const float cby = 2 * (Ki[3] * K[5] + Ki[4] * K[2] + Ki[5] * 1.f);
-    const float cbz = 2 * (Ki[6] * K[5] + Ki[7] * K[2] + Ki[8] * 1.f);
-    const float gbx = T[0] * cbx + T[1] * cby + T[2] * cbz + T[3];
-    const float gby = T[4] * cbx + T[5] * cby + T[6] * cbz + T[7];
-    const float gbz = T[8] * cbx + T[9] * cby + T[10] * cbz + T[11];
-    st->center_dir[0] = gbx - gax;
-    st->center_dir[1] = gby - gay;
-    st->center_dir[2] = gbz - gaz;
+    float cd[3];
+    integ_center_dir(K, Ki, T, cd);
+    st->center_dir[0] = cd[0];
+    st->center_dir[1] = cd[1];
+    st->center_dir[2] = cd[2];
     st->center_dir[3] = 0.f;
   }
   double lo = __builtin_inf(), hi = -__builtin_inf();
@@ -836,173 +998,47 @@ __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ d
   }
 }
 
-// Per-voxel part.  A 256-thread block owns 32 consecutive bricks (8x8x8
-// voxels each).
+// Per-voxel part, on the bricks (8x8x8 voxels) that survive three culls.
 //
-// * Brick cull.  Lane (brick, corner) evaluates, in FP64, the affine
-//   quantities the per-voxel test depends on at one of the brick's 8 corner
-//   voxel centres: the projection p = K T^-1 c and the front-test dot
-//   mag = dir . (c - t).  If one of "mag < -M", "pz > M and px < 0.45 pz - M",
-//   "px > (W + 0.55) pz + M" or the same two for py/H holds at all 8 corners,
-//   it holds on the whole brick, and the margin M (16 + 3e-5 |K| |l|, many
-//   times the rounding error of the float evaluation) guarantees that the
-//   reference's float test rejects every voxel of the brick.  Such bricks are
-//   skipped; no decision changes.  The surviving bricks of the block are
-//   shared by its 4 waves.
-// * Per brick, lane (lx, ly) walks its z column of 8 voxels.  The reference
-//   sums T^-1 c left to right, so Ti0*cx + Ti1*cy (and the x/y parts of the
-//   front test and of |c - t|^2) are the same for the column and are computed
-//   once, bit-identically.
+// * Box culls.  Lanes evaluate, in FP64, the affine quantities the
+//   per-voxel test depends on at the 8 corner voxel centres of a box: the
+//   projection p = K T^-1 c and the front-test dot mag = dir . (c - t).  If
+//   one of "mag < -M", "px < 0.45 pz - M", "px > (W + 0.55) pz + M" or the
+//   same two for py/H holds at all 8 corners, it holds on the whole box
+//   (integ_corner), and the margin M (16 + 3e-5 |K| |l|, many times the
+//   rounding error of the float evaluation) guarantees that the reference's
+//   float test rejects every voxel of the box.  k_integ_table applies these
+//   to groups of 2x2x2 bricks, then to the bricks of the surviving groups,
+//   and lists the surviving bricks; no decision changes.
+// * Band cull (here, per listed brick): the brick is skipped when its
+//   distance range to the camera misses the tile intervals of updatable
+//   distances under its footprint (k_integ_table).
+// * One listed brick per wave, so that no wave updates several bricks in a
+//   row.  Lane (lx, ly) walks its z column of 8 voxels.  The reference sums
+//   T^-1 c left to right, so Ti0*cx + Ti1*cy (and the x/y parts of the front
+//   test and of |c - t|^2) are the same for the column and are computed once,
+//   bit-identically.
 // * Division pre-check.  When pz > 0 and px < 0.25 pz (or > (W+1) pz, and
 //   the same for y, mirrored for pz < 0), the rounded quotient cannot land in
 //   [1, W]; the voxel is rejected before the two correctly rounded divisions.
-// * A kept brick costs one memory round trip: each lane loads its column's 8
+// * A brick costs one memory round trip: each lane loads its column's 8
 //   TSDF and weight values up front (vector loads; bricks are padded), then
 //   computes the 8 projections and issues the 8 table gathers together.
 // A voxel is written only when its signed distance falls in [sdf_min,
 // sdf_max], as in the reference.
-constexpr int kBricksPerBlock = 32;
 
+// Updates one brick with the calling wave: lane (lx, ly) walks its z column.
 template <int PREC, bool FIRST>
-__global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
-                                                   void* __restrict__ weight_,
-                                                   const float2* __restrict__ tab, kf_cam cam,
-                                                   kf_vol vg, int H, int W, float trunc,
-                                                   float smin, float smax,
-                                                   const kf_dev_state* __restrict__ st,
-                                                   int nbricks, const float2* __restrict__ tiles,
-                                                   int tiles_x, int tiles_y) {
-  __shared__ int keep_list[kBricksPerBlock];
-  __shared__ uint32_t keep_mask[4];
-  __shared__ int n_keep;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float Ti[12], t[3], cd[3];
-  load_uniform<12>(st->Tinv, Ti);
-  t[0] = st->T[3];
-  t[1] = st->T[7];
-  t[2] = st->T[11];
-  load_uniform<3>(st->center_dir, cd);
-  const float* K = cam.K;
+__device__ __forceinline__ void integrate_brick(void* __restrict__ tsdf_, void* __restrict__ weight_,
+                                                const float2* __restrict__ tab, const float* K,
+                                                const kf_vol& vg, int H, int W, float trunc,
+                                                float smin, float smax, const float* Ti,
+                                                const float* t, const float* cd, int brick) {
+  const int lane = threadIdx.x & 63;
   const float d = vg.d;
   const float hd = d / 2.f;
-
-  {  // brick cull: thread = (brick of the block, corner)
-    const int brick = blockIdx.x * kBricksPerBlock + (int)(threadIdx.x >> 3);
-    const int corner = threadIdx.x & 7;
-    bool r1 = true, r2 = true, r3 = true, r4 = true, r5 = true, r6 = false;
-    if (brick < nbricks) {
-      const int bz = brick % vg.bz;
-      const int q = brick / vg.bz;
-      const int by = q % vg.by;
-      const int bx = q / vg.by;
-      const int x = min(bx * 8 + ((corner & 4) ? 7 : 0), vg.nx - 1);
-      const int y = min(by * 8 + ((corner & 2) ? 7 : 0), vg.ny - 1);
-      const int z = min(bz * 8 + ((corner & 1) ? 7 : 0), vg.nz - 1);
-      const double cx = (double)((float)(x + vg.sx) * d + hd);
-      const double cy = (double)((float)(y + vg.sy) * d + hd);
-      const double cz = (double)((float)(z + vg.sz) * d + hd);
-      const double lx = (double)Ti[0] * cx + (double)Ti[1] * cy + (double)Ti[2] * cz + (double)Ti[3];
-      const double ly = (double)Ti[4] * cx + (double)Ti[5] * cy + (double)Ti[6] * cz + (double)Ti[7];
-      const double lz = (double)Ti[8] * cx + (double)Ti[9] * cy + (double)Ti[10] * cz + (double)Ti[11];
-      const double px = (double)K[0] * lx + (double)K[1] * ly + (double)K[2] * lz;
-      const double py = (double)K[3] * lx + (double)K[4] * ly + (double)K[5] * lz;
-      const double pz = (double)K[6] * lx + (double)K[7] * ly + (double)K[8] * lz;
-      double kabs = 0.0;
-      for (int i = 0; i < 9; ++i) kabs = fmax(kabs, fabs((double)K[i]));
-      const double M = 16.0 + 3e-5 * kabs * (fabs(lx) + fabs(ly) + fabs(lz) + 1.0);
-      const double tx = cx - (double)t[0], ty = cy - (double)t[1], tz = cz - (double)t[2];
-      const double mag = (double)cd[0] * tx + (double)cd[1] * ty + (double)cd[2] * tz;
-      const double Mm =
-          1.0 + 1e-5 * (fabs((double)cd[0]) + fabs((double)cd[1]) + fabs((double)cd[2])) *
-                    (fabs(tx) + fabs(ty) + fabs(tz));
-      const bool front = pz > M;
-      r1 = mag < -Mm;
-      r2 = front && px < 0.45 * pz - M;
-      r3 = front && px > ((double)W + 0.55) * pz + M;
-      r4 = front && py < 0.45 * pz - M;
-      r5 = front && py > ((double)H + 0.55) * pz + M;
-      // band cull: every voxel of the brick is nearer than the updatable
-      // distances of every pixel it can project to, or farther
-      double u = front ? px / pz : 0.0, v = front ? py / pz : 0.0;
-      double umin = u, umax = u, vmin = v, vmax = v;
-      double dmax = sqrt(tx * tx + ty * ty + tz * tz);
-      double xmin = cx, xmax = cx, ymin = cy, ymax = cy, zmin = cz, zmax = cz;
-      int all_front = front ? 1 : 0;
-#pragma unroll
-      for (int o = 1; o < 8; o <<= 1) {
-        umin = fmin(umin, __shfl_xor(umin, o, 8));
-        umax = fmax(umax, __shfl_xor(umax, o, 8));
-        vmin = fmin(vmin, __shfl_xor(vmin, o, 8));
-        vmax = fmax(vmax, __shfl_xor(vmax, o, 8));
-        dmax = fmax(dmax, __shfl_xor(dmax, o, 8));
-        xmin = fmin(xmin, __shfl_xor(xmin, o, 8));
-        xmax = fmax(xmax, __shfl_xor(xmax, o, 8));
-        ymin = fmin(ymin, __shfl_xor(ymin, o, 8));
-        ymax = fmax(ymax, __shfl_xor(ymax, o, 8));
-        zmin = fmin(zmin, __shfl_xor(zmin, o, 8));
-        zmax = fmax(zmax, __shfl_xor(zmax, o, 8));
-        all_front = all_front & __shfl_xor(all_front, o, 8);
-      }
-      if (all_front) {
-        // footprint: the voxels project inside the hull of the corner
-        // projections; one pixel more on each side covers roundf and the
-        // float evaluation
-        const int u0 = (int)fmax(floor(umin) - 1.0, 1.0), u1 = (int)fmin(ceil(umax) + 1.0, (double)W);
-        const int v0 = (int)fmax(floor(vmin) - 1.0, 1.0), v1 = (int)fmin(ceil(vmax) + 1.0, (double)H);
-        if (u0 <= u1 && v0 <= v1) {
-          const int tx0 = (u0 - 1) >> 4, tx1 = (u1 - 1) >> 4;
-          const int ty0 = (v0 - 1) >> 4, ty1 = (v1 - 1) >> 4;
-          const int ntx = tx1 - tx0 + 1, nt = ntx * (ty1 - ty0 + 1);
-          if (nt <= 64) {
-            float lo = __builtin_inff(), hi = -__builtin_inff();
-            for (int k = corner; k < nt; k += 8) {
-              const float2 b = tiles[(ty0 + k / ntx) * tiles_x + tx0 + k % ntx];
-              lo = fminf(lo, b.x);
-              hi = fmaxf(hi, b.y);
-            }
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1) {
-              lo = fminf(lo, __shfl_xor(lo, o, 8));
-              hi = fmaxf(hi, __shfl_xor(hi, o, 8));
-            }
-            // nearest voxel centre of the brick
-            const double ex = fmax(fmax(xmin - (double)t[0], (double)t[0] - xmax), 0.0);
-            const double ey = fmax(fmax(ymin - (double)t[1], (double)t[1] - ymax), 0.0);
-            const double ez = fmax(fmax(zmin - (double)t[2], (double)t[2] - zmax), 0.0);
-            const double dmin = sqrt(ex * ex + ey * ey + ez * ez);
-            const double margin = 1.0 + 1e-5 * dmax;
-            r6 = dmax + margin < (double)lo || dmin - margin > (double)hi;
-          }
-        }
-      }
-    }
-    const uint64_t m1 = __ballot(r1), m2 = __ballot(r2), m3 = __ballot(r3), m4 = __ballot(r4),
-                   m5 = __ballot(r5), m6 = __ballot(r6);
-    if (lane == 0) {
-      uint32_t keep = 0;
-      for (int j = 0; j < 8; ++j) {
-        const int sh = 8 * j;
-        const bool rej = ((m1 >> sh) & 0xff) == 0xff || ((m2 >> sh) & 0xff) == 0xff ||
-                         ((m3 >> sh) & 0xff) == 0xff || ((m4 >> sh) & 0xff) == 0xff ||
-                         ((m5 >> sh) & 0xff) == 0xff || ((m6 >> sh) & 0xff) == 0xff;
-        if (!rej) keep |= 1u << j;
-      }
-      keep_mask[wave] = keep;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int n = 0;
-      for (int w = 0; w < 4; ++w)
-        for (int j = 0; j < 8; ++j)
-          if (keep_mask[w] & (1u << j)) keep_list[n++] = blockIdx.x * kBricksPerBlock + w * 8 + j;
-      n_keep = n;
-    }
-    __syncthreads();
-  }
-
   const float Wp1 = (float)(W + 1), Hp1 = (float)(H + 1);
-  for (int kb = wave; kb < n_keep; kb += 4) {
-    const int brick = keep_list[kb];
+  do {
     const int bz = brick % vg.bz;
     const int q = brick / vg.bz;
     const int by = q % vg.by;
@@ -1115,6 +1151,140 @@ __global__ __launch_bounds__(256) void k_integrate(void* __restrict__ tsdf_,
         }
       }
     }
+  } while (0);
+}
+
+// Brick cull: one listed group per wave, lane = (brick of the group,
+// corner): the frustum tests and the band cull; the surviving bricks are
+// appended to the brick list (block b to partition b % KF_RAY_PARTS).
+__global__ __launch_bounds__(256) void k_integ_cull(kf_cam cam, kf_vol vg, int H, int W,
+                                                    const kf_dev_state* __restrict__ st,
+                                                    GroupList gl,
+                                                    const float2* __restrict__ tiles,
+                                                    int tiles_x) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float Ti[12], t[3], cd[3];
+  load_uniform<12>(st->Tinv, Ti);
+  t[0] = st->T[3];
+  t[1] = st->T[7];
+  t[2] = st->T[11];
+  load_uniform<3>(st->center_dir, cd);
+  const float* K = cam.K;
+  uint32_t incl, excl;
+  const uint32_t n_groups = list_prefix(gl.gcount, incl, excl);
+  // the next integration appends to the other slot, which nothing reads now
+  if (blockIdx.x == 0 && threadIdx.x < 64) gl.gnext[lane * KF_RAY_COUNT_STRIDE] = 0u;
+  const uint32_t n_waves = gridDim.x * 4;
+  const int corner = lane & 7, b = lane >> 3;
+  const int part = blockIdx.x & (KF_RAY_PARTS - 1);
+  for (uint32_t e = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); e < n_groups;
+       e += n_waves) {
+    const int g = list_entry(gl.glist, gl.gcap, incl, excl, e);
+    const int gz = g % gl.gz, q = g / gl.gz;
+    const int gy = q % gl.gy, gx = q / gl.gy;
+    const int bx = gx * 2 + (b >> 2), by = gy * 2 + ((b >> 1) & 1), bz = gz * 2 + (b & 1);
+    const bool inside = bx < vg.bx && by < vg.by && bz < vg.bz;
+    bool r1 = true, r2 = true, r3 = true, r4 = true, r5 = true, r6 = false;
+    if (inside) {
+      const int x = min(bx * 8 + ((corner & 4) ? 7 : 0), vg.nx - 1);
+      const int y = min(by * 8 + ((corner & 2) ? 7 : 0), vg.ny - 1);
+      const int z = min(bz * 8 + ((corner & 1) ? 7 : 0), vg.nz - 1);
+      const BoxCorner c = integ_corner(x, y, z, vg, Ti, t, cd, K, H, W, gl.any_side != 0);
+      r1 = c.r1, r2 = c.r2, r3 = c.r3, r4 = c.r4, r5 = c.r5;
+      const bool front = c.front;
+      const double px = c.px, py = c.py, pz = c.pz, tx = c.tx, ty = c.ty, tz = c.tz;
+      const double cx = c.cx, cy = c.cy, cz = c.cz;
+      // band cull: every voxel of the brick is nearer than the updatable
+      // distances of every pixel it can project to, or farther
+      double u = front ? px / pz : 0.0, v = front ? py / pz : 0.0;
+      double umin = u, umax = u, vmin = v, vmax = v;
+      double dmax = sqrt(tx * tx + ty * ty + tz * tz);
+      double xmin = cx, xmax = cx, ymin = cy, ymax = cy, zmin = cz, zmax = cz;
+      int all_front = front ? 1 : 0;
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        umin = fmin(umin, __shfl_xor(umin, o, 8));
+        umax = fmax(umax, __shfl_xor(umax, o, 8));
+        vmin = fmin(vmin, __shfl_xor(vmin, o, 8));
+        vmax = fmax(vmax, __shfl_xor(vmax, o, 8));
+        dmax = fmax(dmax, __shfl_xor(dmax, o, 8));
+        xmin = fmin(xmin, __shfl_xor(xmin, o, 8));
+        xmax = fmax(xmax, __shfl_xor(xmax, o, 8));
+        ymin = fmin(ymin, __shfl_xor(ymin, o, 8));
+        ymax = fmax(ymax, __shfl_xor(ymax, o, 8));
+        zmin = fmin(zmin, __shfl_xor(zmin, o, 8));
+        zmax = fmax(zmax, __shfl_xor(zmax, o, 8));
+        all_front = all_front & __shfl_xor(all_front, o, 8);
+      }
+      if (all_front) {
+        // footprint: the voxels project inside the hull of the corner
+        // projections; one pixel more on each side covers roundf and the
+        // float evaluation
+        const int u0 = (int)fmax(floor(umin) - 1.0, 1.0), u1 = (int)fmin(ceil(umax) + 1.0, (double)W);
+        const int v0 = (int)fmax(floor(vmin) - 1.0, 1.0), v1 = (int)fmin(ceil(vmax) + 1.0, (double)H);
+        if (u0 <= u1 && v0 <= v1) {
+          const int tx0 = (u0 - 1) >> 4, tx1 = (u1 - 1) >> 4;
+          const int ty0 = (v0 - 1) >> 4, ty1 = (v1 - 1) >> 4;
+          const int ntx = tx1 - tx0 + 1, nt = ntx * (ty1 - ty0 + 1);
+          if (nt <= 64) {
+            float lo = __builtin_inff(), hi = -__builtin_inff();
+            for (int k = corner; k < nt; k += 8) {
+              const float2 bb = tiles[(ty0 + k / ntx) * tiles_x + tx0 + k % ntx];
+              lo = fminf(lo, bb.x);
+              hi = fmaxf(hi, bb.y);
+            }
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1) {
+              lo = fminf(lo, __shfl_xor(lo, o, 8));
+              hi = fmaxf(hi, __shfl_xor(hi, o, 8));
+            }
+            // nearest voxel centre of the brick
+            const double ex = fmax(fmax(xmin - (double)t[0], (double)t[0] - xmax), 0.0);
+            const double ey = fmax(fmax(ymin - (double)t[1], (double)t[1] - ymax), 0.0);
+            const double ez = fmax(fmax(zmin - (double)t[2], (double)t[2] - zmax), 0.0);
+            const double dmin = sqrt(ex * ex + ey * ey + ez * ez);
+            const double margin = 1.0 + 1e-5 * dmax;
+            r6 = dmax + margin < (double)lo || dmin - margin > (double)hi;
+          }
+        }
+      }
+    }
+    const uint32_t keep = wave_box_keep(r1, r2, r3, r4, r5, r6);
+    if (keep) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&gl.bcount[part * KF_RAY_COUNT_STRIDE], (uint32_t)__popc(keep));
+      base = __shfl(base, 0);
+      if (lane < 8 && ((keep >> lane) & 1u)) {
+        const int kx = gx * 2 + (lane >> 2), ky = gy * 2 + ((lane >> 1) & 1), kz = gz * 2 + (lane & 1);
+        gl.blist[part * gl.bcap + base + __popc(keep & ((1u << lane) - 1u))] =
+            (kx * vg.by + ky) * vg.bz + kz;
+      }
+    }
+  }
+}
+
+// Brick update: one listed brick per wave.
+template <int PREC, bool FIRST>
+__global__ __launch_bounds__(256) void k_integrate(
+    void* __restrict__ tsdf_, void* __restrict__ weight_, const float2* __restrict__ tab,
+    kf_cam cam, kf_vol vg, int H, int W, float trunc, float smin, float smax,
+    const kf_dev_state* __restrict__ st, GroupList gl) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float Ti[12], t[3], cd[3];
+  load_uniform<12>(st->Tinv, Ti);
+  t[0] = st->T[3];
+  t[1] = st->T[7];
+  t[2] = st->T[11];
+  load_uniform<3>(st->center_dir, cd);
+  uint32_t incl, excl;
+  const uint32_t n_bricks = list_prefix(gl.bcount, incl, excl);
+  if (blockIdx.x == 0 && threadIdx.x < 64) gl.bnext[lane * KF_RAY_COUNT_STRIDE] = 0u;
+  const uint32_t n_waves = gridDim.x * 4;
+  for (uint32_t e = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); e < n_bricks;
+       e += n_waves) {
+    const int brick = list_entry(gl.blist, gl.bcap, incl, excl, e);
+    integrate_brick<PREC, FIRST>(tsdf_, weight_, tab, cam.K, vg, H, W, trunc, smin, smax, Ti, t,
+                                 cd, brick);
   }
 }
 
@@ -2458,23 +2628,112 @@ hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
   }
 }
 
+static int integ_groups(const kf_vol& vg, int* gx, int* gy, int* gz) {
+  *gx = (vg.bx + 1) / 2;
+  *gy = (vg.by + 1) / 2;
+  *gz = (vg.bz + 1) / 2;
+  return *gx * *gy * *gz;
+}
+
+// group-cull blocks of k_integ_table (32 groups each) and list entries per
+// partition
+static int integ_group_blocks(const kf_vol& vg) {
+  int gx, gy, gz;
+  return (integ_groups(vg, &gx, &gy, &gz) + 31) / 32;
+}
+
+// Grids of the brick cull (one listed group per wave) and of the brick
+// update (one listed brick per wave); both loop over longer lists.
+constexpr int kIntegCullBlocks = 512;
+constexpr int kIntegBlocks = 2048;
+
+// list entries per partition: the groups of the group-cull blocks that
+// append to it, and the bricks of the brick-cull blocks that append to it
+static int integ_group_cap(const kf_vol& vg) {
+  return (integ_group_blocks(vg) + KF_RAY_PARTS - 1) / KF_RAY_PARTS * 32;
+}
+
+static int integ_brick_cap(const kf_vol& vg) {
+  int gx, gy, gz;
+  const int n_groups = integ_groups(vg, &gx, &gy, &gz);
+  // a brick-cull block takes at most ceil(n_groups / (4 blocks)) groups
+  const int per_block = (n_groups + 4 * kIntegCullBlocks - 1) / (4 * kIntegCullBlocks) * 4;
+  return (kIntegCullBlocks + KF_RAY_PARTS - 1) / KF_RAY_PARTS * per_block * 8;
+}
+
+size_t kf_integ_group_entries(const kf_vol& vg) {
+  return (size_t)KF_RAY_PARTS * (integ_group_cap(vg) + integ_brick_cap(vg));
+}
+
+// Whether lc . K^-1 (u, v, 1) > 0 over the image (at its four corners, with
+// a margin), lc = K^-1 (K[5], K[2], 1): the condition under which the box
+// culls may apply the image-side tests to boxes that reach behind the
+// camera plane (see integ_corner).
+static bool integ_side_cull_ok(const kf_cam& cam, int H, int W) {
+  const float* Ki = cam.Kinv;
+  const float* K = cam.K;
+  double lc[3];
+  for (int r = 0; r < 3; ++r)
+    lc[r] = (double)Ki[3 * r] * K[5] + (double)Ki[3 * r + 1] * K[2] + (double)Ki[3 * r + 2];
+  const double us[2] = {0.5, W + 0.5}, vs[2] = {0.5, H + 0.5};
+  for (double u : us)
+    for (double v : vs) {
+      double dot = 0.0;
+      for (int r = 0; r < 3; ++r)
+        dot += lc[r] * ((double)Ki[3 * r] * u + (double)Ki[3 * r + 1] * v + (double)Ki[3 * r + 2]);
+      const double nl = sqrt(lc[0] * lc[0] + lc[1] * lc[1] + lc[2] * lc[2]);
+      double nr = 0.0;
+      for (int r = 0; r < 3; ++r) {
+        const double q = (double)Ki[3 * r] * u + (double)Ki[3 * r + 1] * v + (double)Ki[3 * r + 2];
+        nr += q * q;
+      }
+      if (!(dot > 0.05 * nl * sqrt(nr))) return false;
+    }
+  return true;
+}
+
+// The lists and their counters: integ_groups holds the group list, then the
+// brick list; integ_counts holds [group, brick] x [slot 0, slot 1] counter
+// blocks of KF_RAY_PARTS lines.
+static GroupList make_group_list(const kf_context* c, const kf_cam& cam) {
+  GroupList gl;
+  gl.any_side = integ_side_cull_ok(cam, c->H, c->W) ? 1 : 0;
+  integ_groups(c->vg, &gl.gx, &gl.gy, &gl.gz);
+  const size_t lines = (size_t)KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
+  const int s = c->integ_slot;
+  gl.gcap = integ_group_cap(c->vg);
+  gl.bcap = integ_brick_cap(c->vg);
+  gl.glist = c->integ_groups;
+  gl.blist = c->integ_groups + (size_t)KF_RAY_PARTS * gl.gcap;
+  gl.gcount = c->integ_counts + (size_t)s * lines;
+  gl.gnext = c->integ_counts + (size_t)(s ^ 1) * lines;
+  gl.bcount = c->integ_counts + (size_t)(2 + s) * lines;
+  gl.bnext = c->integ_counts + (size_t)(2 + (s ^ 1)) * lines;
+  return gl;
+}
+
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam) {
-  const dim3 grid((c->W + 15) / 16, (c->H + 15) / 16);
+  const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
+  const int group_rows = (integ_group_blocks(c->vg) + tiles_x - 1) / tiles_x;
+  const dim3 grid(tiles_x, tiles_y + group_rows);
   k_integ_table<<<grid, 256, 0, c->stream>>>(c->depth, cam, c->H, c->W, c->state, c->integ_tab,
                                              c->integ_tiles, c->prm.sdf_min_mm,
-                                             c->prm.sdf_max_mm);
+                                             c->prm.sdf_max_mm, tiles_y, c->vg,
+                                             make_group_list(c, cam));
   return hipGetLastError();
 }
 
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
-  const int blocks = (c->n_bricks + kBricksPerBlock - 1) / kBricksPerBlock;
   const kf_params& p = c->prm;
-#define KF_INTEG(P, FI)                                                                    \
-  k_integrate<P, FI><<<blocks, 256, 0, c->stream>>>(c->tsdf, c->weight, c->integ_tab, cam, \
-                                                    c->vg, c->H, c->W, p.truncation_mm,    \
-                                                    p.sdf_min_mm, p.sdf_max_mm, c->state,  \
-                                                    c->n_bricks, c->integ_tiles,           \
-                                                    (c->W + 15) / 16, (c->H + 15) / 16)
+  const GroupList gl = make_group_list(c, cam);
+  c->integ_slot ^= 1;
+  k_integ_cull<<<kIntegCullBlocks, 256, 0, c->stream>>>(cam, c->vg, c->H, c->W, c->state, gl,
+                                                         c->integ_tiles, (c->W + 15) / 16);
+#define KF_INTEG(P, FI)                                                                   \
+  k_integrate<P, FI><<<kIntegBlocks, 256, 0, c->stream>>>(c->tsdf, c->weight, c->integ_tab, \
+                                                          cam, c->vg, c->H, c->W,           \
+                                                          p.truncation_mm, p.sdf_min_mm,    \
+                                                          p.sdf_max_mm, c->state, gl)
   if (c->vol.precision == KF_TSDF_F32) {
     if (first)
       KF_INTEG(KF_TSDF_F32, true);
