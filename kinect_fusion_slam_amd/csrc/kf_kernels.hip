@@ -855,31 +855,9 @@ struct GroupList {
   int any_side;       // side tests without the front requirement (see integ_corner)
 };
 
-// Lane l of the calling wave: partition l's count (KF_RAY_PARTS == 64) and
-// its exclusive prefix sum; returns the list length.  Entry e then lies in
-// partition p = #{l : incl_l <= e}, at offset e - excl_p.
-__device__ __forceinline__ uint32_t list_prefix(const uint32_t* count, uint32_t& incl,
-                                                uint32_t& excl) {
-  static_assert(KF_RAY_PARTS == 64, "one list partition per lane");
-  const int lane = threadIdx.x & 63;
-  const uint32_t pc = count[lane * KF_RAY_COUNT_STRIDE];
-  incl = pc;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(incl, o);
-    incl += lane >= o ? u : 0u;
-  }
-  excl = incl - pc;
-  return __builtin_amdgcn_readlane(incl, 63);
-}
-
-// Entry e (wave-uniform, below the list length) of a partitioned list.
-__device__ __forceinline__ int list_entry(const int* list, int cap, uint32_t incl,
-                                          uint32_t excl, uint32_t e) {
-  const int part = __builtin_amdgcn_readfirstlane(__popcll(__ballot(incl <= e)));
-  const uint32_t off = e - (uint32_t)__builtin_amdgcn_readlane((int)excl, part);
-  return __builtin_amdgcn_readfirstlane(list[part * cap + off]);
-}
+// Consumers of a partitioned list: wave w of the grid serves partition
+// w % KF_RAY_PARTS and takes its entries w / KF_RAY_PARTS + k * (waves /
+// KF_RAY_PARTS), reading one counter.
 
 // Per-pixel part: for a pixel (u, v) with depth d != 0, the ray magnitude
 // |T(2 lambda) - T(lambda)|, the predicted depth |T(d lambda) - t| / |.| and
@@ -1170,16 +1148,16 @@ __global__ __launch_bounds__(256) void k_integ_cull(kf_cam cam, kf_vol vg, int H
   t[2] = st->T[11];
   load_uniform<3>(st->center_dir, cd);
   const float* K = cam.K;
-  uint32_t incl, excl;
-  const uint32_t n_groups = list_prefix(gl.gcount, incl, excl);
+  const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  const int gpart = wid & (KF_RAY_PARTS - 1);
+  const uint32_t n_groups = __builtin_amdgcn_readfirstlane(gl.gcount[gpart * KF_RAY_COUNT_STRIDE]);
   // the next integration appends to the other slot, which nothing reads now
-  if (blockIdx.x == 0 && threadIdx.x < 64) gl.gnext[lane * KF_RAY_COUNT_STRIDE] = 0u;
-  const uint32_t n_waves = gridDim.x * 4;
+  if (blockIdx.x == 0 && threadIdx.x < KF_RAY_PARTS) gl.gnext[lane * KF_RAY_COUNT_STRIDE] = 0u;
+  const uint32_t stride = gridDim.x * 4 / KF_RAY_PARTS;
   const int corner = lane & 7, b = lane >> 3;
   const int part = blockIdx.x & (KF_RAY_PARTS - 1);
-  for (uint32_t e = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); e < n_groups;
-       e += n_waves) {
-    const int g = list_entry(gl.glist, gl.gcap, incl, excl, e);
+  for (uint32_t e = (uint32_t)wid / KF_RAY_PARTS; e < n_groups; e += stride) {
+    const int g = __builtin_amdgcn_readfirstlane(gl.glist[gpart * gl.gcap + e]);
     const int gz = g % gl.gz, q = g / gl.gz;
     const int gy = q % gl.gy, gx = q / gl.gy;
     const int bx = gx * 2 + (b >> 2), by = gy * 2 + ((b >> 1) & 1), bz = gz * 2 + (b & 1);
@@ -1276,13 +1254,13 @@ __global__ __launch_bounds__(256) void k_integrate(
   t[1] = st->T[7];
   t[2] = st->T[11];
   load_uniform<3>(st->center_dir, cd);
-  uint32_t incl, excl;
-  const uint32_t n_bricks = list_prefix(gl.bcount, incl, excl);
-  if (blockIdx.x == 0 && threadIdx.x < 64) gl.bnext[lane * KF_RAY_COUNT_STRIDE] = 0u;
-  const uint32_t n_waves = gridDim.x * 4;
-  for (uint32_t e = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave); e < n_bricks;
-       e += n_waves) {
-    const int brick = list_entry(gl.blist, gl.bcap, incl, excl, e);
+  const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  const int part = wid & (KF_RAY_PARTS - 1);
+  const uint32_t n_bricks = __builtin_amdgcn_readfirstlane(gl.bcount[part * KF_RAY_COUNT_STRIDE]);
+  if (blockIdx.x == 0 && threadIdx.x < KF_RAY_PARTS) gl.bnext[lane * KF_RAY_COUNT_STRIDE] = 0u;
+  const uint32_t stride = gridDim.x * 4 / KF_RAY_PARTS;
+  for (uint32_t e = (uint32_t)wid / KF_RAY_PARTS; e < n_bricks; e += stride) {
+    const int brick = __builtin_amdgcn_readfirstlane(gl.blist[part * gl.bcap + e]);
     integrate_brick<PREC, FIRST>(tsdf_, weight_, tab, cam.K, vg, H, W, trunc, smin, smax, Ti, t,
                                  cd, brick);
   }
@@ -2344,7 +2322,8 @@ __device__ __forceinline__ void uniform_ray(RayState& r) {
 // Pass 2: finishes the queued rays, one ray per wave (wave_walk).  Rays are
 // numbered over the concatenated partitions and dealt to waves in a fixed
 // order (wave w takes rays w, w + waves, ...), so that the ray loop has no
-// lane-dependent branch ahead of its cross-lane steps.  Lane l holds
+// lane-dependent branch ahead of its cross-lane steps and the long rays,
+// which cluster in some partitions, spread over all waves.  Lane l holds
 // partition l's count and its inclusive prefix sum; ray q lies in the
 // partition p = #{l : incl_l <= q}, at offset q - (incl_p - count_p).
 template <int PREC, bool COUNT, int MODE>
@@ -2653,12 +2632,16 @@ static int integ_group_cap(const kf_vol& vg) {
   return (integ_group_blocks(vg) + KF_RAY_PARTS - 1) / KF_RAY_PARTS * 32;
 }
 
+static_assert(kIntegCullBlocks * 4 % KF_RAY_PARTS == 0 && kIntegBlocks * 4 % KF_RAY_PARTS == 0,
+              "list consumers need whole waves per partition");
+
 static int integ_brick_cap(const kf_vol& vg) {
-  int gx, gy, gz;
-  const int n_groups = integ_groups(vg, &gx, &gy, &gz);
-  // a brick-cull block takes at most ceil(n_groups / (4 blocks)) groups
-  const int per_block = (n_groups + 4 * kIntegCullBlocks - 1) / (4 * kIntegCullBlocks) * 4;
-  return (kIntegCullBlocks + KF_RAY_PARTS - 1) / KF_RAY_PARTS * per_block * 8;
+  // a brick-cull wave takes at most ceil(gcap / waves per partition) groups
+  // of its partition, 8 bricks each; blocks b, b + 64, ... append to
+  // partition b % 64
+  const int per_part = kIntegCullBlocks * 4 / KF_RAY_PARTS;
+  const int per_wave = (integ_group_cap(vg) + per_part - 1) / per_part;
+  return (kIntegCullBlocks + KF_RAY_PARTS - 1) / KF_RAY_PARTS * 4 * per_wave * 8;
 }
 
 size_t kf_integ_group_entries(const kf_vol& vg) {
