@@ -432,40 +432,35 @@ struct IcpShared {
   int status;
 };
 
-// Sums the ROWS x nb partials into sh.sums (all threads take part).  Wave w
-// takes rows w, w + NW, ...; each lane loads its up to
-// KF_ICP_MAX_PART_PER_LANE entries of those rows with unconditional
-// (clamped) loads, so that they are all in flight together, adds them in
-// FP64, and the 64 lane sums of a row are added by a DPP tree (a fixed
-// order).
-template <int ROWS, int NT, typename PT>
+// Sums each of the ROWS rows of nb partials into sh.rowsum (all threads
+// take part).  Wave w takes rows w, w + NW, ...; each lane loads its PPL
+// entries of those rows (nb <= 64 PPL) with unconditional (clamped) loads,
+// so that they are all in flight together, adds them in FP64, and the 64
+// lane sums of a row are added by a DPP tree (a fixed order).  Ends with a
+// block barrier; icp_update_pose forms the sums.
+template <int ROWS, int NT, int PPL, typename PT>
 __device__ __forceinline__ void icp_reduce_partials(const PT* __restrict__ P, int nb,
                                                     IcpShared<ROWS>& sh) {
   const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
   constexpr int NW = NT / 64;
   constexpr int RPW = (ROWS + NW - 1) / NW;  // rows per wave
-  PT v[RPW][KF_ICP_MAX_PART_PER_LANE];
+  PT v[RPW][PPL];
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     const int row = min(wave + r * NW, ROWS - 1);
 #pragma unroll
-    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j)
+    for (int j = 0; j < PPL; ++j)
       v[r][j] = P[(size_t)row * nb + min(ln + 64 * j, nb - 1)];
   }
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
     double t = 0.0;
 #pragma unroll
-    for (int j = 0; j < KF_ICP_MAX_PART_PER_LANE; ++j)
+    for (int j = 0; j < PPL; ++j)
       t += ln + 64 * j < nb ? (double)v[r][j] : 0.0;
     t = wave_total(t);
     const int row = wave + r * NW;
     if (ln == 63 && row < ROWS) sh.rowsum[row] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < KF_ICP_SUMS) {
-    const int k = threadIdx.x;
-    sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? sh.rowsum[2 * k] + sh.rowsum[2 * k + 1] : sh.rowsum[k];
   }
   __syncthreads();
 }
@@ -477,11 +472,18 @@ __device__ __forceinline__ void icp_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Finishes an iteration in wave 0: the solve (one lane), T_new = inc(sums) *
-// T0 and F = Tp * T_new (16 lanes each), then one block barrier.
+// Finishes an iteration in wave 0: the sums from the row sums (a sum and
+// its compensation row in the compensated variant), the solve (one lane),
+// T_new = inc(sums) * T0 and F = Tp * T_new (16 lanes each), then one block
+// barrier.
 template <int ROWS>
 __device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh) {
   if (threadIdx.x < 64) {
+    if (threadIdx.x < KF_ICP_SUMS) {
+      const int k = threadIdx.x;
+      sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? sh.rowsum[2 * k] + sh.rowsum[2 * k + 1] : sh.rowsum[k];
+    }
+    icp_wave_sync();
     if (threadIdx.x == 0) sh.status = kf_icp_increment(sh.sums, sh.inc);
     icp_wave_sync();
     if (threadIdx.x < 16)
@@ -506,7 +508,7 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 
 // Per-pixel association and gates (sensor_pose_estimation.cu:16-136) plus the
 // block reduction, for iteration `iter`.
-template <bool KAHAN, int PX>
+template <bool KAHAN, int PX, int PPL>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const float4* __restrict__ vmap, const float4* __restrict__ nmap,
     const float4* __restrict__ mv, const float4* __restrict__ mn, kf_cam cam, int H, int W,
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       sh.T0[threadIdx.x] = iter == 1 ? st->Tprev[threadIdx.x] : st->iter_T[iter - 2][threadIdx.x];
       sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
     }
-    icp_reduce_partials<ROWS, KF_ICP_BLOCK, PT>(part_in, nb, sh);
+    icp_reduce_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(part_in, nb, sh);
     icp_update_pose<ROWS>(sh);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
   }
@@ -713,7 +715,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 // Finishes the last iteration: final pose, its inverse and the record.
 constexpr int kIcpFinishThreads = 1024;
 
-template <bool KAHAN>
+template <bool KAHAN, int PPL>
 __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* __restrict__ st,
                                                              const void* __restrict__ part_in,
                                                              int nb, int last) {
@@ -723,7 +725,7 @@ __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* 
     sh.T0[threadIdx.x] = last == 0 ? st->Tprev[threadIdx.x] : st->iter_T[last - 1][threadIdx.x];
     sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
   }
-  icp_reduce_partials<ROWS, kIcpFinishThreads>(static_cast<const IcpPartial<KAHAN>*>(part_in),
+  icp_reduce_partials<ROWS, kIcpFinishThreads, PPL>(static_cast<const IcpPartial<KAHAN>*>(part_in),
                                                nb, sh);
   icp_update_pose<ROWS>(sh);
   icp_record<ROWS>(st, last, sh);
@@ -2567,7 +2569,7 @@ hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam) {
   return hipGetLastError();
 }
 
-template <bool KAHAN, int PX>
+template <bool KAHAN, int PX, int PPL>
 static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
   const int iters = c->prm.icp_iterations;
   const int n = c->H * c->W;
@@ -2580,7 +2582,7 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
     uint8_t* m = (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
     const void* pin = pbase + (size_t)((it + 1) & 1) * half;
     void* pout = pbase + (size_t)(it & 1) * half;
-    k_icp_iter<KAHAN, PX><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
+    k_icp_iter<KAHAN, PX, PPL><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
         c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
         c->icp_cos_lo, c->state, pin, pout, m, it);
     hipError_t e = hipGetLastError();
@@ -2588,22 +2590,33 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
   }
   if (iters > 0) {
     const void* pin = pbase + (size_t)((iters - 1) & 1) * half;
-    k_icp_finish<KAHAN><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb, iters - 1);
+    k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb,
+                                                                      iters - 1);
   }
   return hipGetLastError();
 }
 
-hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
+// Partials per lane of the iteration prologue: 4 covers up to 256 blocks
+// (one per compute unit on this GPU), 8 the maximum block count.
+template <int PX>
+static hipError_t launch_icp_px(kf_context* c, const kf_cam& cam) {
   const bool kahan = c->prm.icp_compensated != 0;
+  if (c->icp_blocks <= 4 * 64)
+    return kahan ? launch_icp_t<true, PX, 4>(c, cam) : launch_icp_t<false, PX, 4>(c, cam);
+  return kahan ? launch_icp_t<true, PX, KF_ICP_MAX_PART_PER_LANE>(c, cam)
+               : launch_icp_t<false, PX, KF_ICP_MAX_PART_PER_LANE>(c, cam);
+}
+
+hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
   switch (c->icp_px) {
     case 1:
-      return kahan ? launch_icp_t<true, 1>(c, cam) : launch_icp_t<false, 1>(c, cam);
+      return launch_icp_px<1>(c, cam);
     case 2:
-      return kahan ? launch_icp_t<true, 2>(c, cam) : launch_icp_t<false, 2>(c, cam);
+      return launch_icp_px<2>(c, cam);
     case 3:
-      return kahan ? launch_icp_t<true, 3>(c, cam) : launch_icp_t<false, 3>(c, cam);
+      return launch_icp_px<3>(c, cam);
     default:
-      return kahan ? launch_icp_t<true, 4>(c, cam) : launch_icp_t<false, 4>(c, cam);
+      return launch_icp_px<4>(c, cam);
   }
 }
 
