@@ -271,20 +271,75 @@ __global__ __launch_bounds__(256) void k_depth_u16(const uint16_t* __restrict__ 
 // expf(-(t * t) / den) depends only on the integer |t|: with lut_n > 0 it is
 // read from the table of those exact float results (staged in LDS), entries
 // past lut_n being exactly 0.
-template <int R, bool U16>
-__global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
-                                                   float* __restrict__ out,
-                                                   const float* __restrict__ ws, int H, int W,
-                                                   float sigma_r, float dmin, float dmax,
-                                                   const float* __restrict__ lut, int lut_n) {
-  constexpr int TS = 16, L = 2 * R + 1, SW = TS + 2 * R;
-  __shared__ float tile[SW][SW + 1];
-  __shared__ float sw[L * L];
-  __shared__ float s_lut[U16 ? KF_RANGE_LUT_MAX : 1];
-  const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+// The filtered value of the pixel whose window starts at tile[ty][tx] (the
+// window is tile[ty .. ty + 2R][tx .. tx + 2R], centre tile[ty + R][tx + R]),
+// with the reference's arithmetic (bilateral_filtering.cu:26-44).  lut_n > 0:
+// range weights from the table of whole-millimetre differences (s_lut).
+template <int R, int PITCH>
+__device__ __forceinline__ float bilateral_value(const float (*tile)[PITCH], int ty, int tx,
+                                                 const float* sw, const float* s_lut, int lut_n,
+                                                 float sigma_r) {
+  constexpr int L = 2 * R + 1;
+  const float dc = tile[ty + R][tx + R];
+  const float den = 2 * sigma_r * sigma_r;
+  float sum = 0.f, fin = 0.f;
+  if (lut_n > 0) {
+    float w[L * L];
+#pragma unroll
+    for (int k = 0; k < L * L; ++k) {
+      const int j = (int)fabsf(tile[ty + k / L][tx + k % L] - dc);
+      const float t = j < lut_n ? s_lut[j] : 0.0f;
+      w[k] = sw[k] * t + 0.000000001f;
+      sum = sum + w[k];
+    }
+#pragma unroll
+    for (int k = 0; k < L * L; ++k) {
+      const float wr = w[k] / sum;
+      fin = fin + wr * tile[ty + k / L][tx + k % L];
+    }
+  } else if constexpr (L * L <= 49) {
+    // small windows: keep the weights in registers (fully unrolled)
+    float w[L * L];
+#pragma unroll
+    for (int k = 0; k < L * L; ++k) {
+      float t = tile[ty + k / L][tx + k % L] - dc;
+      t = kf_expf(-(t * t) / den);
+      w[k] = sw[k] * t + 0.000000001f;
+      sum = sum + w[k];
+    }
+#pragma unroll
+    for (int k = 0; k < L * L; ++k) {
+      const float wr = w[k] / sum;
+      fin = fin + wr * tile[ty + k / L][tx + k % L];
+    }
+  } else {
+    // large windows: recompute the (identical) weights in the second pass
+    for (int k = 0; k < L * L; ++k) {
+      float t = tile[ty + k / L][tx + k % L] - dc;
+      t = kf_expf(-(t * t) / den);
+      sum = sum + (sw[k] * t + 0.000000001f);
+    }
+    for (int k = 0; k < L * L; ++k) {
+      const float v = tile[ty + k / L][tx + k % L];
+      float t = v - dc;
+      t = kf_expf(-(t * t) / den);
+      const float wr = (sw[k] * t + 0.000000001f) / sum;
+      fin = fin + wr * v;
+    }
+  }
+  return fin;
+}
+
+// Stages the (SW)^2 window with top-left pixel (x0 - R, y0 - R) of the depth
+// into tile, read by linear index y*W + x exactly as the reference reads it
+// (a column past the right edge wraps to the next row; indices outside the
+// image read 0).  U16: the raw frame with the range gate applied.
+template <int R, int SW, bool U16>
+__device__ __forceinline__ void bilateral_stage(const void* __restrict__ in_, float (*tile)[SW + 1],
+                                                int x0, int y0, int H, int W, float dmin,
+                                                float dmax) {
   const int n = H * W;
-  for (int i = tid; i < SW * SW; i += 256) {
+  for (int i = threadIdx.x; i < SW * SW; i += 256) {
     const int r = i / SW, cc = i - r * SW;
     const int idx = (y0 - R + r) * W + (x0 - R + cc);
     float v = 0.0f;
@@ -298,6 +353,21 @@ __global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
     }
     tile[r][cc] = v;
   }
+}
+
+template <int R, bool U16>
+__global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
+                                                   float* __restrict__ out,
+                                                   const float* __restrict__ ws, int H, int W,
+                                                   float sigma_r, float dmin, float dmax,
+                                                   const float* __restrict__ lut, int lut_n) {
+  constexpr int TS = 16, L = 2 * R + 1, SW = TS + 2 * R;
+  __shared__ float tile[SW][SW + 1];
+  __shared__ float sw[L * L];
+  __shared__ float s_lut[U16 ? KF_RANGE_LUT_MAX : 1];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+  bilateral_stage<R, SW, U16>(in_, tile, x0, y0, H, W, dmin, dmax);
   for (int i = tid; i < L * L; i += 256) sw[i] = ws[i];
   if constexpr (U16)
     for (int i = tid; i < lut_n; i += 256) s_lut[i] = lut[i];
@@ -306,80 +376,28 @@ __global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
   const int x = x0 + tx, y = y0 + ty;
   if (x >= W || y >= H) return;
   float result = 0.0f;
-  if (y >= R && y <= H - R && x >= R && x <= W - R) {
-    const float dc = tile[ty + R][tx + R];
-    const float den = 2 * sigma_r * sigma_r;
-    float sum = 0.f, fin = 0.f;
-    if (U16 && lut_n > 0) {
-      float w[L * L];
-#pragma unroll
-      for (int k = 0; k < L * L; ++k) {
-        const int j = (int)fabsf(tile[ty + k / L][tx + k % L] - dc);
-        const float t = j < lut_n ? s_lut[j] : 0.0f;
-        w[k] = sw[k] * t + 0.000000001f;
-        sum = sum + w[k];
-      }
-#pragma unroll
-      for (int k = 0; k < L * L; ++k) {
-        const float wr = w[k] / sum;
-        fin = fin + wr * tile[ty + k / L][tx + k % L];
-      }
-    } else if constexpr (L * L <= 49) {
-      // small windows: keep the weights in registers (fully unrolled)
-      float w[L * L];
-#pragma unroll
-      for (int k = 0; k < L * L; ++k) {
-        float t = tile[ty + k / L][tx + k % L] - dc;
-        t = kf_expf(-(t * t) / den);
-        w[k] = sw[k] * t + 0.000000001f;
-        sum = sum + w[k];
-      }
-#pragma unroll
-      for (int k = 0; k < L * L; ++k) {
-        const float wr = w[k] / sum;
-        fin = fin + wr * tile[ty + k / L][tx + k % L];
-      }
-    } else {
-      // large windows: recompute the (identical) weights in the second pass
-      for (int k = 0; k < L * L; ++k) {
-        float t = tile[ty + k / L][tx + k % L] - dc;
-        t = kf_expf(-(t * t) / den);
-        sum = sum + (sw[k] * t + 0.000000001f);
-      }
-      for (int k = 0; k < L * L; ++k) {
-        const float v = tile[ty + k / L][tx + k % L];
-        float t = v - dc;
-        t = kf_expf(-(t * t) / den);
-        const float wr = (sw[k] * t + 0.000000001f) / sum;
-        fin = fin + wr * v;
-      }
-    }
-    result = fin;
-  }
+  if (y >= R && y <= H - R && x >= R && x <= W - R)
+    result = bilateral_value<R, SW + 1>(tile, ty, tx, sw, s_lut, U16 ? lut_n : 0, sigma_r);
   out[y * W + x] = result;
 }
 
 // ---------------------------------------------------------------------------
 // A3  vertex and normal maps (raw_vertices.cu:3-51).  Writes every pixel, so
 // the zero defaults of the reference's CleanDepthData need no separate fill.
-__global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_cam cam, int H,
-                                                int W, float4* __restrict__ vmap,
-                                                float4* __restrict__ nmap,
-                                                uint8_t* __restrict__ mask) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= H * W) return;
-  const int y = i / W, x = i - y * W;
-  const float* Ki = cam.Kinv;
-  const float di = d[i];
-  float4 V = make_float4(0.f, 0.f, di, 0.f);
-  float4 N = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint8_t m = 0;
+// Vertex, normal and mask of pixel (x, y) from its depth di and the depths
+// of the pixels below (dr) and to the right (dcn); dr and dcn are read only
+// where the reference reads them (di != 0, not the last row or column).
+__device__ __forceinline__ void vertex_at(int x, int y, int H, int W, const float* Ki, float di,
+                                          float dr, float dcn, float4& V, float4& N,
+                                          uint8_t& m) {
+  V = make_float4(0.f, 0.f, di, 0.f);
+  N = make_float4(0.f, 0.f, 0.f, 0.f);
+  m = 0;
   if (di != 0 && y != (H - 1) && x != (W - 1)) {
     m = 1;
     V.w = 1.0f;
     const int nr = (y + 1) * W + x;
     const int nc = y * W + (x + 1);
-    const float dr = d[nr], dcn = d[nc];
     if (nr > 0 && nc > 0 && dr != 0 && dcn != 0) {
       const float X1 = (float)(x + 1), Y1 = (float)(y + 1);
       const float X2 = (float)(x + 2), Y2 = (float)(y + 2);
@@ -400,6 +418,24 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_
       N.z = nK / nn;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_cam cam, int H,
+                                                int W, float4* __restrict__ vmap,
+                                                float4* __restrict__ nmap,
+                                                uint8_t* __restrict__ mask) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= H * W) return;
+  const int y = i / W, x = i - y * W;
+  // the three loads are unconditional (clamped indices), so that they are in
+  // flight together; vertex_at uses the neighbours only where they exist
+  const int n = H * W;
+  const float di = d[i];
+  const float dr = d[min((y + 1) * W + x, n - 1)];
+  const float dcn = d[min(y * W + (x + 1), n - 1)];
+  float4 V, N;
+  uint8_t m;
+  vertex_at(x, y, H, W, cam.Kinv, di, dr, dcn, V, N, m);
   vmap[i] = V;
   nmap[i] = N;
   mask[i] = m;

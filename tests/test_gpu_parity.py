@@ -83,22 +83,27 @@ def test_bilateral_bitexact(ctx_room, room_stream, oracle, radius):
 @pytest.mark.parametrize("radius", [2, 5])
 @pytest.mark.parametrize("sigma_r", [5.0, 30.0, 150.0])
 def test_pipeline_gate_bilateral_bitexact(kf, room_stream, oracle, radius, sigma_r):
-    """The pipeline's fused range gate + bilateral on u16 frames.  Its range
-    weights of integer depth differences come from a table while the exact-zero
+    """The pipeline's per-frame path: fused range gate + bilateral on u16
+    frames, then the vertex/normal maps.  Its range weights of integer depth differences come from a table while the exact-zero
     tail starts within 2048 entries (sigma_r 5 and 30 mm: 73 and 433 entries)
     and from expf otherwise (150 mm)."""
     vol, prm, K, H, W, frames, gt = room_stream
     vs, ps, _, _, _ = kf.preset(kf.PRESET_ROOM)
     vs.dims[:] = [40, 40, 36]
     ps.bilateral, ps.bilateral_radius, ps.sigma_r_mm = 1, radius, sigma_r
-    say(f"[gpu] pipeline gate + bilateral r={radius} sigma_r={sigma_r}")
+    say(f"[gpu] pipeline gate + bilateral + maps r={radius} sigma_r={sigma_r}")
     with kf.KinfuContext(vs, ps, H, W) as c:
         c.stream_upload(frames[:3])
         c.pipeline_run(K, 0, 3)
         got = c.depth()
+        maps = c.vertex_maps()
     d = oracle.range_gate(frames[2], prm.depth_min_mm, prm.depth_max_mm)
     want = oracle.bilateral(d, radius, ps.sigma_s_px, sigma_r)
     assert_same(got, want, "bilateral")
+    wm = oracle.vertex_normals(want, kf.mat3_inverse(K).ravel())
+    for k in ("vx", "vy", "nx", "ny", "nz", "mask"):
+        assert_same(maps[k], wm[k], k)
+    assert_same(maps["vz"], want, "vz")
 
 
 def test_vertex_normals_bitexact(kf, ctx_room, room_stream, oracle):
