@@ -1680,16 +1680,37 @@ __device__ __forceinline__ bool tri_sample(const VolRead<PREC>& F, const kf_vol&
 // LDS hash table (key = native voxel index) and flushed with one global
 // atomic per distinct voxel; a full probe sequence, and pass 2, fall back to
 // global atomics.
-constexpr int kHashBits = 12;
-constexpr int kHash = 1 << kHashBits;
-constexpr int kHashProbes = 16;
+// The first raycast pass of the adjoint sums dL/dF in an LDS table keyed by
+// trilinear cell (the lower corner's voxel): a key and the 8 corners' sums
+// per entry, so that a sample costs one lookup, and lanes sharing a cell
+// (neighbouring rays) add to its 8 slots in lane-rotated order.
+constexpr int kCellBits = 9;
+constexpr int kCells = 1 << kCellBits;
+constexpr int kCellProbes = 16;
+
+// LDS-qualified pointers, so that the table's atomics are LDS instructions
+// (through a generic pointer the float adds become flat atomics).
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(3))) float lds_float;
 
 struct Sink {
-  float* gF;   // global dL/dF (native order); null: no TSDF gradient
-  int* key;    // LDS table keys (-1 = empty); null: global atomics only
-  float* val;  // LDS table values
+  float* gF;          // global dL/dF (native order); null: no TSDF gradient
+  bool table;         // the LDS cell table below is in use (else global atomics only)
+  lds_int* key;       // cell keys (-1 = empty), kCells
+  lds_float* val;     // corner sums, kCells x 8
   uint32_t* touched;  // one bit per brick that received dL/dF (for zeroing)
 };
+
+__device__ __forceinline__ int lds_cas(lds_int* p, int cmp, int v) {
+  int expected = cmp;
+  __hip_atomic_compare_exchange_strong(p, &expected, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+  return expected;
+}
+
+__device__ __forceinline__ void lds_fadd(lds_float* p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Marks brick b as holding dL/dF (a bit already seen set is not set again).
 __device__ __forceinline__ void mark_brick(uint32_t* touched, int b) {
@@ -1697,25 +1718,24 @@ __device__ __forceinline__ void mark_brick(uint32_t* touched, int b) {
   if (!(touched[b >> 5] & m)) atomicOr(&touched[b >> 5], m);
 }
 
-__device__ __forceinline__ void sink_add(const Sink& k, int idx, float v) {
-  if (k.key) {
-    uint32_t h = ((uint32_t)idx * 2654435761u) >> (32 - kHashBits);
-    for (int probe = 0; probe < kHashProbes; ++probe) {
-      const int old = atomicCAS(&k.key[h], -1, idx);
-      if (old == -1 || old == idx) {
-        atomicAdd(&k.val[h], v);
-        return;
-      }
-      h = (h + 1) & (kHash - 1);
-    }
+// dL/dF into voxel idx through a global atomic.  Its brick is marked (test,
+// then set if clear) only when it differs from *last, the brick this lane
+// marked last: the corners of a cell nearly always share a brick, a test per
+// corner puts a dependent load before every add, and setting without the
+// test sends every ray's lanes to the same few words.
+__device__ __forceinline__ void sink_global(const Sink& k, int idx, float v, int* last) {
+  const int b = idx >> 9;
+  if (b != *last) {
+    mark_brick(k.touched, b);
+    *last = b;
   }
-  mark_brick(k.touched, idx >> 9);
   atomicAdd(&k.gF[idx], v);
 }
 
-// dL/dF of every corner of sample s: a * weight_q + b . grad(weight_q).
-__device__ __forceinline__ void tri_scatter(const Sink& sink, const kf_vol& vg, const Tri& s,
-                                            float a, float bx, float by, float bz) {
+// dL/dF of the 8 corners q (index (dx << 2) | (dy << 1) | dz) of sample s:
+// a * weight_q + b . grad(weight_q).
+__device__ __forceinline__ void tri_corner_vals(const kf_vol& vg, const Tri& s, float a, float bx,
+                                                float by, float bz, float (&val)[8]) {
   const float d = vg.d;
   const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
   const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
@@ -1724,10 +1744,62 @@ __device__ __forceinline__ void tri_scatter(const Sink& sink, const kf_vol& vg, 
     const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
     const float wx = qx ? fx : ox, wy = qy ? fy : oy, wz = qz ? fz : oz;
     const float sx = qx ? 1.0f : -1.0f, sy = qy ? 1.0f : -1.0f, sz = qz ? 1.0f : -1.0f;
-    const float val = a * (wx * wy * wz) +
-                      (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
-    if (val != 0.0f) sink_add(sink, kf_store_index(vg, s.i + qx, s.j + qy, s.k + qz), val);
+    val[q] = a * (wx * wy * wz) +
+             (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
   }
+}
+
+// Adds val[q] to the 8 corners of the cell with lower corner (i, j, k).
+// With the cell table: keys are never removed during a pass, so an entry that
+// already holds the cell (a plain read, which lanes reading one address
+// share) needs no compare-and-swap; only a cell's first insert swaps.  The 8
+// adds go in lane-rotated corner order and return nothing.  A cell that finds
+// no entry within kCellProbes slots takes global atomics.
+__device__ __forceinline__ void sink_add_cell(const Sink& k, const kf_vol& vg, int i, int j,
+                                              int kk, const float (&val)[8]) {
+  int slot = -1;
+  if (k.table) {
+    const int cell = (i * vg.ny + j) * vg.nz + kk;
+    uint32_t h = ((uint32_t)cell * 2654435761u) >> (32 - kCellBits);
+    if (k.key[h] == cell) {
+      slot = (int)h;
+    } else {
+#pragma unroll 1
+      for (int probe = 0; probe < kCellProbes; ++probe) {
+        int expected = -1;
+        __hip_atomic_compare_exchange_strong(&k.key[h], &expected, cell, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (expected == -1 || expected == cell) {
+          slot = (int)h;
+          break;
+        }
+        h = (h + 1) & (kCells - 1);
+      }
+    }
+  }
+  if (slot >= 0) {
+    const int r = lane_id() & 7;
+    lds_float* e = k.val + slot * 8;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int q = (t + r) & 7;
+      float v = val[0];
+#pragma unroll
+      for (int u = 1; u < 8; ++u) v = q == u ? val[u] : v;
+      if (v != 0.0f) __hip_atomic_fetch_add(e + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  } else {
+    int last = -1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (val[q] != 0.0f)
+        sink_global(k, kf_store_index(vg, i + (q >> 2), j + ((q >> 1) & 1), kk + (q & 1)), val[q],
+                    &last);
+  }
+}
+
+__device__ __forceinline__ bool same_cell(const Tri& a, const Tri& b) {
+  return a.i == b.i && a.j == b.j && a.k == b.k;
 }
 
 // Trilinear hit of a ray whose walk crossed at a voxel entered at parameter
@@ -1815,7 +1887,8 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
     const float lgx = (gNv.x - nx * ndot) / gn;
     const float lgy = (gNv.y - ny * ndot) / gn;
     const float lgz = (gNv.z - nz * ndot) / gn;
-    if (sink.gF) tri_scatter(sink, vg, s, 0.0f, lgx, lgy, lgz);
+    float vs[8];
+    if (sink.gF) tri_corner_vals(vg, s, 0.0f, lgx, lgy, lgz, vs);
     // dL/dp* = H lg, H the trilinear Hessian at p* (zero diagonal)
     const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
     const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
@@ -1842,8 +1915,22 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
     const float dFa = dts * d * (-b.v) / den2;
     const float dFb = dts * d * a.v / den2;
     if (sink.gF) {
-      tri_scatter(sink, vg, a, dFa, 0.0f, 0.0f, 0.0f);
-      tri_scatter(sink, vg, b, dFb, 0.0f, 0.0f, 0.0f);
+      // dL/dF of the three samples' corners; samples in the same cell are
+      // added in registers first (p* lies between the two depth samples)
+      float va[8], vb[8];
+      tri_corner_vals(vg, a, dFa, 0.0f, 0.0f, 0.0f, va);
+      tri_corner_vals(vg, b, dFb, 0.0f, 0.0f, 0.0f, vb);
+      const bool s_in_a = same_cell(s, a), s_in_b = !s_in_a && same_cell(s, b);
+      const bool b_in_a = same_cell(b, a);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        va[q] = s_in_a ? va[q] + vs[q] : va[q];
+        vb[q] = s_in_b ? vb[q] + vs[q] : vb[q];
+        va[q] = b_in_a ? va[q] + vb[q] : va[q];
+      }
+      if (!s_in_a && !s_in_b) sink_add_cell(sink, vg, s.i, s.j, s.k, vs);
+      sink_add_cell(sink, vg, a.i, a.j, a.k, va);
+      if (!b_in_a) sink_add_cell(sink, vg, b.i, b.j, b.k, vb);
     }
     const float tb = ta + d;
     const float pax = sgx + ta * dx, pay = sgy + ta * dy, paz = sgz + ta * dz;
@@ -2242,7 +2329,7 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 #define KF_PASS1_BATCHES 1
 #endif
 #ifndef KF_PASS1_STEPS
-#define KF_PASS1_STEPS 16
+#define KF_PASS1_STEPS 8
 #endif
 constexpr int kPass1Batches = KF_PASS1_BATCHES;
 constexpr int kPass1Steps = KF_PASS1_STEPS;  // steps per batch
@@ -2270,22 +2357,20 @@ __global__ __launch_bounds__(256) void k_raycast(
 
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
-  constexpr int kTable = MODE == kModeAdjoint ? kHash : 1;
+  constexpr int kTable = MODE == kModeAdjoint ? kCells : 1;
   __shared__ int s_key[kTable];
-  __shared__ float s_val[kTable];
+  __shared__ float s_val[kTable * 8];
   // bricks this block has marked in io.touched (a small open-addressing set)
   constexpr int kBrickSet = MODE == kModeAdjoint ? 64 : 1;
   __shared__ int s_brick[kBrickSet];
   const bool use_table = MODE == kModeAdjoint && io.gF != nullptr;
   if (use_table) {
-    for (int e = threadIdx.x; e < kTable; e += 256) {
-      s_key[e] = -1;
-      s_val[e] = 0.0f;
-    }
+    for (int e = threadIdx.x; e < kTable; e += 256) s_key[e] = -1;
+    for (int e = threadIdx.x; e < kTable * 8; e += 256) s_val[e] = 0.0f;
     if (threadIdx.x < kBrickSet) s_brick[threadIdx.x] = -1;
     __syncthreads();
   }
-  const Sink sink{io.gF, use_table ? s_key : nullptr, s_val, io.touched};
+  const Sink sink{io.gF, use_table, (lds_int*)s_key, (lds_float*)s_val, io.touched};
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
   int hit = -1;
   bool pending = false;
@@ -2320,9 +2405,16 @@ __global__ __launch_bounds__(256) void k_raycast(
   }
   if (use_table) {
     __syncthreads();
-    for (int e = threadIdx.x; e < kTable; e += 256) {
-      const int key = s_key[e];
-      if (key < 0 || s_val[e] == 0.0f) continue;
+    // entry e -> its 8 corners; thread t takes (entry, corner) pairs t, t + 256, ...
+    const kf_vol& vg = io.vg;
+    for (int f = threadIdx.x; f < kTable * 8; f += 256) {
+      const int e = f >> 3, q = f & 7;
+      const int cell = s_key[e];
+      const float v = s_val[f];
+      if (cell < 0 || v == 0.0f) continue;
+      const int kk = cell % vg.nz, c2 = cell / vg.nz;
+      const int j = c2 % vg.ny, i = c2 / vg.ny;
+      const int key = kf_store_index(vg, i + (q >> 2), j + ((q >> 1) & 1), kk + (q & 1));
       // the first thread of the block to meet a brick marks it globally
       const int b = key >> 9;
       uint32_t h = (((uint32_t)b * 2654435761u) >> 26) & (kBrickSet - 1);
@@ -2334,7 +2426,7 @@ __global__ __launch_bounds__(256) void k_raycast(
         h = (h + 1) & (kBrickSet - 1);
       }
       if (!seen) mark_brick(io.touched, b);
-      atomicAdd(&io.gF[key], s_val[e]);
+      atomicAdd(&io.gF[key], v);
     }
   }
   if (!active || MODE == kModeAdjoint || MODE == kModeRefine) return;
@@ -2398,7 +2490,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     uniform_ray(r);
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
-    const Sink sink{io.gF, nullptr, nullptr, io.touched};
+    const Sink sink{io.gF, false, nullptr, nullptr, io.touched};
     const int owner =
         wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, gn);
     if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
@@ -2436,23 +2528,42 @@ __global__ __launch_bounds__(256) void k_zero_touched(float4* __restrict__ gF,
 
 // dL/dxi: the FP64 sums of the block partials in a fixed order (thread t adds
 // blocks t, t + 256, ..., then a tree over the 256 threads).
-__global__ __launch_bounds__(256) void k_adjoint_finish(const double* __restrict__ part,
-                                                        int n_part, kf_dev_state* st) {
-  __shared__ double red[6][256];
+// Sums the block partials in a fixed order: thread t adds partials t,
+// t + 1024, ... (loaded 4 at a time, clamped, so that they are in flight
+// together), then a DPP tree per wave and the 16 wave sums in order.
+constexpr int kAdjFinishThreads = 1024;
+
+__global__ __launch_bounds__(kAdjFinishThreads) void k_adjoint_finish(
+    const double* __restrict__ part, int n_part, kf_dev_state* st) {
+  constexpr int NT = kAdjFinishThreads, NW = NT / 64;
+  __shared__ double red[NW][6];
   double v[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = threadIdx.x; b < n_part; b += 256)
+  for (int b0 = 0; b0 < n_part; b0 += 4 * NT) {
+    double t[4][6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) v[q] += part[6 * b + q];
+    for (int u = 0; u < 4; ++u) {
+      const int b = min(b0 + u * NT + (int)threadIdx.x, n_part - 1);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = v[q];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+      for (int q = 0; q < 6; ++q) t[u][q] = part[6 * b + q];
+    }
 #pragma unroll
-      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-    __syncthreads();
+    for (int u = 0; u < 4; ++u)
+      if (b0 + u * NT + (int)threadIdx.x < n_part)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[q] += t[u][q];
   }
-  if (threadIdx.x < 6) st->xi_grad[threadIdx.x] = red[threadIdx.x][0];
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double w = wave_total(v[q]);
+    if (ln == 63) red[wave][q] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double sum = red[0][threadIdx.x];
+    for (int w = 1; w < NW; ++w) sum += red[w][threadIdx.x];
+    st->xi_grad[threadIdx.x] = sum;
+  }
 }
 
 
@@ -2878,8 +2989,8 @@ hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam) {
   }
   hipError_t e = launch_raycast_mode(c, cam, kModeAdjoint);
   if (e != hipSuccess) return e;
-  k_adjoint_finish<<<1, 256, 0, c->stream>>>(c->adj_partials,
-                                             (int)(kf_adjoint_partials(c) / 6), c->state);
+  k_adjoint_finish<<<1, kAdjFinishThreads, 0, c->stream>>>(
+      c->adj_partials, (int)(kf_adjoint_partials(c) / 6), c->state);
   return hipGetLastError();
 }
 
