@@ -15,8 +15,10 @@ Runs, each on a fresh context over the same frames:
   4. FP32 with the raycast work counters on      -> algorithmic bytes
   5. FP32 with the trilinear raycast, timed      -> trilinear.value, ATE
   6. the same with stage events, then on the final state, after one untimed
-     pair each, 20 raycast + pose-only adjoint pairs and 10 raycast +
-     pose-and-TSDF adjoint pairs                 -> adjoint cost
+     pair each, 20 raycast + pose-only adjoint pairs, 10 raycast +
+     pose-and-TSDF adjoint pairs (the adjoint from the raycast's saved
+     crossings) and 10 such pairs with the depth uploaded again in between
+     (the adjoint walks every ray)               -> adjoint cost
 and the CPU oracle on a bounded sample of the same workload.
 
 Prints one JSON line.  The roofline object is for the raycast stage (the
@@ -91,20 +93,25 @@ def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, 
         poses = ctx.poses(warmup + 1 + steps)
         if adjoint_reps:
             T = ctx.pose()
+            d = ctx.depth()
             ones = np.ones((H, W), np.float32)
             work = {}
-            for full in (False, True):
+            for key in ("pose_only", "full", "full_walk"):
                 # one untimed pair first, then the timed pairs (a full adjoint
-                # also copies dL/dTSDF to the host, outside the timed stage)
-                ctx.raycast(K, T)
-                ctx.raycast_adjoint(K, T, ones, want_tsdf=full)
-                ctx.reset_stage_times()
-                for _ in range(adjoint_reps if not full else max(1, adjoint_reps // 2)):
+                # also copies dL/dTSDF to the host, outside the timed stage);
+                # full_walk uploads the same depth again between the two,
+                # which drops the raycast's saved crossings
+                full = key != "pose_only"
+                for rep in range(1 + (adjoint_reps if not full else max(1, adjoint_reps // 2))):
+                    if rep == 1:
+                        ctx.reset_stage_times()
                     ctx.raycast(K, T)
+                    if key == "full_walk":
+                        ctx.upload_depth_f32(d)
                     ctx.raycast_adjoint(K, T, ones, want_tsdf=full)
                 adj = ctx.stage_times()
-                key = "full" if full else "pose_only"
-                work["raycast"] = adj["raycast"][0] / max(adj["raycast"][1], 1)
+                if key != "full_walk":
+                    work["raycast"] = adj["raycast"][0] / max(adj["raycast"][1], 1)
                 work[key] = adj["raycast_adjoint"][0] / max(adj["raycast_adjoint"][1], 1)
     finally:
         ctx.close()
@@ -255,7 +262,12 @@ def main():
             "adjoint_pose_only_ms": round(adj["pose_only"], 4),
             "adjoint_pose_and_tsdf_ms": round(adj["full"], 4),
             "adjoint_vs_forward": round(adj["full"] / adj["raycast"], 3),
-            "adjoint_pose_only_vs_forward": round(adj["pose_only"] / adj["raycast"], 3)},
+            "adjoint_pose_only_vs_forward": round(adj["pose_only"] / adj["raycast"], 3),
+            "adjoint_note": "the adjoint right after the raycast at the same inputs starts from "
+                            "the raycast's saved per-pixel crossings; with any input changed "
+                            "in between it walks every ray again (walk figures)",
+            "adjoint_walk_pose_and_tsdf_ms": round(adj["full_walk"], 4),
+            "adjoint_walk_vs_forward": round(adj["full_walk"] / adj["raycast"], 3)},
         "roofline": roofline,
         "stage_roofline": other,
     }

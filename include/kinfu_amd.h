@@ -212,7 +212,10 @@ int kf_download_volume_box(kf_context* ctx, const int32_t origin[3], const int32
  * bytes must equal that size; kf_volume_device_ptr reports it. */
 int kf_download_volume_raw(kf_context* ctx, void* out, size_t bytes);
 int kf_upload_volume_raw(kf_context* ctx, const void* in, size_t bytes);
-/* Device address of the native volume (for zero-copy consumers). */
+/* Device address of the native volume (for zero-copy consumers).  The call
+ * drops a trilinear raycast's saved crossings (see kf_raycast_adjoint):
+ * after writing the volume through the address, call it again before the
+ * next adjoint. */
 int kf_volume_device_ptr(kf_context* ctx, void** ptr, size_t* bytes);
 
 /* ---- A9-A10: voxel-traversal raycast (voxel_traversal.cu:4-544) ---------
@@ -247,10 +250,14 @@ int kf_download_rendered_depth(kf_context* ctx, float* depth_mm);
  * interleaved, world frame], returns dL/dTSDF in the reference layout
  * [Nx*Ny*Nz] (nullable) and dL/dxi (nullable) for the left perturbation
  * T(xi) = exp([omega, v]^) T, xi = (omega_x, omega_y, omega_z, v_x, v_y,
- * v_z) in rad and mm.  The walk and the refinement are recomputed per ray;
- * nothing is stored by the forward pass.  dL/dTSDF is accumulated with
- * float atomics, so its low bits depend on the order of arrival; with
- * dl_dtsdf NULL it is not computed at all (pose-only adjoint). */
+ * v_z) in rad and mm.  When the last call that changed the context was a
+ * kf_raycast in KF_RAYCAST_TRILINEAR mode with the same K and T (bitwise),
+ * the adjoint starts from the crossing that raycast saved per pixel (the
+ * voxel-entry parameter, in the unused w lane of its normal map) and skips
+ * the walk; otherwise it walks every ray again.  Both give the same sums up
+ * to the order of float additions.  dL/dTSDF is accumulated with float
+ * atomics, so its low bits depend on the order of arrival; with dl_dtsdf
+ * NULL it is not computed at all (pose-only adjoint). */
 int kf_raycast_adjoint(kf_context* ctx, const float K[9], const float T[16],
                        const float* dl_ddepth, const float* dl_dnormal, float* dl_dtsdf,
                        double dl_dxi[6]);

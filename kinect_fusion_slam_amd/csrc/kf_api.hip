@@ -548,6 +548,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
 
 int kf_set_params(kf_context* c, const kf_params* prm) {
   KF_REQUIRE(c && prm, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   return apply_params(c, prm);
 }
@@ -570,6 +571,7 @@ int kf_set_pose(kf_context* c, const float T[16]) {
 
 int kf_reset(kf_context* c) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  c->saved_ok = false;
   KF_LAUNCH(kf_launch_volume_fill(c));
   const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   return kf_set_pose(c, I);
@@ -599,6 +601,7 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
 
 int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) {
   KF_REQUIRE(c && depth, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   kf_stage_begin(c, KF_STAGE_DEPTH);
   const uint16_t* src = depth;
   if (mem_kind == KF_MEM_HOST) {
@@ -614,6 +617,7 @@ int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) 
 
 int kf_upload_depth_f32(kf_context* c, const float* depth_mm, int32_t mem_kind) {
   KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   KF_HIP_CHECK(hipMemcpyAsync(c->depth, depth_mm, (size_t)c->H * c->W * 4,
                               mem_kind == KF_MEM_HOST ? hipMemcpyHostToDevice
                                                       : hipMemcpyDeviceToDevice,
@@ -647,6 +651,7 @@ int kf_download_depth(kf_context* c, float* depth_mm) {
 
 int kf_bilateral(kf_context* c) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
+  c->saved_ok = false;
   KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
   kf_stage_begin(c, KF_STAGE_BILATERAL);
   KF_LAUNCH(kf_launch_bilateral(c));
@@ -658,6 +663,7 @@ int kf_bilateral(kf_context* c) {
 
 int kf_vertex_normals(kf_context* c, const float K[9]) {
   KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
   kf_cam cam;
   kf_make_cam(K, &cam);
@@ -692,6 +698,7 @@ int kf_upload_vertex_maps(kf_context* c, const float* vx, const float* vy, const
                           const float* nx, const float* ny, const float* nz,
                           const uint8_t* mask) {
   KF_REQUIRE(c && vx && vy && vz && nx && ny && nz && mask, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   const size_t n = (size_t)c->H * c->W;
   std::vector<float4> V(n), N(n);
   for (size_t i = 0; i < n; ++i) {
@@ -711,6 +718,7 @@ int kf_upload_vertex_maps(kf_context* c, const float* vx, const float* vy, const
 
 int kf_estimate_pose(kf_context* c, const float K[9], float T_inout[16]) {
   KF_REQUIRE(c && K && T_inout, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   int rc = kf_set_pose(c, T_inout);
   if (rc != KF_OK) return rc;
   kf_cam cam;
@@ -761,6 +769,7 @@ static int integrate_at_current_pose(kf_context* c, const kf_cam& cam, int first
 
 int kf_integrate(kf_context* c, const float K[9], const float T[16], int32_t first_frame) {
   KF_REQUIRE(c && K && T, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
   int rc = kf_set_pose(c, T);
   if (rc != KF_OK) return rc;
@@ -857,6 +866,7 @@ int kf_download_volume_box(kf_context* c, const int32_t origin[3], const int32_t
 
 int kf_upload_volume(kf_context* c, const float* tsdf, const float* weight) {
   KF_REQUIRE(c && tsdf && weight, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   const size_t ns = (size_t)c->n_store;
   if (c->vol.precision == KF_TSDF_F32) {
@@ -889,6 +899,7 @@ int kf_download_volume_raw(kf_context* c, void* out, size_t bytes) {
 
 int kf_upload_volume_raw(kf_context* c, const void* in, size_t bytes) {
   KF_REQUIRE(c && in && bytes == c->volume_bytes, KF_ERR_INVALID, "size mismatch");
+  c->saved_ok = false;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   KF_HIP_CHECK(hipMemcpy(c->volume, in, bytes, hipMemcpyHostToDevice));
   return KF_OK;
@@ -896,6 +907,7 @@ int kf_upload_volume_raw(kf_context* c, const void* in, size_t bytes) {
 
 int kf_volume_device_ptr(kf_context* c, void** ptr, size_t* bytes) {
   KF_REQUIRE(c && ptr, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   *ptr = c->volume;
   if (bytes) *bytes = c->volume_bytes;
   return KF_OK;
@@ -917,7 +929,14 @@ int kf_raycast(kf_context* c, const float K[9], const float T[16]) {
   if (rc != KF_OK) return rc;
   kf_cam cam;
   kf_make_cam(K, &cam);
-  return raycast_at_current_pose(c, cam);
+  c->saved_ok = false;
+  rc = raycast_at_current_pose(c, cam);
+  if (rc == KF_OK && c->prm.raycast_mode == KF_RAYCAST_TRILINEAR) {
+    memcpy(c->saved_K, K, sizeof(c->saved_K));
+    memcpy(c->saved_T, T, sizeof(c->saved_T));
+    c->saved_ok = true;
+  }
+  return rc;
 }
 
 int kf_download_model_maps(kf_context* c, float* x, float* y, float* z, float* nx, float* ny,
@@ -944,6 +963,7 @@ int kf_download_model_maps(kf_context* c, float* x, float* y, float* z, float* n
 int kf_refine_pose(kf_context* c, const float K[9], int32_t steps, float T_inout[16]) {
   KF_REQUIRE(c && K && steps >= 0 && steps <= KF_MAX_REFINE_STEPS, KF_ERR_INVALID,
              "bad argument");
+  c->saved_ok = false;
   KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
   if (T_inout) {
     int rc = kf_set_pose(c, T_inout);
@@ -1029,8 +1049,10 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
   }
   kf_cam cam;
   kf_make_cam(K, &cam);
+  const bool saved = c->saved_ok && memcmp(K, c->saved_K, sizeof(c->saved_K)) == 0 &&
+                     memcmp(T, c->saved_T, sizeof(c->saved_T)) == 0;
   kf_stage_begin(c, KF_STAGE_RAYCAST_ADJOINT);
-  KF_LAUNCH(kf_launch_raycast_adjoint(c, cam));
+  KF_LAUNCH(kf_launch_raycast_adjoint(c, cam, saved));
   kf_stage_end(c, KF_STAGE_RAYCAST_ADJOINT);
   double xi[6];
   KF_HIP_CHECK(hipMemcpyAsync(xi, c->state->xi_grad, sizeof(xi), hipMemcpyDeviceToHost,
@@ -1049,6 +1071,7 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
 int kf_upload_model_maps(kf_context* c, const float* x, const float* y, const float* z,
                          const float* nx, const float* ny, const float* nz) {
   KF_REQUIRE(c && x && y && z && nx && ny && nz, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   const size_t n = (size_t)c->H * c->W;
   std::vector<float4> V(n), N(n);
   for (size_t i = 0; i < n; ++i) {
@@ -1066,6 +1089,7 @@ int kf_upload_model_maps(kf_context* c, const float* x, const float* y, const fl
 int kf_stream_upload_u16(kf_context* c, const uint16_t* frames, int32_t n_frames,
                          int32_t mem_kind) {
   KF_REQUIRE(c && frames && n_frames > 0, KF_ERR_INVALID, "bad argument");
+  c->saved_ok = false;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   if (c->frames) (void)hipFree(c->frames);
   if (c->poses) (void)hipFree(c->poses);
@@ -1115,6 +1139,7 @@ static int record_pose(kf_context* c, int32_t frame) {
 
 int kf_pipeline_initial(kf_context* c, const float K[9], int32_t frame) {
   KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   int rc = load_stream_frame(c, frame);
   if (rc != KF_OK) return rc;
   kf_cam cam;
@@ -1126,6 +1151,7 @@ int kf_pipeline_initial(kf_context* c, const float K[9], int32_t frame) {
 
 int kf_pipeline_incoming(kf_context* c, const float K[9], int32_t frame) {
   KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
+  c->saved_ok = false;
   int rc = load_stream_frame(c, frame);
   if (rc != KF_OK) return rc;
   kf_cam cam;
@@ -1149,6 +1175,7 @@ int kf_pipeline_incoming(kf_context* c, const float K[9], int32_t frame) {
 
 int kf_pipeline_run(kf_context* c, const float K[9], int32_t first, int32_t count) {
   KF_REQUIRE(c && K && count > 0, KF_ERR_INVALID, "bad argument");
+  c->saved_ok = false;
   int rc = kf_pipeline_initial(c, K, first);
   for (int32_t f = first + 1; rc == KF_OK && f < first + count; ++f)
     rc = kf_pipeline_incoming(c, K, f);

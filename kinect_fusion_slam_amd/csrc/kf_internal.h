@@ -156,6 +156,12 @@ struct kf_context {
   float4* adj_normal = nullptr; // dL/d(normal), H*W
   bool adj_normal_set = false;
   bool adj_want_tsdf = false;   // the current adjoint call returns dL/dTSDF
+  // The model maps hold a trilinear raycast's crossings (model_n.w = te where
+  // hit >= 0) made at saved_K / saved_T with the current depth, volume and
+  // parameters: every call that changes one of those clears saved_ok.
+  bool saved_ok = false;
+  float saved_K[9] = {};
+  float saved_T[16] = {};
   double* adj_partials = nullptr; // dL/dxi block partials
   float* gn_partials = nullptr;   // refinement block partials
   int refine_steps_done = 0;      // steps of the last refinement
@@ -180,7 +186,9 @@ hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first);
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam);
-hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam);
+// saved: start from the crossings the last (trilinear) raycast saved, which
+// was made at the same pose, camera, depth and volume (kf_context::saved_ok).
+hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam, bool saved);
 size_t kf_adjoint_partials(const kf_context* c);  // doubles
 size_t kf_refine_partials(const kf_context* c);   // floats
 hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps);

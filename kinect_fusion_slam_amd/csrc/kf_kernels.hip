@@ -1491,6 +1491,18 @@ constexpr int kModeAdjoint = 2;
 constexpr int kModeRefine = 3;
 constexpr int kGnSums = 29;  // 21 J^TJ (upper, row-major), 6 J^Tr, count, r^2
 
+// World-frame unit direction of the ray through the camera-frame point
+// (lamx, lamy, lamz) = K^-1 (x + 1, y + 1, 1) (voxel_traversal.cu:90-200).
+__device__ __forceinline__ void ray_direction(float lamx, float lamy, float lamz, const float* T,
+                                              float& dx, float& dy, float& dz) {
+  const float camx = T[3], camy = T[7], camz = T[11];
+  const float rgx = T[0] * lamx + T[1] * lamy + T[2] * lamz + T[3] * 1.f;
+  const float rgy = T[4] * lamx + T[5] * lamy + T[6] * lamz + T[7] * 1.f;
+  const float rgz = T[8] * lamx + T[9] * lamy + T[10] * lamz + T[11] * 1.f;
+  dx = rgx - camx, dy = rgy - camy, dz = rgz - camz;
+  normalize3(dx, dy, dz);
+}
+
 // Ray setup (voxel_traversal.cu:90-200): ray skipping to (depth - trunc),
 // slab test, 1-based start voxel, steps and boundary distances.  Returns
 // false when the ray never enters the traversal.
@@ -1509,12 +1521,8 @@ __device__ __forceinline__ bool ray_setup(int x, int y, float dval, const float*
   const float sgx = T[0] * slx + T[1] * sly + T[2] * slz + T[3] * 1.f;
   const float sgy = T[4] * slx + T[5] * sly + T[6] * slz + T[7] * 1.f;
   const float sgz = T[8] * slx + T[9] * sly + T[10] * slz + T[11] * 1.f;
-  const float camx = T[3], camy = T[7], camz = T[11];
-  const float rgx = T[0] * lamx + T[1] * lamy + T[2] * lamz + T[3] * 1.f;
-  const float rgy = T[4] * lamx + T[5] * lamy + T[6] * lamz + T[7] * 1.f;
-  const float rgz = T[8] * lamx + T[9] * lamy + T[10] * lamz + T[11] * 1.f;
-  float dx = rgx - camx, dy = rgy - camy, dz = rgz - camz;
-  normalize3(dx, dy, dz);
+  float dx, dy, dz;
+  ray_direction(lamx, lamy, lamz, T, dx, dy, dz);
   const float mnx = (float)vg.sx * d, mny = (float)vg.sy * d, mnz = (float)vg.sz * d;
   const float mxx = (float)(vg.sx + Nx) * d, mxy = (float)(vg.sy + Ny) * d,
               mxz = (float)(vg.sz + Nz) * d;
@@ -1803,8 +1811,10 @@ __device__ __forceinline__ bool same_cell(const Tri& a, const Tri& b) {
 }
 
 // Trilinear hit of a ray whose walk crossed at a voxel entered at parameter
-// te.  MODE 1 writes vertex (w = depth) and normal; MODE 2 adds the adjoint
-// contributions of this pixel (dL/dF scattered, dL/dxi into xi).
+// te.  MODE 1 writes vertex (w = depth) and normal (w = te); MODE 2 adds the
+// adjoint contributions of this pixel (dL/dF scattered, dL/dxi into xi).
+// Nothing else of the walk enters, so an adjoint at the inputs of a MODE 1
+// raycast can start from its saved te (k_adjoint_saved).
 template <int PREC, int MODE>
 __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io, const float* T,
                                      int pix, float te, float dx, float dy, float dz,
@@ -1854,7 +1864,7 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
   const float nx = s.g[0] / gn, ny = s.g[1] / gn, nz = s.g[2] / gn;
   if constexpr (MODE == kModeTrilinear) {
     ov = make_float4(psx, psy, psz, sc + ts / lam);
-    on = make_float4(nx, ny, nz, 0.0f);
+    on = make_float4(nx, ny, nz, te);  // te: what the adjoint needs of the walk
   } else if constexpr (MODE == kModeRefine) {
     const float r = (sc + ts / lam) - dval;
     if (!(fabsf(r) <= io.max_res)) return true;
@@ -2334,19 +2344,64 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 constexpr int kPass1Batches = KF_PASS1_BATCHES;
 constexpr int kPass1Steps = KF_PASS1_STEPS;  // steps per batch
 
+// Block number that gives each XCD a contiguous run of the n_blocks tiles
+// (neighbouring tiles read neighbouring bricks, and each XCD has its own L2):
+// XCD x receives hardware blocks x, x+8, ...: q+1 of them for x < r, q
+// otherwise.
+__device__ __forceinline__ int xcd_contiguous(int bid, int n_blocks) {
+  const int q = n_blocks >> 3, r = n_blocks & 7;
+  const int xcd = bid & 7, loc = bid >> 3;
+  const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return base + loc;
+}
+
+// Per-block dL/dF table of the adjoint: kCells cell keys (-1 free) with 8
+// corner sums each, and the bricks the block has marked in io.touched (a
+// small open-addressing set, -1 free).
+constexpr int kBrickSet = 64;
+
+__device__ __forceinline__ void table_clear(int* key, float* val, int* brick) {
+  for (int e = threadIdx.x; e < kCells; e += 256) key[e] = -1;
+  for (int e = threadIdx.x; e < kCells * 8; e += 256) val[e] = 0.0f;
+  if (threadIdx.x < kBrickSet) brick[threadIdx.x] = -1;
+  __syncthreads();
+}
+
+// Adds the table to io.gF once every wave of the block is done with it:
+// thread t of the 256 takes the (entry, corner) pairs t, t + 256, ...; the first
+// thread of the block to meet a brick marks it in io.touched.
+__device__ __forceinline__ void table_flush(const int* key, const float* val, int* brick,
+                                            const RayIO& io) {
+  __syncthreads();
+  const kf_vol& vg = io.vg;
+  for (int f = threadIdx.x; f < kCells * 8; f += 256) {
+    const int e = f >> 3, q = f & 7;
+    const int cell = key[e];
+    const float v = val[f];
+    if (cell < 0 || v == 0.0f) continue;
+    const int kk = cell % vg.nz, c2 = cell / vg.nz;
+    const int j = c2 % vg.ny, i = c2 / vg.ny;
+    const int idx = kf_store_index(vg, i + (q >> 2), j + ((q >> 1) & 1), kk + (q & 1));
+    const int b = idx >> 9;
+    uint32_t h = (((uint32_t)b * 2654435761u) >> 26) & (kBrickSet - 1);
+    bool seen = false, placed = false;
+    for (int probe = 0; probe < kBrickSet && !seen && !placed; ++probe) {
+      const int old = atomicCAS(&brick[h], -1, b);
+      seen = old == b;
+      placed = old == -1;
+      h = (h + 1) & (kBrickSet - 1);
+    }
+    if (!seen) mark_brick(io.touched, b);
+    atomicAdd(&io.gF[idx], v);
+  }
+}
+
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast(
     const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
     float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
     RayState* __restrict__ queue, int tiles_x, int n_blocks) {
-  int bid = blockIdx.x;
-  {
-    // XCD x receives blocks x, x+8, ...: q+1 of them for x < r, q otherwise.
-    const int q = n_blocks >> 3, r = n_blocks & 7;
-    const int xcd = bid & 7, loc = bid >> 3;
-    const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-    bid = base + loc;
-  }
+  const int bid = xcd_contiguous(blockIdx.x, n_blocks);
   const int W = io.W;
   const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
   const int bx = bid % tiles_x, by = bid / tiles_x;
@@ -2360,15 +2415,11 @@ __global__ __launch_bounds__(256) void k_raycast(
   constexpr int kTable = MODE == kModeAdjoint ? kCells : 1;
   __shared__ int s_key[kTable];
   __shared__ float s_val[kTable * 8];
-  // bricks this block has marked in io.touched (a small open-addressing set)
-  constexpr int kBrickSet = MODE == kModeAdjoint ? 64 : 1;
-  __shared__ int s_brick[kBrickSet];
+  constexpr int kBricks = MODE == kModeAdjoint ? kBrickSet : 1;
+  __shared__ int s_brick[kBricks];
   const bool use_table = MODE == kModeAdjoint && io.gF != nullptr;
-  if (use_table) {
-    for (int e = threadIdx.x; e < kTable; e += 256) s_key[e] = -1;
-    for (int e = threadIdx.x; e < kTable * 8; e += 256) s_val[e] = 0.0f;
-    if (threadIdx.x < kBrickSet) s_brick[threadIdx.x] = -1;
-    __syncthreads();
+  if constexpr (MODE == kModeAdjoint) {
+    if (use_table) table_clear(s_key, s_val, s_brick);
   }
   const Sink sink{io.gF, use_table, (lds_int*)s_key, (lds_float*)s_val, io.touched};
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -2403,31 +2454,8 @@ __global__ __launch_bounds__(256) void k_raycast(
     if (pending)
       queue[(size_t)part * io.qcap + base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
-  if (use_table) {
-    __syncthreads();
-    // entry e -> its 8 corners; thread t takes (entry, corner) pairs t, t + 256, ...
-    const kf_vol& vg = io.vg;
-    for (int f = threadIdx.x; f < kTable * 8; f += 256) {
-      const int e = f >> 3, q = f & 7;
-      const int cell = s_key[e];
-      const float v = s_val[f];
-      if (cell < 0 || v == 0.0f) continue;
-      const int kk = cell % vg.nz, c2 = cell / vg.nz;
-      const int j = c2 % vg.ny, i = c2 / vg.ny;
-      const int key = kf_store_index(vg, i + (q >> 2), j + ((q >> 1) & 1), kk + (q & 1));
-      // the first thread of the block to meet a brick marks it globally
-      const int b = key >> 9;
-      uint32_t h = (((uint32_t)b * 2654435761u) >> 26) & (kBrickSet - 1);
-      bool seen = false, placed = false;
-      for (int probe = 0; probe < kBrickSet && !seen && !placed; ++probe) {
-        const int old = atomicCAS(&s_brick[h], -1, b);
-        seen = old == b;
-        placed = old == -1;
-        h = (h + 1) & (kBrickSet - 1);
-      }
-      if (!seen) mark_brick(io.touched, b);
-      atomicAdd(&io.gF[key], v);
-    }
+  if constexpr (MODE == kModeAdjoint) {
+    if (use_table) table_flush(s_key, s_val, s_brick, io);
   }
   if (!active || MODE == kModeAdjoint || MODE == kModeRefine) return;
   out_v[p] = ov;
@@ -2505,6 +2533,53 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * (io.part_base + blockIdx.x));
   if constexpr (MODE == kModeRefine)
     block_gn(gn, io.gn_part + kGnSums * (io.part_base + blockIdx.x));
+}
+
+// Adjoint from the saved crossings of a trilinear raycast at the same pose,
+// camera, depth and volume: the walk is deterministic in those inputs, so
+// each pixel whose raycast hit (hit index >= 0) repeats tri_hit at the te
+// that raycast saved in model_n.w, with the same sums as the walking adjoint
+// (k_raycast / k_raycast_rest in kModeAdjoint) and no walk.  Same tiles, XCD
+// order, dL/dF table and per-block dL/dxi slot as the first pass.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_adjoint_saved(
+    const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
+    const float4* __restrict__ model_n, const int32_t* __restrict__ hit_map, int tiles_x,
+    int n_blocks) {
+  const int bid = xcd_contiguous(blockIdx.x, n_blocks);
+  const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  const int bx = bid % tiles_x, by = bid / tiles_x;
+  const int x = bx * 16 + (tile & 1) * 8 + (ln & 7);
+  const int y = by * 16 + (tile >> 1) * 8 + (ln >> 3);
+  const bool active = x < io.W && y < H;
+  const int p = y * io.W + x;
+  VolRead<PREC> F;
+  F.v = reinterpret_cast<decltype(F.v)>(volume);
+  __shared__ int s_key[kCells];
+  __shared__ float s_val[kCells * 8];
+  __shared__ int s_brick[kBrickSet];
+  const bool use_table = io.gF != nullptr;
+  if (use_table) table_clear(s_key, s_val, s_brick);
+  const Sink sink{io.gF, use_table, (lds_int*)s_key, (lds_float*)s_val, io.touched};
+  double xi[6] = {0, 0, 0, 0, 0, 0};
+  float T[12];
+  load_uniform<12>(st->T, T);
+  const int h = active ? hit_map[p] : -1;
+  const float te = active ? model_n[p].w : 0.0f;
+  if (h >= 0) {
+    const float* Ki = io.Ki;
+    const float X = (float)x + 1.0f, Y = (float)y + 1.0f;
+    const float lamx = Ki[0] * X + Ki[1] * Y + Ki[2] * 1.f;
+    const float lamy = Ki[3] * X + Ki[4] * Y + Ki[5] * 1.f;
+    const float lamz = Ki[6] * X + Ki[7] * Y + Ki[8] * 1.f;
+    float dx, dy, dz;
+    ray_direction(lamx, lamy, lamz, T, dx, dy, dz);
+    float4 ov, on;
+    float gn[1] = {};
+    tri_hit<PREC, kModeAdjoint>(F, io, T, p, te, dx, dy, dz, ov, on, xi, sink, gn);
+  }
+  block_xi(xi, io.xi_part + 6 * blockIdx.x);
+  if (use_table) table_flush(s_key, s_val, s_brick, io);
 }
 
 // Zeroes dL/dF in the bricks the previous adjoint wrote and clears their
@@ -2981,16 +3056,28 @@ size_t kf_adjoint_partials(const kf_context* c) {
   return (size_t)(((c->W + 15) / 16) * ((c->H + 15) / 16) + kRestBlocks) * 6;
 }
 
-hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam) {
+hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam, bool saved) {
   if (c->adj_want_tsdf) {
     const int n_words = (c->n_bricks + 31) / 32;
     k_zero_touched<<<(unsigned)((n_words + 3) / 4), 256, 0, c->stream>>>(
         reinterpret_cast<float4*>(c->grad_tsdf), c->adj_touched, n_words);
   }
-  hipError_t e = launch_raycast_mode(c, cam, kModeAdjoint);
-  if (e != hipSuccess) return e;
-  k_adjoint_finish<<<1, kAdjFinishThreads, 0, c->stream>>>(
-      c->adj_partials, (int)(kf_adjoint_partials(c) / 6), c->state);
+  int n_part = (int)(kf_adjoint_partials(c) / 6);
+  if (saved) {
+    const int tiles_x = (c->W + 15) / 16, nb = tiles_x * ((c->H + 15) / 16);
+    const RayIO io = make_ray_io(c, cam);
+    if (c->vol.precision == KF_TSDF_F32)
+      k_adjoint_saved<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
+                                                               c->model_n, c->hit, tiles_x, nb);
+    else
+      k_adjoint_saved<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
+                                                               c->model_n, c->hit, tiles_x, nb);
+    n_part = nb;
+  } else {
+    hipError_t e = launch_raycast_mode(c, cam, kModeAdjoint);
+    if (e != hipSuccess) return e;
+  }
+  k_adjoint_finish<<<1, kAdjFinishThreads, 0, c->stream>>>(c->adj_partials, n_part, c->state);
   return hipGetLastError();
 }
 

@@ -2,7 +2,8 @@
 
 forward(tsdf, xi) renders the camera depth map at T = exp(xi^) T0 through
 the C ABI (kf_raycast in KF_RAYCAST_TRILINEAR mode); backward calls
-kf_raycast_adjoint, which recomputes every ray and returns dL/dTSDF and
+kf_raycast_adjoint, which walks every ray again (the inputs are uploaded
+again first, which drops the forward's saved crossings) and returns dL/dTSDF and
 dL/dxi.  dL/dxi is the left-perturbation gradient at T, i.e. the gradient
 with respect to xi at xi = 0.  torch only carries the arrays here; every
 computation runs in libkinfu_amd.so.

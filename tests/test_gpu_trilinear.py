@@ -3,7 +3,10 @@
 * forward (KF_RAYCAST_TRILINEAR): vertex, normal, depth and hit index
   bit-exact against the oracle's FP32 transcription (F32 and F16 volumes);
 * adjoint: dL/dxi and dL/dTSDF against the oracle's FP64 adjoint of the same
-  inputs, to 1e-4 relative (FP32 arithmetic and float atomics on the GPU);
+  inputs, to 1e-4 relative (FP32 arithmetic and float atomics on the GPU),
+  from a raycast's saved crossings and by walking the rays again; the two
+  against each other on the room workload, and a changed volume dropping the
+  saved crossings;
 * the 128^3 three-sphere workload at 320x240 with 16 poses jittered 2 cm /
   2 deg: gradients of an L2 depth loss through a torch.autograd.Function
   against FP64 central finite differences (h = 1e-4) of the oracle, for the
@@ -68,7 +71,11 @@ def _small_scene():
     return vol, trunc, K, T0.astype(np.float32), 48, 64
 
 
-def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle):
+@pytest.mark.parametrize("path", ["saved", "walk"])
+def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle, path):
+    """saved: the adjoint right after the raycast at the same inputs, which
+    starts from the raycast's saved crossings; walk: the depth uploaded again
+    in between (same values), which drops them, so every ray is walked."""
     geo, trunc, K, T0, H, W = _small_scene()
     vol, prm, _, _, _ = kf.preset(kf.PRESET_SPHERES_128)
     vol.dims[:] = list(geo["dims"])
@@ -90,15 +97,74 @@ def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle):
     am, bm = np.where(same, a, 0), np.where(same[..., None], b, 0)
     want = oracle.raycast_tri(guide, K, T0.astype(np.float64), F.astype(np.float64), vol, trunc,
                               grad_depth=am, grad_normal=bm)
-    say("[gpu] raycast adjoint, 40^3 three-sphere volume")
+    say(f"[gpu] raycast adjoint ({path}), 40^3 three-sphere volume")
     with kf.KinfuContext(vol, prm, H, W) as c:
         c.upload_volume(F, np.where(np.isnan(F), np.nan, 1.0).astype(np.float32))
         c.upload_depth_f32(guide)
         c.raycast(K, T0)
         assert_same(c.rendered_depth(), f32["depth"], "forward depth")
+        if path == "walk":
+            c.upload_depth_f32(guide)
         gF, gxi = c.raycast_adjoint(K, T0, am.astype(np.float32), bm.astype(np.float32))
     assert np.linalg.norm(gxi - want["grad_xi"]) / np.linalg.norm(want["grad_xi"]) < 1e-4
     assert np.linalg.norm(gF - want["grad_tsdf"]) / np.linalg.norm(want["grad_tsdf"]) < 1e-4
+
+
+def test_adjoint_saved_crossings_match_walk(kf, room_stream):
+    """Room workload after a 4-frame trilinear pipeline (rays finished by the
+    second raycast pass included): the adjoint from a raycast's saved
+    crossings against the walking adjoint at the same inputs, with and
+    without dL/dTSDF, to the order of float additions; and a volume changed
+    after the raycast (TSDF moved 60 mm along z, twice the window tri_hit
+    searches around a crossing) must send the adjoint back to the walk."""
+    vol, prm, K, H, W, frames, gt = room_stream
+    rng = np.random.default_rng(8)
+    a = rng.normal(size=(H, W)).astype(np.float32)
+    b = rng.normal(size=(H, W, 3)).astype(np.float32)
+    say("[gpu] adjoint from saved crossings vs walk, room 640x480")
+    with kf.KinfuContext.from_preset(kf.PRESET_ROOM, raycast_mode=kf.KF_RAYCAST_TRILINEAR,
+                                     stage_timing=1) as c:
+        c.stream_upload(frames[:4])
+        c.pipeline_run(K, 0, 4)
+        T = c.pose()
+        d = c.depth()
+
+        def adjoint(saved, want_tsdf, reps=1):
+            c.reset_stage_times()
+            for _ in range(reps):
+                if saved:
+                    c.raycast(K, T)
+                else:
+                    c.upload_depth_f32(d)
+                out = c.raycast_adjoint(K, T, a, b, want_tsdf=want_tsdf)
+            st = c.stage_times()["raycast_adjoint"]
+            return out, st[0] / st[1]
+
+        (gF_s, gxi_s), t_s = adjoint(True, True, 3)
+        (gF_w, gxi_w), t_w = adjoint(False, True, 3)
+        (_, gxi_sp), _ = adjoint(True, False)
+        (_, gxi_wp), _ = adjoint(False, False)
+        say(f"[gpu] adjoint with dL/dTSDF: saved {t_s * 1e3:.1f} us, walk {t_w * 1e3:.1f} us")
+        t, w = c.volume()
+        n = vol.dims
+        t2 = np.roll(t.reshape(n), 6, axis=2).ravel()
+        w2 = np.roll(w.reshape(n), 6, axis=2).ravel()
+        c.raycast(K, T)
+        c.upload_volume(t2, w2)
+        gF_x, gxi_x = c.raycast_adjoint(K, T, a, b)
+        c.raycast(K, T)
+        gF_y, gxi_y = c.raycast_adjoint(K, T, a, b)
+    assert np.abs(gxi_w).max() > 0 and np.abs(gF_w).max() > 0
+    assert np.linalg.norm(gxi_s - gxi_w) <= 1e-6 * np.linalg.norm(gxi_w)
+    assert np.linalg.norm(gxi_sp - gxi_wp) <= 1e-6 * np.linalg.norm(gxi_wp)
+    assert np.linalg.norm(gxi_sp - gxi_s) <= 1e-6 * np.linalg.norm(gxi_s)
+    assert np.abs(gF_s - gF_w).max() <= 1e-5 * np.abs(gF_w).max()
+    # changed volume: the walk (x) and a fresh raycast's crossings (y) agree,
+    # and differ from the old volume's sums
+    assert np.linalg.norm(gxi_x - gxi_y) <= 1e-6 * np.linalg.norm(gxi_y)
+    assert np.abs(gF_x - gF_y).max() <= 1e-5 * np.abs(gF_y).max()
+    assert np.linalg.norm(gxi_y - gxi_w) > 1e-2 * np.linalg.norm(gxi_w)
+    assert t_s < 0.5 * t_w
 
 
 # ---------------------------------------------------------------- 128^3 spheres workload
