@@ -115,6 +115,7 @@ _SIGNATURES = {
     "kf_refine_history": (C.c_int, [_P, _I32, _D, _I32, _I32]),
     "kf_raycast_adjoint": (C.c_int, [_P, _F, _F, _F, _F, _F, _D]),
     "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
+    "kf_adjoint_used_saved": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
     "kf_mat4_inverse": (C.c_int, [_F, _F]),
@@ -531,6 +532,13 @@ class KinfuContext:
                                      fptr(gF) if gF is not None else None,
                                      gxi.ctypes.data_as(_D)))
         return gF, gxi
+
+    def adjoint_used_saved(self):
+        """True if the last raycast_adjoint started from the saved crossings
+        of the raycast before it (same inputs) instead of walking the rays."""
+        v = C.c_int32()
+        check(lib.kf_adjoint_used_saved(self.h, C.byref(v)))
+        return bool(v.value)
 
     def count_raycast_work(self, enable=True):
         check(lib.kf_count_raycast_work(self.h, 1 if enable else 0))

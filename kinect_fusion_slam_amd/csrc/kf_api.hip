@@ -1051,6 +1051,7 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
   kf_make_cam(K, &cam);
   const bool saved = c->saved_ok && memcmp(K, c->saved_K, sizeof(c->saved_K)) == 0 &&
                      memcmp(T, c->saved_T, sizeof(c->saved_T)) == 0;
+  c->adj_used_saved = saved;
   kf_stage_begin(c, KF_STAGE_RAYCAST_ADJOINT);
   KF_LAUNCH(kf_launch_raycast_adjoint(c, cam, saved));
   kf_stage_end(c, KF_STAGE_RAYCAST_ADJOINT);
@@ -1065,6 +1066,12 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
   if (dl_dxi)
     for (int i = 0; i < 6; ++i) dl_dxi[i] = xi[i];
   if (dl_dtsdf) for_each_voxel(c, [&](int64_t lin, int64_t s) { dl_dtsdf[lin] = G[s]; });
+  return KF_OK;
+}
+
+int kf_adjoint_used_saved(kf_context* c, int32_t* used_saved) {
+  KF_REQUIRE(c && used_saved, KF_ERR_INVALID, "null argument");
+  *used_saved = c->adj_used_saved ? 1 : 0;
   return KF_OK;
 }
 

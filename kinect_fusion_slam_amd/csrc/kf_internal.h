@@ -162,6 +162,7 @@ struct kf_context {
   bool saved_ok = false;
   float saved_K[9] = {};
   float saved_T[16] = {};
+  bool adj_used_saved = false;  // the last adjoint started from the saved crossings
   double* adj_partials = nullptr; // dL/dxi block partials
   float* gn_partials = nullptr;   // refinement block partials
   int refine_steps_done = 0;      // steps of the last refinement

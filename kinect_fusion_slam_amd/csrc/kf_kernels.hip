@@ -1740,9 +1740,21 @@ __device__ __forceinline__ void sink_global(const Sink& k, int idx, float v, int
   atomicAdd(&k.gF[idx], v);
 }
 
-// dL/dF of the 8 corners q (index (dx << 2) | (dy << 1) | dz) of sample s:
-// a * weight_q + b . grad(weight_q).
-__device__ __forceinline__ void tri_corner_vals(const kf_vol& vg, const Tri& s, float a, float bx,
+// dL/dF of the 8 corners q (index (dx << 2) | (dy << 1) | dz) of sample s
+// through its value: a * weight_q.
+__device__ __forceinline__ void tri_corner_value(const Tri& s, float a, float (&val)[8]) {
+  const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
+  const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+    const float wx = qx ? fx : ox, wy = qy ? fy : oy, wz = qz ? fz : oz;
+    val[q] = a * (wx * wy * wz);
+  }
+}
+
+// ... and through its gradient: b . grad(weight_q).
+__device__ __forceinline__ void tri_corner_grad(const kf_vol& vg, const Tri& s, float bx,
                                                 float by, float bz, float (&val)[8]) {
   const float d = vg.d;
   const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
@@ -1752,8 +1764,7 @@ __device__ __forceinline__ void tri_corner_vals(const kf_vol& vg, const Tri& s, 
     const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
     const float wx = qx ? fx : ox, wy = qy ? fy : oy, wz = qz ? fz : oz;
     const float sx = qx ? 1.0f : -1.0f, sy = qy ? 1.0f : -1.0f, sz = qz ? 1.0f : -1.0f;
-    val[q] = a * (wx * wy * wz) +
-             (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
+    val[q] = (bx * (sx * wy * wz) + by * (wx * sy * wz) + bz * (wx * wy * sz)) / d;
   }
 }
 
@@ -1897,8 +1908,10 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
     const float lgx = (gNv.x - nx * ndot) / gn;
     const float lgy = (gNv.y - ny * ndot) / gn;
     const float lgz = (gNv.z - nz * ndot) / gn;
-    float vs[8];
-    if (sink.gF) tri_corner_vals(vg, s, 0.0f, lgx, lgy, lgz, vs);
+    // dL/dF through the normal (zero without a normal gradient)
+    const bool has_n = io.gN != nullptr;
+    float vs[8] = {};
+    if (sink.gF && has_n) tri_corner_grad(vg, s, lgx, lgy, lgz, vs);
     // dL/dp* = H lg, H the trilinear Hessian at p* (zero diagonal)
     const float fx = s.w[0], fy = s.w[1], fz = s.w[2];
     const float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
@@ -1928,9 +1941,9 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
       // dL/dF of the three samples' corners; samples in the same cell are
       // added in registers first (p* lies between the two depth samples)
       float va[8], vb[8];
-      tri_corner_vals(vg, a, dFa, 0.0f, 0.0f, 0.0f, va);
-      tri_corner_vals(vg, b, dFb, 0.0f, 0.0f, 0.0f, vb);
-      const bool s_in_a = same_cell(s, a), s_in_b = !s_in_a && same_cell(s, b);
+      tri_corner_value(a, dFa, va);
+      tri_corner_value(b, dFb, vb);
+      const bool s_in_a = has_n && same_cell(s, a), s_in_b = has_n && !s_in_a && same_cell(s, b);
       const bool b_in_a = same_cell(b, a);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -1938,7 +1951,7 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
         vb[q] = s_in_b ? vb[q] + vs[q] : vb[q];
         va[q] = b_in_a ? va[q] + vb[q] : va[q];
       }
-      if (!s_in_a && !s_in_b) sink_add_cell(sink, vg, s.i, s.j, s.k, vs);
+      if (has_n && !s_in_a && !s_in_b) sink_add_cell(sink, vg, s.i, s.j, s.k, vs);
       sink_add_cell(sink, vg, a.i, a.j, a.k, va);
       if (!b_in_a) sink_add_cell(sink, vg, b.i, b.j, b.k, vb);
     }
