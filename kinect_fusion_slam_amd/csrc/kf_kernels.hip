@@ -275,6 +275,19 @@ __global__ __launch_bounds__(256) void k_depth_u16(const uint16_t* __restrict__ 
 // window is tile[ty .. ty + 2R][tx .. tx + 2R], centre tile[ty + R][tx + R]),
 // with the reference's arithmetic (bilateral_filtering.cu:26-44).  lut_n > 0:
 // range weights from the table of whole-millimetre differences (s_lut).
+// a / b, correctly rounded, from y = 1 / b (correctly rounded): q = a * y,
+// then one fused correction q + (a - b q) y (Markstein's theorem: Markstein,
+// IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point
+// Arithmetic, ch. 4).  Exact whenever no intermediate overflows or
+// underflows, which holds for the filter's weights: 1e-9 <= a <= b <= the tap
+// count.  The filter normalises every tap by the same sum, so one division
+// per pixel replaces one per tap.
+__device__ __forceinline__ float div_by_recip(float a, float b, float y) {
+  const float q = a * y;
+  const float r = fmaf(-q, b, a);
+  return fmaf(r, y, q);
+}
+
 template <int R, int PITCH>
 __device__ __forceinline__ float bilateral_value(const float (*tile)[PITCH], int ty, int tx,
                                                  const float* sw, const float* s_lut, int lut_n,
@@ -292,9 +305,10 @@ __device__ __forceinline__ float bilateral_value(const float (*tile)[PITCH], int
       w[k] = sw[k] * t + 0.000000001f;
       sum = sum + w[k];
     }
+    const float y = 1.0f / sum;
 #pragma unroll
     for (int k = 0; k < L * L; ++k) {
-      const float wr = w[k] / sum;
+      const float wr = div_by_recip(w[k], sum, y);  // = w[k] / sum
       fin = fin + wr * tile[ty + k / L][tx + k % L];
     }
   } else if constexpr (L * L <= 49) {
@@ -307,9 +321,10 @@ __device__ __forceinline__ float bilateral_value(const float (*tile)[PITCH], int
       w[k] = sw[k] * t + 0.000000001f;
       sum = sum + w[k];
     }
+    const float y = 1.0f / sum;
 #pragma unroll
     for (int k = 0; k < L * L; ++k) {
-      const float wr = w[k] / sum;
+      const float wr = div_by_recip(w[k], sum, y);  // = w[k] / sum
       fin = fin + wr * tile[ty + k / L][tx + k % L];
     }
   } else {
