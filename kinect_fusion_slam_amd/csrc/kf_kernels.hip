@@ -1490,6 +1490,11 @@ struct RayIO {
   int part_base;       // first partial slot of this launch
   float* gn_part;      // refinement partials, kGnSums per block
   float max_res;       // refinement: residuals beyond this (mm) are ignored
+  // Refinement: a walk whose voxel-entry parameter (mm from the skip point)
+  // exceeds this can only find crossings whose residual exceeds max_res, so
+  // it stops there (+inf in the other modes).
+  float walk_tmax;
+  int refine_batches;  // refinement: first-pass batches (enough to reach walk_tmax)
   uint32_t* qcount;    // this raycast's partition append counters
   uint32_t* qnext;     // the next raycast's counters (cleared by the second pass)
   int qcap;            // rays per queue partition
@@ -2055,6 +2060,9 @@ __device__ __forceinline__ bool ray_walk(const VolRead<PREC>& F, RayState& r, co
     if (done || nv < kRayBatch) return true;
     r.cur = s;
     r.tent = te_s;
+    if constexpr (MODE == kModeRefine) {
+      if (r.tent > io.walk_tmax) return true;  // no crossing ahead can be used
+    }
   }
   return false;
 }
@@ -2302,6 +2310,9 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     r.tent = uni(L.key[kWaveSteps - 1]);
     r.prev = uni(__shfl(f, kWaveSteps - 1));
     wave_lds_sync();  // every lane has read this round's records
+    if constexpr (MODE == kModeRefine) {
+      if (r.tent > io.walk_tmax) return 0;  // no crossing ahead can be used
+    }
   }
 }
 
@@ -2463,8 +2474,9 @@ __global__ __launch_bounds__(256) void k_raycast(
   if (dval != 0) {
     if (ray_setup(x, y, dval, io.Ki, T, io.vg, io.trunc, r)) {
       r.pix = p;
-      pending = !ray_walk<PREC, kPass1Steps, COUNT, MODE>(F, r, io, T, kPass1Batches, ov, on, hit,
-                                                 visited, xi, sink, gn);
+      pending = !ray_walk<PREC, kPass1Steps, COUNT, MODE>(
+          F, r, io, T, MODE == kModeRefine ? io.refine_batches : kPass1Batches, ov, on, hit,
+          visited, xi, sink, gn);
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
@@ -2531,6 +2543,19 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   const uint32_t n = __builtin_amdgcn_readlane(incl, 63);
   // the next raycast appends to the other slot, which nothing reads now
   if (blockIdx.x == 0 && threadIdx.x < 64) io.qnext[ln * KF_RAY_COUNT_STRIDE] = 0u;
+  // a block none of whose waves receives a ray (n is the same in every wave)
+  // only writes zero partial sums
+  if constexpr (MODE == kModeAdjoint || MODE == kModeRefine) {
+    if (blockIdx.x * 4u >= n) {
+      if constexpr (MODE == kModeRefine) {
+        if (threadIdx.x < kGnSums)
+          io.gn_part[kGnSums * (io.part_base + blockIdx.x) + threadIdx.x] = 0.0f;
+      } else {
+        if (threadIdx.x < 6) io.xi_part[6 * (io.part_base + blockIdx.x) + threadIdx.x] = 0.0;
+      }
+      return;
+    }
+  }
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
   float T[12];
@@ -2674,24 +2699,26 @@ __global__ __launch_bounds__(kAdjFinishThreads) void k_adjoint_finish(
 // partials, the 6x6 solve (LDLT, FP64) and T <- exp(-x^) T in FP64, with
 // x = (omega, v) the solution of (J^T J) x = J^T r.  Records the mean squared
 // residual before the step, the pixel count and the solver status.
-__global__ __launch_bounds__(256) void k_refine_finish(const float* __restrict__ part,
-                                                       int n_part, kf_dev_state* st, int step) {
+constexpr int kRefineFinishThreads = 512;
+
+__global__ __launch_bounds__(kRefineFinishThreads) void k_refine_finish(
+    const float* __restrict__ part, int n_part, kf_dev_state* st, int step) {
   // FP64 sums of the partials in a fixed order: thread t adds blocks t,
-  // t + 256, ..., then a tree over the 256 threads
-  __shared__ double red[kGnSums][256];
+  // t + kRefineFinishThreads, ..., then a tree over the threads
+  __shared__ double red[kGnSums][kRefineFinishThreads];
   __shared__ double s[kGnSums];
   {
     double v[kGnSums];
 #pragma unroll
     for (int q = 0; q < kGnSums; ++q) v[q] = 0.0;
-    for (int b = threadIdx.x; b < n_part; b += 256)
+    for (int b = threadIdx.x; b < n_part; b += kRefineFinishThreads)
 #pragma unroll
       for (int q = 0; q < kGnSums; ++q) v[q] += (double)part[kGnSums * b + q];
 #pragma unroll
     for (int q = 0; q < kGnSums; ++q) red[q][threadIdx.x] = v[q];
   }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = kRefineFinishThreads / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w)
 #pragma unroll
       for (int q = 0; q < kGnSums; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
@@ -3012,7 +3039,46 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   io.part_base = 0;
   io.gn_part = c->gn_partials;
   io.max_res = c->prm.refine_max_residual_mm;
+  io.walk_tmax = INFINITY;
+  io.refine_batches = 1;
   return io;
+}
+
+// Walk limit of the refinement.  A crossing found at voxel-entry parameter te
+// (mm along the ray from the skip point) has t* >= te - 3d (the samples start
+// at floor(te/d) - 2 voxels), and its residual is (t* - trunc) / |lambda|
+// with |lambda| = |K^-1 (x+1, y+1, 1)|.  So beyond
+//   te > trunc + max|lambda| * max_res + 3d + margin
+// every residual exceeds max_res and the pixel is not used.  |lambda| is
+// convex over the image, so its maximum is at a corner; the margin (2d +
+// 1 mm) covers the FP32 rounding of the residual.
+static float refine_walk_tmax(const kf_context* c, const kf_cam& cam) {
+  const float* Ki = cam.Kinv;
+  double lam = 0.0;
+  for (int cy = 0; cy < 2; ++cy)
+    for (int cx = 0; cx < 2; ++cx) {
+      const double X = cx ? c->W : 1.0, Y = cy ? c->H : 1.0;
+      const double lx = Ki[0] * X + Ki[1] * Y + Ki[2];
+      const double ly = Ki[3] * X + Ki[4] * Y + Ki[5];
+      const double lz = Ki[6] * X + Ki[7] * Y + Ki[8];
+      lam = std::max(lam, std::sqrt(lx * lx + ly * ly + lz * lz));
+    }
+  const double d = c->vg.d, res = c->prm.refine_max_residual_mm;
+  if (!(res >= 0.0) || !std::isfinite(res)) return INFINITY;
+  return (float)(c->prm.truncation_mm + lam * res + 5.0 * d + 1.0);
+}
+
+// First-pass batches of the refinement: a walk stopped at walk_tmax has taken
+// at most sqrt(3) tmax / d + 3 steps (a segment of length L crosses at most
+// (|dx| + |dy| + |dz|) L / d + 3 voxel boundaries), so with that many steps
+// every ray finishes in the first pass, one lane each, and the second pass's
+// queue stays empty.  Capped at 32 batches (an unbounded walk keeps the
+// default single batch and the second pass).
+static int refine_pass1_batches(const kf_context* c, float tmax) {
+  if (!std::isfinite(tmax)) return kPass1Batches;
+  const double steps = std::sqrt(3.0) * std::max(0.0, (double)tmax) / c->vg.d + 4.0;
+  const int b = (int)std::ceil(steps / kPass1Steps);
+  return std::max(kPass1Batches, std::min(32, b));
 }
 
 // Second-pass grid: 8192 waves, one queued ray per wave at the few thousand
@@ -3051,6 +3117,10 @@ static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
   RayIO io = make_ray_io(c, cam);
+  if (mode == kModeRefine) {
+    io.walk_tmax = refine_walk_tmax(c, cam);
+    io.refine_batches = refine_pass1_batches(c, io.walk_tmax);
+  }
   io.qcount = c->ray_counts + (size_t)c->ray_slot * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
   io.qnext = c->ray_counts + (size_t)(c->ray_slot ^ 1) * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
   io.qcap = kf_ray_part_capacity(nb);
@@ -3071,7 +3141,8 @@ hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps) {
   for (int it = 0; it < steps; ++it) {
     hipError_t e = launch_raycast_mode(c, cam, kModeRefine);
     if (e != hipSuccess) return e;
-    k_refine_finish<<<1, 256, 0, c->stream>>>(c->gn_partials, n_part, c->state, it);
+    k_refine_finish<<<1, kRefineFinishThreads, 0, c->stream>>>(c->gn_partials, n_part,
+                                                                c->state, it);
   }
   return hipGetLastError();
 }
