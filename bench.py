@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # --no-cpu` (FETCH_SIZE x 2 + WRITE_SIZE x 1; the factors are what the same
 # counters read for a device-to-device copy of 256 MiB).  The passes and the
 # per-kernel figures are in profiles/raycast_room_640x480_pmc.txt.
-RAYCAST_TRAFFIC_BYTES = 26320691
+RAYCAST_TRAFFIC_BYTES = 26294477
 
 
 def ensure_built():
