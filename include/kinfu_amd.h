@@ -116,6 +116,13 @@ typedef struct kf_params {
   int32_t refine_steps;     /* pipeline: Gauss-Newton pose refinement steps
                                after ICP (0 = off, <= KF_MAX_REFINE_STEPS) */
   float refine_max_residual_mm; /* refinement ignores |residual| above this  */
+  int32_t icp_robust;       /* 0: the reference's ICP update (a failed solve
+                               applies the right-hand side); 1: an iteration
+                               whose solve fails, or that has fewer than
+                               icp_min_inliers inliers, leaves the pose
+                               unchanged (no reference counterpart)       */
+  int32_t icp_min_inliers;  /* icp_robust: inlier count below which an
+                               iteration is not applied                    */
 } kf_params;
 
 /* Raycast hit handling.  REFERENCE: the reference's per-axis interpolation
@@ -187,8 +194,11 @@ int kf_estimate_pose(kf_context* ctx, const float K[9], float T_inout[16]);
 /* Per-iteration record of the last kf_estimate_pose / pipeline frame:
  * pose after each iteration [n][16], FP64 sums [n][KF_ICP_SUMS] (order:
  * upper triangle of J^TJ row-major, J^Tr, count), solver status [n]
- * (0 = LDLT solved, 1 = not positive definite: increment = right-hand side),
- * inlier masks [n][H*W] when record_icp is set.  Any pointer may be NULL. */
+ * (0 = LDLT solved, 1 = not positive definite: increment = right-hand side,
+ * or none with icp_robust; KF_ICP_STATUS_GATED = solved but fewer than
+ * icp_min_inliers inliers, no increment), inlier masks [n][H*W] when
+ * record_icp is set.  Any pointer may be NULL. */
+#define KF_ICP_STATUS_GATED 2
 int kf_icp_history(kf_context* ctx, int32_t* n_iters, float* T_iters,
                    double* sums, int32_t* status, uint8_t* masks);
 

@@ -17,6 +17,7 @@ KF_TSDF_F32, KF_TSDF_F16 = 0, 1
 KF_MEM_HOST, KF_MEM_DEVICE = 0, 1
 KF_MAX_ICP_ITERS = 64
 KF_ICP_SUMS = 28
+KF_ICP_STATUS_GATED = 2
 STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep",
           "raycast_adjoint", "refine")
 KF_RAYCAST_REFERENCE, KF_RAYCAST_TRILINEAR = 0, 1
@@ -49,7 +50,8 @@ class Params(C.Structure):
                 ("icp_compensated", C.c_int32), ("record_icp", C.c_int32),
                 ("stage_timing", C.c_int32), ("raycast_fix_bounds", C.c_int32),
                 ("raycast_mode", C.c_int32), ("refine_steps", C.c_int32),
-                ("refine_max_residual_mm", C.c_float)]
+                ("refine_max_residual_mm", C.c_float), ("icp_robust", C.c_int32),
+                ("icp_min_inliers", C.c_int32)]
 
 
 class SynthScene(C.Structure):

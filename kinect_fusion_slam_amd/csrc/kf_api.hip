@@ -126,6 +126,10 @@ bool params_valid(const kf_params* p, std::string* why) {
     *why = "refine_steps must be in [0, 64] and refine_max_residual_mm positive";
     return false;
   }
+  if ((p->icp_robust != 0 && p->icp_robust != 1) || p->icp_min_inliers < 0) {
+    *why = "icp_robust must be 0 or 1 and icp_min_inliers non-negative";
+    return false;
+  }
   return true;
 }
 
@@ -265,6 +269,8 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   p.raycast_mode = KF_RAYCAST_REFERENCE;
   p.refine_steps = 0;
   p.refine_max_residual_mm = 50.0f;
+  p.icp_robust = 0;
+  p.icp_min_inliers = 0;
   switch (which) {
     case KF_PRESET_REFERENCE: {
       // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,

@@ -245,11 +245,31 @@ KF_HD int kf_icp_increment(const double* sums, float* inc) {
   return status;
 }
 
+// The tracking-robust increment (no reference counterpart): an iteration
+// whose solve fails (status 1), or that solved with fewer than min_inliers
+// inliers (sums[27], status 2), gets the identity increment instead.
+KF_HD int kf_icp_increment_robust(const double* sums, float* inc, double min_inliers) {
+  int status = kf_icp_increment(sums, inc);
+  if (status == 0 && sums[27] < min_inliers) status = 2;
+  if (status != 0)
+    KF_UNROLL for (int i = 0; i < 16; ++i) inc[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  return status;
+}
+
 // One ICP update: T <- inc * T (sensor_pose_estimation.cu:428).  Returns the
 // solver status.
 KF_HD int kf_icp_update(const double* sums, float* T) {
   float inc[16];
   const int status = kf_icp_increment(sums, inc);
+  kf_mat4_mul(inc, T, T);
+  return status;
+}
+
+// The same with the robust increment when min_inliers >= 0.
+KF_HD int kf_icp_update_gated(const double* sums, float* T, double min_inliers) {
+  if (!(min_inliers >= 0.0)) return kf_icp_update(sums, T);
+  float inc[16];
+  const int status = kf_icp_increment_robust(sums, inc, min_inliers);
   kf_mat4_mul(inc, T, T);
   return status;
 }

@@ -526,16 +526,19 @@ __device__ __forceinline__ void icp_wave_sync() {
 // Finishes an iteration in wave 0: the sums from the row sums (a sum and
 // its compensation row in the compensated variant), the solve (one lane),
 // T_new = inc(sums) * T0 and F = Tp * T_new (16 lanes each), then one block
-// barrier.
+// barrier.  min_inliers >= 0 selects the robust increment.
 template <int ROWS>
-__device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh) {
+__device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh, float min_inliers) {
   if (threadIdx.x < 64) {
     if (threadIdx.x < KF_ICP_SUMS) {
       const int k = threadIdx.x;
       sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? sh.rowsum[2 * k] + sh.rowsum[2 * k + 1] : sh.rowsum[k];
     }
     icp_wave_sync();
-    if (threadIdx.x == 0) sh.status = kf_icp_increment(sh.sums, sh.inc);
+    if (threadIdx.x == 0)
+      sh.status = min_inliers >= 0.0f
+                      ? kf_icp_increment_robust(sh.sums, sh.inc, (double)min_inliers)
+                      : kf_icp_increment(sh.sums, sh.inc);
     icp_wave_sync();
     if (threadIdx.x < 16)
       sh.Tn[threadIdx.x] = kf_mat4_mul_elem(sh.inc, sh.T0, threadIdx.x >> 2, threadIdx.x & 3);
@@ -563,7 +566,7 @@ template <bool KAHAN, int PX, int PPL>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const float4* __restrict__ vmap, const float4* __restrict__ nmap,
     const float4* __restrict__ mv, const float4* __restrict__ mn, kf_cam cam, int H, int W,
-    float dist_gate, float cos_lo, kf_dev_state* __restrict__ st,
+    float dist_gate, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
     const void* __restrict__ part_in_, void* __restrict__ part_out_,
     uint8_t* __restrict__ mask_out, int iter) {
   constexpr int NS = KF_ICP_SUMS;
@@ -614,7 +617,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
     }
     icp_reduce_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(part_in, nb, sh);
-    icp_update_pose<ROWS>(sh);
+    icp_update_pose<ROWS>(sh, min_inliers);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
   }
   float F[12], Te[12];
@@ -769,7 +772,7 @@ constexpr int kIcpFinishThreads = 1024;
 template <bool KAHAN, int PPL>
 __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* __restrict__ st,
                                                              const void* __restrict__ part_in,
-                                                             int nb, int last) {
+                                                             int nb, int last, float min_inliers) {
   constexpr int ROWS = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
   __shared__ IcpShared<ROWS> sh;
   if (threadIdx.x < 16) {
@@ -778,7 +781,7 @@ __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* 
   }
   icp_reduce_partials<ROWS, kIcpFinishThreads, PPL>(static_cast<const IcpPartial<KAHAN>*>(part_in),
                                                nb, sh);
-  icp_update_pose<ROWS>(sh);
+  icp_update_pose<ROWS>(sh, min_inliers);
   icp_record<ROWS>(st, last, sh);
   if (threadIdx.x < 16) {
     st->T[threadIdx.x] = sh.Tn[threadIdx.x];
@@ -2855,20 +2858,22 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
   // (compensated), per block
   const size_t half = (size_t)KF_ICP_SUMS * 8 * nb;
   char* const pbase = reinterpret_cast<char*>(c->icp_partials);
+  // robust update: the inlier threshold (>= 0); the reference's update: -1
+  const float min_inl = c->prm.icp_robust ? (float)c->prm.icp_min_inliers : -1.0f;
   for (int it = 0; it < iters; ++it) {
     uint8_t* m = (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
     const void* pin = pbase + (size_t)((it + 1) & 1) * half;
     void* pout = pbase + (size_t)(it & 1) * half;
     k_icp_iter<KAHAN, PX, PPL><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
         c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
-        c->icp_cos_lo, c->state, pin, pout, m, it);
+        c->icp_cos_lo, min_inl, c->state, pin, pout, m, it);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (iters > 0) {
     const void* pin = pbase + (size_t)((iters - 1) & 1) * half;
     k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb,
-                                                                      iters - 1);
+                                                                      iters - 1, min_inl);
   }
   return hipGetLastError();
 }

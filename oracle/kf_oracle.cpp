@@ -1021,7 +1021,7 @@ void oracle_icp(const float* vx, const float* vy, const float* vz, const float* 
                 const float* snz, int H, int W, const float* K, float* T_inout,
                 const float* T_start, int iters, float dist_gate, float angle_gate,
                 int reduce_mode, float* T_iters, double* sums_out, int32_t* status_out,
-                uint8_t* masks) {
+                uint8_t* masks, double min_inliers) {
   const IcpInputs in{vx, vy, vz, nx, ny, nz, mask, sx, sy, sz, snx, sny, snz, H, W};
   const int64_t n = (int64_t)H * W;
   const int64_t nblk = (n + 63) / 64;
@@ -1067,7 +1067,8 @@ void oracle_icp(const float* vx, const float* vy, const float* vz, const float* 
         sums[27] += 1.0;
       }
     }
-    const int st = kf_icp_update(sums, Te);
+    // min_inliers < 0: the reference's update; else the robust one
+    const int st = kf_icp_update_gated(sums, Te, min_inliers);
     if (T_iters)
       for (int i = 0; i < 16; ++i) T_iters[it * 16 + i] = Te[i];
     if (sums_out)
@@ -1128,7 +1129,7 @@ void oracle_pipeline(const uint16_t* frames, int n_frames, int H, int W, const f
       oracle_icp(vx.data(), vy.data(), depth.data(), nx.data(), ny.data(), nz.data(),
                  mask.data(), sx.data(), sy.data(), sz.data(), snx.data(), sny.data(),
                  snz.data(), H, W, K, T, nullptr, icp_iters, dist_gate, angle_gate, reduce_mode,
-                 nullptr, nullptr, nullptr, nullptr);
+                 nullptr, nullptr, nullptr, nullptr, -1.0);
       acc[3] += now() - t3;
     }
     double t4 = now();

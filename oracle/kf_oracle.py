@@ -53,7 +53,7 @@ def _load():
         "oracle_normal_flip": (C.c_int, [C.c_float]),
         "oracle_icp": (None, [_F, _F, _F, _F, _F, _F, _U8, _F, _F, _F, _F, _F, _F, C.c_int,
                               C.c_int, _F, _F, _F, C.c_int, C.c_float, C.c_float, C.c_int, _F,
-                              _D, _I32, _U8]),
+                              _D, _I32, _U8, C.c_double]),
         "oracle_pipeline": (None, [_U16, C.c_int, C.c_int, C.c_int, _F, _I, _I, C.c_float,
                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                    C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
@@ -227,8 +227,10 @@ def raycast_tri(depth, K, T, tsdf, vol, trunc, grad_depth=None, grad_normal=None
 
 
 def icp(frame, model, K, T_prev, iters, dist_gate, angle_gate, reduce_mode=0, masks=True,
-        T_start=None):
-    """frame: dict vx, vy, vz(depth), nx, ny, nz, mask; model: dict x, y, z, nx, ny, nz."""
+        T_start=None, min_inliers=None):
+    """frame: dict vx, vy, vz(depth), nx, ny, nz, mask; model: dict x, y, z, nx, ny, nz.
+    min_inliers None: the reference's update (sensor_pose_estimation.cu:389-429);
+    a number: the robust update (a failed solve or fewer inliers: no increment)."""
     H, W = frame["vz"].shape
     fr = [_c32(frame[k]) for k in ("vx", "vy", "vz", "nx", "ny", "nz")]
     m = np.ascontiguousarray(frame["mask"], np.uint8)
@@ -242,7 +244,8 @@ def icp(frame, model, K, T_prev, iters, dist_gate, angle_gate, reduce_mode=0, ma
                    _f(_c32(K, 9)), _f(T), _f(_c32(T_start, 16)) if T_start is not None else None,
                    iters, dist_gate, angle_gate, reduce_mode, _f(Ts),
                    sums.ctypes.data_as(_D), st.ctypes.data_as(_I32),
-                   mk.ctypes.data_as(_U8) if masks else None)
+                   mk.ctypes.data_as(_U8) if masks else None,
+                   -1.0 if min_inliers is None else float(min_inliers))
     return {"T": T.reshape(4, 4), "T_iters": Ts[:iters].reshape(-1, 4, 4), "sums": sums[:iters],
             "status": st[:iters], "masks": mk[:iters] if masks else None}
 

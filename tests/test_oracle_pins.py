@@ -567,3 +567,43 @@ def test_render_noise_and_holes_statistics(kf):
     sigma = 1000 * (0.0012 + 0.0019 * (2.0 - 0.4) ** 2)  # mm at z = 2 m
     assert abs(res.std() - math.sqrt(sigma ** 2 + 1 / 12)) < 0.05 * sigma
     assert abs(res.mean()) < 0.1
+
+
+def test_icp_robust_failed_solve_keeps_the_pose(kf, oracle):
+    # the rank-1 case above: with the robust update the failed solve applies
+    # no increment (status 1), and the pose stays exactly where it was
+    K = intr(4.0, 2.5, 2.5)
+    fr, model = _gate_maps(K, 4, 4, np.full((4, 4), 1000.0, np.float32), np.zeros((4, 4)))
+    model["z"][1, 2] = fr["vz"][1, 2] + 3.0
+    I4 = np.eye(4, dtype=np.float32)
+    r = oracle.icp(fr, model, K, I4, 2, 100.0, 20.0, min_inliers=0)
+    assert np.all(r["status"] == 1) and np.all(r["sums"][:, 27] == 1)
+    assert np.array_equal(r["T"], I4) and np.array_equal(r["T_iters"][0], I4)
+
+
+def test_icp_robust_inlier_gate(kf, oracle):
+    """Known rigid motion: with the gate at 0 the robust update is the
+    reference's (every solve succeeds); with the gate above the inlier count
+    no iteration is applied (status 2) and the pose is unchanged."""
+    scene = kf.preset_scene(kf.PRESET_ROOM)
+    K = intr(131.25, 79.5, 59.5)
+    H, W = 120, 160
+    I4 = np.eye(4, dtype=np.float32)
+    T1 = pose(t=(8.0, -5.0, 6.0), rx=0.004, ry=-0.006, rz=0.003)
+    d0 = kf.render(scene, K, I4, H, W, noise_scale=0.0, hole_frac=0.0).astype(np.float32)
+    d1 = kf.render(scene, K, T1, H, W, noise_scale=0.0, hole_frac=0.0).astype(np.float32)
+    m = _frame_from_depth(kf, oracle, d0, K)
+    model = {"x": m["vx"], "y": m["vy"], "z": d0, "nx": m["nx"], "ny": m["ny"], "nz": m["nz"]}
+    fr = _frame_from_depth(kf, oracle, d1, K)
+    ref = oracle.icp(fr, model, K, I4, 5, 100.0, 20.0)
+    rob = oracle.icp(fr, model, K, I4, 5, 100.0, 20.0, min_inliers=0)
+    assert np.all(ref["status"] == 0)
+    assert np.array_equal(ref["T_iters"], rob["T_iters"]) and np.array_equal(rob["status"], ref["status"])
+    n0 = int(ref["sums"][0][27])
+    gated = oracle.icp(fr, model, K, I4, 5, 100.0, 20.0, min_inliers=n0 + 1)
+    assert np.all(gated["status"] == kf.KF_ICP_STATUS_GATED)
+    assert np.array_equal(gated["T"], I4)
+    # a gate between the counts of the first and the later iterations: the
+    # first iteration is applied only if its count reaches the gate
+    part = oracle.icp(fr, model, K, I4, 5, 100.0, 20.0, min_inliers=n0)
+    assert part["status"][0] == 0 and np.array_equal(part["T_iters"][0], ref["T_iters"][0])
