@@ -279,9 +279,11 @@ __global__ __launch_bounds__(256) void k_depth_u16(const uint16_t* __restrict__ 
 // then one fused correction q + (a - b q) y (Markstein's theorem: Markstein,
 // IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point
 // Arithmetic, ch. 4).  Exact whenever no intermediate overflows or
-// underflows, which holds for the filter's weights: 1e-9 <= a <= b <= the tap
-// count.  The filter normalises every tap by the same sum, so one division
-// per pixel replaces one per tap.
+// underflows, which holds for the filter's weights (1e-9 <= a <= b <= the
+// tap count) and for the trilinear samples' quotients by the voxel size and
+// by the gradient norm (mm-scale or unit-scale values, or exact zeros, whose
+// quotient may come out +0 where division gives -0).  The filter normalises
+// every tap by the same sum, so one division per pixel replaces one per tap.
 __device__ __forceinline__ float div_by_recip(float a, float b, float y) {
   const float q = a * y;
   const float r = fmaf(-q, b, a);
@@ -1669,10 +1671,10 @@ struct Tri {
 template <int PREC>
 __device__ __forceinline__ bool tri_sample(const VolRead<PREC>& F, const kf_vol& vg, float px,
                                            float py, float pz, Tri& s) {
-  const float d = vg.d;
-  const float ux = (px - (float)vg.sx * d) / d - 0.5f;
-  const float uy = (py - (float)vg.sy * d) / d - 0.5f;
-  const float uz = (pz - (float)vg.sz * d) / d - 0.5f;
+  const float d = vg.d, rd = 1.0f / d;  // quotients by d: div_by_recip
+  const float ux = div_by_recip(px - (float)vg.sx * d, d, rd) - 0.5f;
+  const float uy = div_by_recip(py - (float)vg.sy * d, d, rd) - 0.5f;
+  const float uz = div_by_recip(pz - (float)vg.sz * d, d, rd) - 0.5f;
   const int i = kf_floor_to_int(ux), j = kf_floor_to_int(uy), k = kf_floor_to_int(uz);
   if (!(i >= 0 && i < vg.nx - 1 && j >= 0 && j < vg.ny - 1 && k >= 0 && k < vg.nz - 1))
     return false;
@@ -1694,9 +1696,9 @@ __device__ __forceinline__ bool tri_sample(const VolRead<PREC>& F, const kf_vol&
   s.v = c0 * oz + c1 * fz;
   const float e00 = s.c[4] - s.c[0], e01 = s.c[5] - s.c[1];
   const float e10 = s.c[6] - s.c[2], e11 = s.c[7] - s.c[3];
-  s.g[0] = ((e00 * oy + e10 * fy) * oz + (e01 * oy + e11 * fy) * fz) / d;
-  s.g[1] = ((c10 - c00) * oz + (c11 - c01) * fz) / d;
-  s.g[2] = (c1 - c0) / d;
+  s.g[0] = div_by_recip((e00 * oy + e10 * fy) * oz + (e01 * oy + e11 * fy) * fz, d, rd);
+  s.g[1] = div_by_recip((c10 - c00) * oz + (c11 - c01) * fz, d, rd);
+  s.g[2] = div_by_recip(c1 - c0, d, rd);
   s.w[0] = fx;
   s.w[1] = fy;
   s.w[2] = fz;
@@ -1895,7 +1897,9 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
   if (!tri_sample<PREC>(F, vg, psx, psy, psz, s)) return false;
   const float gn = sqrtf(s.g[0] * s.g[0] + s.g[1] * s.g[1] + s.g[2] * s.g[2]);
   if (!(gn > 0.0f)) return false;
-  const float nx = s.g[0] / gn, ny = s.g[1] / gn, nz = s.g[2] / gn;
+  const float rgn = 1.0f / gn;
+  const float nx = div_by_recip(s.g[0], gn, rgn), ny = div_by_recip(s.g[1], gn, rgn),
+              nz = div_by_recip(s.g[2], gn, rgn);
   if constexpr (MODE == kModeTrilinear) {
     ov = make_float4(psx, psy, psz, sc + ts / lam);
     on = make_float4(nx, ny, nz, te);  // te: what the adjoint needs of the walk
@@ -1928,9 +1932,9 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
     float dts = gD / lam;  // dL/dt* through the depth
     // normal: n = g/|g|, g = grad F(p*) (linear in the corners of p*)
     const float ndot = nx * gNv.x + ny * gNv.y + nz * gNv.z;
-    const float lgx = (gNv.x - nx * ndot) / gn;
-    const float lgy = (gNv.y - ny * ndot) / gn;
-    const float lgz = (gNv.z - nz * ndot) / gn;
+    const float lgx = div_by_recip(gNv.x - nx * ndot, gn, rgn);
+    const float lgy = div_by_recip(gNv.y - ny * ndot, gn, rgn);
+    const float lgz = div_by_recip(gNv.z - nz * ndot, gn, rgn);
     // dL/dF through the normal (zero without a normal gradient)
     const bool has_n = io.gN != nullptr;
     float vs[8] = {};
