@@ -3093,7 +3093,10 @@ static int refine_pass1_batches(const kf_context* c, float tmax) {
 // Second-pass grid: 8192 waves, one queued ray per wave at the few thousand
 // rays a 640x480 frame queues.  The adjoint and refinement modes keep one
 // partial-sum slot per block.
-constexpr int kRestBlocks = 2048;
+#ifndef KF_REST_BLOCKS
+#define KF_REST_BLOCKS 2048
+#endif
+constexpr int kRestBlocks = KF_REST_BLOCKS;
 
 template <int PREC, bool COUNT, int MODE>
 static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
