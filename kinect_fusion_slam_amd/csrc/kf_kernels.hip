@@ -1708,15 +1708,14 @@ __device__ __forceinline__ bool tri_sample(const VolRead<PREC>& F, const kf_vol&
   return true;
 }
 
-// Where the adjoint's dL/dF contributions go.  In pass 1 a block's rays hit
-// a compact set of voxels, so contributions are first summed in a per-block
-// LDS hash table (key = native voxel index) and flushed with one global
-// atomic per distinct voxel; a full probe sequence, and pass 2, fall back to
-// global atomics.
-// The first raycast pass of the adjoint sums dL/dF in an LDS table keyed by
-// trilinear cell (the lower corner's voxel): a key and the 8 corners' sums
-// per entry, so that a sample costs one lookup, and lanes sharing a cell
-// (neighbouring rays) add to its 8 slots in lane-rotated order.
+// Where the adjoint's dL/dF contributions go.  The walk's first pass and the
+// adjoint from saved crossings sum them in a per-block LDS table keyed by
+// trilinear cell (the lower corner's linear voxel index): a key and the 8
+// corners' sums per entry, so that a sample costs one lookup, and lanes
+// sharing a cell (neighbouring rays) add to its 8 slots in lane-rotated
+// order; the block then adds each touched corner to global memory once.  A
+// sample that finds no entry within kCellProbes slots, and the walk's second
+// pass, use global atomics.
 constexpr int kCellBits = 9;
 constexpr int kCells = 1 << kCellBits;
 constexpr int kCellProbes = 16;
@@ -1733,17 +1732,6 @@ struct Sink {
   lds_float* val;     // corner sums, kCells x 8
   uint32_t* touched;  // one bit per brick that received dL/dF (for zeroing)
 };
-
-__device__ __forceinline__ int lds_cas(lds_int* p, int cmp, int v) {
-  int expected = cmp;
-  __hip_atomic_compare_exchange_strong(p, &expected, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-  return expected;
-}
-
-__device__ __forceinline__ void lds_fadd(lds_float* p, float v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 // Marks brick b as holding dL/dF (a bit already seen set is not set again).
 __device__ __forceinline__ void mark_brick(uint32_t* touched, int b) {
