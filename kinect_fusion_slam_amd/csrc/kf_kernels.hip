@@ -1500,6 +1500,7 @@ struct RayIO {
   // it stops there (+inf in the other modes).
   float walk_tmax;
   int refine_batches;  // refinement: first-pass batches (enough to reach walk_tmax)
+  int pass1_batches;   // first-pass batches of the other modes (multi-batch kernels)
   uint32_t* qcount;    // this raycast's partition append counters
   uint32_t* qnext;     // the next raycast's counters (cleared by the second pass)
   int qcap;            // rays per queue partition
@@ -2430,7 +2431,7 @@ __device__ __forceinline__ void table_flush(const int* key, const float* val, in
   }
 }
 
-template <int PREC, bool COUNT, int MODE>
+template <int PREC, bool COUNT, int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_raycast(
     const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
     float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
@@ -2470,8 +2471,9 @@ __global__ __launch_bounds__(256) void k_raycast(
     if (ray_setup(x, y, dval, io.Ki, T, io.vg, io.trunc, r)) {
       r.pix = p;
       pending = !ray_walk<PREC, kPass1Steps, COUNT, MODE>(
-          F, r, io, T, MODE == kModeRefine ? io.refine_batches : kPass1Batches, ov, on, hit,
-          visited, xi, sink, gn);
+          F, r, io, T,
+          MODE == kModeRefine ? io.refine_batches : (MULTI ? io.pass1_batches : kPass1Batches),
+          ov, on, hit, visited, xi, sink, gn);
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
@@ -3019,6 +3021,19 @@ hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
   return hipGetLastError();
 }
 
+// First-pass batches: enough 8-step batches for the walk from the skip point
+// (depth - trunc) to the surface, about 1.3 trunc / d DDA steps on oblique
+// rays plus the crossing, so that few rays are left for the second pass (one
+// wave per ray).  Measured: 640x480 at trunc 30 mm / 10 mm voxels, one batch
+// (two: -2.6 % frames/s); 1280x720 at 20 mm / 4 mm, two batches (+77 %
+// frames/s of ICP-only tracking over one).  Either split walks the same
+// voxels and gives the same results.
+static int pass1_batches(const kf_context* c) {
+  const double steps = 1.3 * c->prm.truncation_mm / c->vg.d + 2.0;
+  const int b = (int)std::ceil(steps / kPass1Steps);
+  return std::max(kPass1Batches, std::min(8, b));
+}
+
 static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   RayIO io;
   io.vg = c->vg;
@@ -3038,6 +3053,7 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   io.max_res = c->prm.refine_max_residual_mm;
   io.walk_tmax = INFINITY;
   io.refine_batches = 1;
+  io.pass1_batches = pass1_batches(c);
   return io;
 }
 
@@ -3089,9 +3105,12 @@ constexpr int kRestBlocks = KF_REST_BLOCKS;
 template <int PREC, bool COUNT, int MODE>
 static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
   RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
-  k_raycast<PREC, COUNT, MODE><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
-                                                          c->model_v, c->model_n, c->hit, q,
-                                                          tiles_x, nb);
+  if (MODE != kModeRefine && io.pass1_batches > kPass1Batches)
+    k_raycast<PREC, COUNT, MODE, true><<<nb, 256, 0, c->stream>>>(
+        c->tsdf, io, c->H, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
+  else
+    k_raycast<PREC, COUNT, MODE, false><<<nb, 256, 0, c->stream>>>(
+        c->tsdf, io, c->H, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
   k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(

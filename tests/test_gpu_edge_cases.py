@@ -137,6 +137,19 @@ def test_ragged_image_and_volume(kf, oracle, precision):
     assert (model["hit"] >= 0).sum() > 500
 
 
+def test_ragged_wide_truncation_multi_batch_first_pass(kf, oracle):
+    """Truncation 8 voxels (160 mm at 20 mm): the first raycast pass runs
+    ceil((1.3 * 8 + 2) / 8) = 2 batches per ray (the multi-batch kernels), and
+    every stage stays bit-exact against the oracle."""
+    H, W = 75, 101
+    vol, prm, K = geometry(kf, H, W, (37, 41, 29), (-18, -20, 5), 20.0, 70.0)
+    prm.truncation_mm, prm.sdf_min_mm, prm.sdf_max_mm = 160.0, -160.0, 1600.0
+    frames, gt = kf.stream(small_scene(kf), K, H, W, 0, 4)
+    say("[gpu] ragged 101x75 image, 37x41x29 volume, truncation 8 voxels")
+    model = run_stages(kf, oracle, vol, prm, K, H, W, frames, gt, kf.KF_TSDF_F32)
+    assert (model["hit"] >= 0).sum() > 500
+
+
 def test_empty_and_out_of_range_frames(kf, oracle):
     H, W = 48, 64
     vol, prm, K = geometry(kf, H, W, (40, 40, 36), (-20, -20, 4), 20.0, 60.0)
