@@ -1909,12 +1909,11 @@ __device__ __forceinline__ bool tri_hit(const VolRead<PREC>& F, const RayIO& io,
     J[3] = dFa * a.g[0] + dFb * b.g[0];
     J[4] = dFa * a.g[1] + dFb * b.g[1];
     J[5] = dFa * a.g[2] + dFb * b.g[2];
-    int q = 0;
-    for (int i = 0; i < 6; ++i)
-      for (int j = i; j < 6; ++j) acc[q++] += J[i] * J[j];
-    for (int i = 0; i < 6; ++i) acc[21 + i] += J[i] * r;
-    acc[27] += 1.0f;
-    acc[28] += r * r;
+    // the pixel's record (a ray hits at most once); gn_add_pixel forms the
+    // sums' terms
+    for (int i = 0; i < 6; ++i) acc[i] = J[i];
+    acc[6] = r;
+    acc[7] = 1.0f;
   } else {
     const float gD = io.gD[pix];
     const float4 gNv = io.gN ? io.gN[pix] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -2342,6 +2341,22 @@ __device__ __forceinline__ void block_xi(const double* xi, double* __restrict__ 
                         s_xi[3][threadIdx.x];
 }
 
+// Refinement: a pixel record {J[6], r, used} into the kGnSums running sums
+// (21 J^TJ upper, 6 J^T r, count, r^2).
+constexpr int kGnPix = 8;
+__device__ __forceinline__ void gn_add_pixel(const float* px, float* acc) {
+  if (px[7] == 0.0f) return;
+  int q = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) acc[q++] += px[i] * px[j];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) acc[21 + i] += px[i] * px[6];
+  acc[27] += 1.0f;
+  acc[28] += px[6] * px[6];
+}
+
 // Refinement: the block's kGnSums partial sums (FP32) into its slot of part.
 __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ part) {
   __shared__ float s_gn[4][kGnSums];
@@ -2462,7 +2477,7 @@ __global__ __launch_bounds__(256) void k_raycast(
   bool pending = false;
   int visited = 0;
   double xi[6] = {0, 0, 0, 0, 0, 0};
-  float gn[MODE == kModeRefine ? kGnSums : 1] = {};
+  float gn[MODE == kModeRefine ? kGnPix : 1] = {};  // refinement: the pixel record
   RayState r;
   float T[12];
   load_uniform<12>(st->T, T);
@@ -2478,7 +2493,11 @@ __global__ __launch_bounds__(256) void k_raycast(
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * blockIdx.x);
-  if constexpr (MODE == kModeRefine) block_gn(gn, io.gn_part + kGnSums * blockIdx.x);
+  if constexpr (MODE == kModeRefine) {
+    float acc[kGnSums] = {};
+    gn_add_pixel(gn, acc);
+    block_gn(acc, io.gn_part + kGnSums * blockIdx.x);
+  }
   // wave-aggregated append of the unfinished rays to this block's partition
   const uint64_t mask = __ballot(pending);
   if (mask) {
@@ -2569,8 +2588,10 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
     int hit = -1, visited = 0;
     const Sink sink{io.gF, false, nullptr, nullptr, io.touched};
+    float px[MODE == kModeRefine ? kGnPix : 1] = {};  // refinement: the ray's pixel record
     const int owner =
-        wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, gn);
+        wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, px);
+    if constexpr (MODE == kModeRefine) gn_add_pixel(px, gn);
     if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
       if (lane == owner) {
         out_v[r.pix] = ov;
