@@ -2394,6 +2394,14 @@ __device__ __forceinline__ void block_gn(const float* gn, float* __restrict__ pa
 constexpr int kPass1Batches = KF_PASS1_BATCHES;
 constexpr int kPass1Steps = KF_PASS1_STEPS;  // steps per batch
 
+// Second-pass grid: 8192 waves, one queued ray per wave at the few thousand
+// rays a 640x480 frame queues.  The adjoint and refinement modes keep one
+// partial-sum slot per block.
+#ifndef KF_REST_BLOCKS
+#define KF_REST_BLOCKS 2048
+#endif
+constexpr int kRestBlocks = KF_REST_BLOCKS;
+
 // Block number that gives each XCD a contiguous run of the n_blocks tiles
 // (neighbouring tiles read neighbouring bricks, and each XCD has its own L2):
 // XCD x receives hardware blocks x, x+8, ...: q+1 of them for x < r, q
@@ -2720,11 +2728,24 @@ __global__ __launch_bounds__(kAdjFinishThreads) void k_adjoint_finish(
 constexpr int kRefineFinishThreads = 512;
 
 __global__ __launch_bounds__(kRefineFinishThreads) void k_refine_finish(
-    const float* __restrict__ part, int n_part, kf_dev_state* st, int step) {
-  // FP64 sums of the partials in a fixed order: thread t adds blocks t,
-  // t + kRefineFinishThreads, ..., then a tree over the threads
+    const float* __restrict__ part, int n_first, const uint32_t* __restrict__ qcount,
+    kf_dev_state* st, int step) {
+  // The first pass's n_first block slots, then the second pass's: only its
+  // first ceil(queued / 4) blocks received rays (one per wave, four waves a
+  // block); the others wrote zeros, which are not read.
   __shared__ double red[kGnSums][kRefineFinishThreads];
   __shared__ double s[kGnSums];
+  __shared__ uint32_t s_queued;
+  if (threadIdx.x < 64) {
+    uint32_t n = qcount[threadIdx.x * KF_RAY_COUNT_STRIDE];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if (threadIdx.x == 0) s_queued = n;
+  }
+  __syncthreads();
+  const int n_part = n_first + (int)min((s_queued + 3u) / 4u, (uint32_t)kRestBlocks);
+  // FP64 sums of the partials in a fixed order: thread t adds blocks t,
+  // t + kRefineFinishThreads, ..., then a tree over the threads
   {
     double v[kGnSums];
 #pragma unroll
@@ -3115,13 +3136,6 @@ static int refine_pass1_batches(const kf_context* c, float tmax) {
   return std::max(kPass1Batches, std::min(32, b));
 }
 
-// Second-pass grid: 8192 waves, one queued ray per wave at the few thousand
-// rays a 640x480 frame queues.  The adjoint and refinement modes keep one
-// partial-sum slot per block.
-#ifndef KF_REST_BLOCKS
-#define KF_REST_BLOCKS 2048
-#endif
-constexpr int kRestBlocks = KF_REST_BLOCKS;
 
 template <int PREC, bool COUNT, int MODE>
 static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
@@ -3177,11 +3191,14 @@ hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam) {
 }
 
 hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps) {
-  const int n_part = ((c->W + 15) / 16) * ((c->H + 15) / 16) + kRestBlocks;
+  const int n_first = ((c->W + 15) / 16) * ((c->H + 15) / 16);
   for (int it = 0; it < steps; ++it) {
+    // the raycast's append counters (launch_raycast_mode flips the slot)
+    const uint32_t* qc =
+        c->ray_counts + (size_t)c->ray_slot * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
     hipError_t e = launch_raycast_mode(c, cam, kModeRefine);
     if (e != hipSuccess) return e;
-    k_refine_finish<<<1, kRefineFinishThreads, 0, c->stream>>>(c->gn_partials, n_part,
+    k_refine_finish<<<1, kRefineFinishThreads, 0, c->stream>>>(c->gn_partials, n_first, qc,
                                                                 c->state, it);
   }
   return hipGetLastError();
