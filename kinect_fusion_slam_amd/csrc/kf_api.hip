@@ -374,7 +374,8 @@ int kf_destroy(kf_context* c) {
                   c->integ_tiles,
                   c->spatial, c->range_lut, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
-                  c->adj_partials, c->gn_partials, c->adj_touched};
+                  c->adj_partials, c->gn_partials, c->adj_touched, c->adj_map_v,
+                  c->adj_map_n, c->adj_map_hit};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1039,6 +1040,9 @@ int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_depth, n * 4));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_normal, n * 16));
     KF_HIP_CHECK(hipMalloc((void**)&c->adj_partials, kf_adjoint_partials(c) * 8));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_map_v, n * 16));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_map_n, n * 16));
+    KF_HIP_CHECK(hipMalloc((void**)&c->adj_map_hit, n * 4));
   }
   int rc = kf_set_pose(c, T);
   if (rc != KF_OK) return rc;

@@ -154,6 +154,9 @@ struct kf_context {
                                    // (zeroed before the next adjoint)
   float* adj_depth = nullptr;   // dL/d(rendered depth), H*W
   float4* adj_normal = nullptr; // dL/d(normal), H*W
+  float4* adj_map_v = nullptr;  // the adjoint's own trilinear raycast maps (when the
+  float4* adj_map_n = nullptr;  //   model maps' saved crossings do not apply), so
+  int32_t* adj_map_hit = nullptr; // that the context's model maps stay as they are
   bool adj_normal_set = false;
   bool adj_want_tsdf = false;   // the current adjoint call returns dL/dTSDF
   // The model maps hold a trilinear raycast's crossings (model_n.w = te where

@@ -3137,37 +3137,47 @@ static int refine_pass1_batches(const kf_context* c, float tmax) {
 }
 
 
+// Model-map outputs of a raycast (the context's, or the adjoint's own).
+struct RayMaps {
+  float4* v;
+  float4* n;
+  int32_t* hit;
+};
+
 template <int PREC, bool COUNT, int MODE>
-static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb) {
+static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb,
+                             const RayMaps& m) {
   RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
   if (MODE != kModeRefine && io.pass1_batches > kPass1Batches)
     k_raycast<PREC, COUNT, MODE, true><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, io, c->H, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
+        c->tsdf, io, c->H, c->state, m.v, m.n, m.hit, q, tiles_x, nb);
   else
     k_raycast<PREC, COUNT, MODE, false><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, io, c->H, c->state, c->model_v, c->model_n, c->hit, q, tiles_x, nb);
+        c->tsdf, io, c->H, c->state, m.v, m.n, m.hit, q, tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
   k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(
-      c->tsdf, io2, c->state, c->model_v, c->model_n, c->hit, q);
+      c->tsdf, io2, c->state, m.v, m.n, m.hit, q);
 }
 
 template <int PREC>
 static void launch_raycast_p(kf_context* c, const RayIO& io, int tiles_x, int nb, bool count,
-                             int mode) {
-  if (mode == kModeAdjoint)
-    launch_raycast_t<PREC, false, kModeAdjoint>(c, io, tiles_x, nb);
-  else if (mode == kModeRefine)
-    launch_raycast_t<PREC, false, kModeRefine>(c, io, tiles_x, nb);
+                             int mode, const RayMaps& m) {
+  if (mode == kModeRefine)
+    launch_raycast_t<PREC, false, kModeRefine>(c, io, tiles_x, nb, m);
   else if (mode == kModeTrilinear)
-    count ? launch_raycast_t<PREC, true, kModeTrilinear>(c, io, tiles_x, nb)
-          : launch_raycast_t<PREC, false, kModeTrilinear>(c, io, tiles_x, nb);
+    count ? launch_raycast_t<PREC, true, kModeTrilinear>(c, io, tiles_x, nb, m)
+          : launch_raycast_t<PREC, false, kModeTrilinear>(c, io, tiles_x, nb, m);
   else
-    count ? launch_raycast_t<PREC, true, kModeReference>(c, io, tiles_x, nb)
-          : launch_raycast_t<PREC, false, kModeReference>(c, io, tiles_x, nb);
+    count ? launch_raycast_t<PREC, true, kModeReference>(c, io, tiles_x, nb, m)
+          : launch_raycast_t<PREC, false, kModeReference>(c, io, tiles_x, nb, m);
 }
 
-static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode) {
+// maps = null: the context's model maps; count = the context's work-counter
+// switch (ignored by the modes that do not count).
+static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode,
+                                      const RayMaps* maps = nullptr, bool count = true) {
+  const RayMaps m = maps ? *maps : RayMaps{c->model_v, c->model_n, c->hit};
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
   RayIO io = make_ray_io(c, cam);
@@ -3180,9 +3190,9 @@ static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode
   io.qcap = kf_ray_part_capacity(nb);
   c->ray_slot ^= 1;
   if (c->vol.precision == KF_TSDF_F32)
-    launch_raycast_p<KF_TSDF_F32>(c, io, tiles_x, nb, c->count_rays, mode);
+    launch_raycast_p<KF_TSDF_F32>(c, io, tiles_x, nb, count && c->count_rays, mode, m);
   else
-    launch_raycast_p<KF_TSDF_F16>(c, io, tiles_x, nb, c->count_rays, mode);
+    launch_raycast_p<KF_TSDF_F16>(c, io, tiles_x, nb, count && c->count_rays, mode, m);
   return hipGetLastError();
 }
 
@@ -3218,22 +3228,24 @@ hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam, bool save
     k_zero_touched<<<(unsigned)((n_words + 3) / 4), 256, 0, c->stream>>>(
         reinterpret_cast<float4*>(c->grad_tsdf), c->adj_touched, n_words);
   }
-  int n_part = (int)(kf_adjoint_partials(c) / 6);
-  if (saved) {
-    const int tiles_x = (c->W + 15) / 16, nb = tiles_x * ((c->H + 15) / 16);
-    const RayIO io = make_ray_io(c, cam);
-    if (c->vol.precision == KF_TSDF_F32)
-      k_adjoint_saved<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
-                                                               c->model_n, c->hit, tiles_x, nb);
-    else
-      k_adjoint_saved<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
-                                                               c->model_n, c->hit, tiles_x, nb);
-    n_part = nb;
-  } else {
-    hipError_t e = launch_raycast_mode(c, cam, kModeAdjoint);
+  // Without saved crossings for these inputs, the trilinear raycast first
+  // walks every ray into the adjoint's own maps (the context's model maps
+  // stay as they are); the adjoint then starts from those crossings.
+  RayMaps m{c->model_v, c->model_n, c->hit};
+  if (!saved) {
+    m = RayMaps{c->adj_map_v, c->adj_map_n, c->adj_map_hit};
+    hipError_t e = launch_raycast_mode(c, cam, kModeTrilinear, &m, false);
     if (e != hipSuccess) return e;
   }
-  k_adjoint_finish<<<1, kAdjFinishThreads, 0, c->stream>>>(c->adj_partials, n_part, c->state);
+  const int tiles_x = (c->W + 15) / 16, nb = tiles_x * ((c->H + 15) / 16);
+  const RayIO io = make_ray_io(c, cam);
+  if (c->vol.precision == KF_TSDF_F32)
+    k_adjoint_saved<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state, m.n,
+                                                             m.hit, tiles_x, nb);
+  else
+    k_adjoint_saved<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state, m.n,
+                                                             m.hit, tiles_x, nb);
+  k_adjoint_finish<<<1, kAdjFinishThreads, 0, c->stream>>>(c->adj_partials, nb, c->state);
   return hipGetLastError();
 }
 
