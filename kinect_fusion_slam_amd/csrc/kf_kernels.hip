@@ -2468,18 +2468,10 @@ __global__ __launch_bounds__(256) void k_raycast(
   const bool active = x < W && y < H;
   const int p = y * W + x;
 
+  static_assert(MODE != kModeAdjoint, "the adjoint runs in k_adjoint_saved");
   VolRead<PREC> F;
   F.v = reinterpret_cast<decltype(F.v)>(volume);
-  constexpr int kTable = MODE == kModeAdjoint ? kCells : 1;
-  __shared__ int s_key[kTable];
-  __shared__ float s_val[kTable * 8];
-  constexpr int kBricks = MODE == kModeAdjoint ? kBrickSet : 1;
-  __shared__ int s_brick[kBricks];
-  const bool use_table = MODE == kModeAdjoint && io.gF != nullptr;
-  if constexpr (MODE == kModeAdjoint) {
-    if (use_table) table_clear(s_key, s_val, s_brick);
-  }
-  const Sink sink{io.gF, use_table, (lds_int*)s_key, (lds_float*)s_val, io.touched};
+  const Sink sink{};  // adjoint only
   float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
   int hit = -1;
   bool pending = false;
@@ -2500,7 +2492,6 @@ __global__ __launch_bounds__(256) void k_raycast(
     }
   }
   if constexpr (COUNT) add_counts(st, visited, hit >= 0);
-  if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * blockIdx.x);
   if constexpr (MODE == kModeRefine) {
     float acc[kGnSums] = {};
     gn_add_pixel(gn, acc);
@@ -2518,10 +2509,7 @@ __global__ __launch_bounds__(256) void k_raycast(
     if (pending)
       queue[(size_t)part * io.qcap + base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
-  if constexpr (MODE == kModeAdjoint) {
-    if (use_table) table_flush(s_key, s_val, s_brick, io);
-  }
-  if (!active || MODE == kModeAdjoint || MODE == kModeRefine) return;
+  if (!active || MODE == kModeRefine) return;
   out_v[p] = ov;
   out_n[p] = on;
   out_hit[p] = hit;
@@ -2567,16 +2555,13 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
   const uint32_t n = __builtin_amdgcn_readlane(incl, 63);
   // the next raycast appends to the other slot, which nothing reads now
   if (blockIdx.x == 0 && threadIdx.x < 64) io.qnext[ln * KF_RAY_COUNT_STRIDE] = 0u;
+  static_assert(MODE != kModeAdjoint, "the adjoint runs in k_adjoint_saved");
   // a block none of whose waves receives a ray (n is the same in every wave)
   // only writes zero partial sums
-  if constexpr (MODE == kModeAdjoint || MODE == kModeRefine) {
+  if constexpr (MODE == kModeRefine) {
     if (blockIdx.x * 4u >= n) {
-      if constexpr (MODE == kModeRefine) {
-        if (threadIdx.x < kGnSums)
-          io.gn_part[kGnSums * (io.part_base + blockIdx.x) + threadIdx.x] = 0.0f;
-      } else {
-        if (threadIdx.x < 6) io.xi_part[6 * (io.part_base + blockIdx.x) + threadIdx.x] = 0.0;
-      }
+      if (threadIdx.x < kGnSums)
+        io.gn_part[kGnSums * (io.part_base + blockIdx.x) + threadIdx.x] = 0.0f;
       return;
     }
   }
@@ -2600,7 +2585,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     const int owner =
         wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, px);
     if constexpr (MODE == kModeRefine) gn_add_pixel(px, gn);
-    if constexpr (MODE != kModeAdjoint && MODE != kModeRefine) {
+    if constexpr (MODE != kModeRefine) {
       if (lane == owner) {
         out_v[r.pix] = ov;
         out_n[r.pix] = on;
@@ -2609,17 +2594,16 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     }
     if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   }
-  if constexpr (MODE == kModeAdjoint) block_xi(xi, io.xi_part + 6 * (io.part_base + blockIdx.x));
   if constexpr (MODE == kModeRefine)
     block_gn(gn, io.gn_part + kGnSums * (io.part_base + blockIdx.x));
 }
 
-// Adjoint from the saved crossings of a trilinear raycast at the same pose,
+// The adjoint, from the crossings of a trilinear raycast at the same pose,
 // camera, depth and volume: the walk is deterministic in those inputs, so
-// each pixel whose raycast hit (hit index >= 0) repeats tri_hit at the te
-// that raycast saved in model_n.w, with the same sums as the walking adjoint
-// (k_raycast / k_raycast_rest in kModeAdjoint) and no walk.  Same tiles, XCD
-// order, dL/dF table and per-block dL/dxi slot as the first pass.
+// each pixel whose raycast hit (hit index >= 0) runs tri_hit at the te that
+// raycast saved in its normal map's w, with no walk.  Same tiles and XCD
+// order as the raycast's first pass; dL/dF through the block's cell table,
+// dL/dxi into the block's slot.
 template <int PREC>
 __global__ __launch_bounds__(256) void k_adjoint_saved(
     const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
@@ -3219,7 +3203,7 @@ size_t kf_refine_partials(const kf_context* c) {
 }
 
 size_t kf_adjoint_partials(const kf_context* c) {
-  return (size_t)(((c->W + 15) / 16) * ((c->H + 15) / 16) + kRestBlocks) * 6;
+  return (size_t)(((c->W + 15) / 16) * ((c->H + 15) / 16)) * 6;  // one slot per tile
 }
 
 hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam, bool saved) {
