@@ -264,8 +264,9 @@ int kf_download_rendered_depth(kf_context* ctx, float* depth_mm);
  * kf_raycast in KF_RAYCAST_TRILINEAR mode with the same K and T (bitwise),
  * the adjoint starts from the crossing that raycast saved per pixel (the
  * voxel-entry parameter, in the unused w lane of its normal map) and skips
- * the walk; otherwise it walks every ray again.  Both give the same sums up
- * to the order of float additions.  dL/dTSDF is accumulated with float
+ * the walk; otherwise it first runs that raycast into buffers of its own
+ * (the context's model maps are left as they are).  Both give the same sums
+ * up to the order of float additions.  dL/dTSDF is accumulated with float
  * atomics, so its low bits depend on the order of arrival; with dl_dtsdf
  * NULL it is not computed at all (pose-only adjoint). */
 int kf_raycast_adjoint(kf_context* ctx, const float K[9], const float T[16],
