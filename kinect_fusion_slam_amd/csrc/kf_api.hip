@@ -1024,27 +1024,58 @@ int kf_download_rendered_depth(kf_context* c, float* depth_mm) {
   return KF_OK;
 }
 
+}  // extern "C"
+
+// The adjoint's buffers, allocated together on the first kf_raycast_adjoint.
+// Either all of them exist (grad_tsdf set last) or none: a failed allocation
+// frees the others, so a later call allocates again instead of launching on
+// null buffers.
+static int adjoint_buffers(kf_context* c) {
+  if (c->grad_tsdf) return KF_OK;
+  const size_t n = (size_t)c->H * c->W;
+  const size_t touched_bytes = (size_t)((c->n_bricks + 31) / 32) * 4;
+  float* grad = nullptr;
+  struct Buf {
+    void** p;
+    size_t bytes;
+  } bufs[] = {
+      {(void**)&c->adj_touched, touched_bytes},  {(void**)&c->adj_depth, n * 4},
+      {(void**)&c->adj_normal, n * 16},           {(void**)&c->adj_partials, kf_adjoint_partials(c) * 8},
+      {(void**)&c->adj_map_v, n * 16},            {(void**)&c->adj_map_n, n * 16},
+      {(void**)&c->adj_map_hit, n * 4},           {(void**)&grad, (size_t)c->n_store * 4},
+  };
+  bool ok = true;
+  for (auto& b : bufs)
+    if (ok && hipMalloc(b.p, b.bytes) != hipSuccess) {
+      *b.p = nullptr;
+      ok = false;
+    }
+  // all zero once; afterwards each adjoint zeroes the bricks the previous
+  // one wrote
+  ok = ok && hipMemsetAsync(grad, 0, (size_t)c->n_store * 4, c->stream) == hipSuccess &&
+       hipMemsetAsync(c->adj_touched, 0, touched_bytes, c->stream) == hipSuccess;
+  if (!ok) {
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& b : bufs) {
+      if (*b.p) (void)hipFree(*b.p);
+      *b.p = nullptr;
+    }
+    return kf_fail(KF_ERR_OOM, "cannot allocate the raycast adjoint's buffers");
+  }
+  c->grad_tsdf = grad;
+  return KF_OK;
+}
+
+extern "C" {
+
 int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const float* dl_ddepth,
                        const float* dl_dnormal, float* dl_dtsdf, double dl_dxi[6]) {
   KF_REQUIRE(c && K && T && dl_ddepth, KF_ERR_INVALID, "null argument");
   KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
   const size_t n = (size_t)c->H * c->W;
-  if (!c->grad_tsdf) {
-    KF_HIP_CHECK(hipMalloc((void**)&c->grad_tsdf, (size_t)c->n_store * 4));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_touched, (size_t)((c->n_bricks + 31) / 32) * 4));
-    // all zero once; afterwards each adjoint zeroes the bricks the previous
-    // one wrote
-    KF_HIP_CHECK(hipMemsetAsync(c->grad_tsdf, 0, (size_t)c->n_store * 4, c->stream));
-    KF_HIP_CHECK(hipMemsetAsync(c->adj_touched, 0, (size_t)((c->n_bricks + 31) / 32) * 4,
-                                c->stream));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_depth, n * 4));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_normal, n * 16));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_partials, kf_adjoint_partials(c) * 8));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_map_v, n * 16));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_map_n, n * 16));
-    KF_HIP_CHECK(hipMalloc((void**)&c->adj_map_hit, n * 4));
-  }
-  int rc = kf_set_pose(c, T);
+  int rc = adjoint_buffers(c);
+  if (rc != KF_OK) return rc;
+  rc = kf_set_pose(c, T);
   if (rc != KF_OK) return rc;
   KF_HIP_CHECK(hipMemcpyAsync(c->adj_depth, dl_ddepth, n * 4, hipMemcpyHostToDevice, c->stream));
   c->adj_normal_set = dl_dnormal != nullptr;

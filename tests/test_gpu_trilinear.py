@@ -71,33 +71,50 @@ def _small_scene():
     return vol, trunc, K, T0.astype(np.float32), 48, 64
 
 
+@pytest.mark.parametrize("precision", ["f32", "f16"])
 @pytest.mark.parametrize("path", ["saved", "walk"])
-def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle, path):
+def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle, path, precision):
     """saved: the adjoint right after the raycast at the same inputs, which
     starts from the raycast's saved crossings; walk: the depth uploaded again
-    in between (same values), which drops them, so every ray is walked."""
+    in between (same values), which drops them, so the adjoint first runs its
+    own trilinear raycast and starts from that one's crossings.
+
+    f16: the volume is stored as half TSDF + u16 weight.  The GPU forward is
+    compared bit-exactly with the oracle's FP32 forward over the same half
+    values, and the adjoint with the oracle's FP64 adjoint of the
+    half-rounded TSDF (the stored values are what the gradient is taken
+    of)."""
     geo, trunc, K, T0, H, W = _small_scene()
     vol, prm, _, _, _ = kf.preset(kf.PRESET_SPHERES_128)
     vol.dims[:] = list(geo["dims"])
     vol.start[:] = list(geo["start"])
     vol.voxel_mm = geo["voxel"]
+    vol.precision = kf.KF_TSDF_F32 if precision == "f32" else kf.KF_TSDF_F16
     prm.truncation_mm, prm.sdf_min_mm, prm.sdf_max_mm = trunc, -trunc, 10 * trunc
     F = sphere_tsdf(vol, [(0, 0, 260.0, 90.0), (70, -40, 200.0, 50.0), (-60, 50, 230.0, 60.0)],
                     trunc).astype(np.float32)
+    # what the forward reads: the float values, or their half bits
+    Fs = F if precision == "f32" else oracle.f32_to_f16_bits(F)
+    if precision == "f16":
+        Fh = oracle.f16_bits_to_f32(Fs)  # the half-rounded values, exact in float
+        fin = np.isfinite(F)
+        band = fin & (np.abs(F) < 1)  # every voxel inside the band rounds
+        assert np.array_equal(fin, np.isfinite(Fh)) and np.all(Fh[band] != F[band])
+        F = Fh
     rng = np.random.default_rng(21)
-    r0 = oracle.raycast_tri(np.full((H, W), 150.0, np.float32), K, T0, F, vol, trunc)
+    r0 = oracle.raycast_tri(np.full((H, W), 150.0, np.float32), K, T0, Fs, vol, trunc)
     guide = np.where(r0["hit"] >= 0, r0["depth"] + rng.uniform(-4, 4, (H, W)), 0).astype(
         np.float32)
     a, b = rng.normal(size=(H, W)), rng.normal(size=(H, W, 3))
     # pixels whose FP32 and FP64 discrete states agree
-    f32 = oracle.raycast_tri(guide, K, T0, F, vol, trunc)
+    f32 = oracle.raycast_tri(guide, K, T0, Fs, vol, trunc)
     f64 = oracle.raycast_tri(guide, K, T0.astype(np.float64), F.astype(np.float64), vol, trunc)
     same = (f32["hit"] >= 0) & np.all(f32["disc"] == f64["disc"], -1) & (f64["hit"] >= 0)
     assert same.sum() > 1500
     am, bm = np.where(same, a, 0), np.where(same[..., None], b, 0)
     want = oracle.raycast_tri(guide, K, T0.astype(np.float64), F.astype(np.float64), vol, trunc,
                               grad_depth=am, grad_normal=bm)
-    say(f"[gpu] raycast adjoint ({path}), 40^3 three-sphere volume")
+    say(f"[gpu] raycast adjoint ({path}, {precision}), 40^3 three-sphere volume")
     with kf.KinfuContext(vol, prm, H, W) as c:
         c.upload_volume(F, np.where(np.isnan(F), np.nan, 1.0).astype(np.float32))
         c.upload_depth_f32(guide)
