@@ -329,6 +329,11 @@ int kf_raycast_work(kf_context* ctx, int64_t* voxels_visited, int64_t* hits);
  *   flip (voxel_traversal.cu:361-364): acos((double)c)*(180/pi_f) > 90
  *        <=>  -1 <= c < flip_hi                                          */
 int kf_gate_thresholds(float icp_angle_deg, float* icp_lo, float* flip_hi);
+/* The ICP distance gate (sensor_pose_estimation.cu:44-48) is evaluated on the
+ * squared distance d2 (the reference's float sum of squares) as
+ *   sqrtf(d2) <= gate && sqrtf(d2) >= 0   <=>   d2 <= dist2_hi
+ * for a correctly rounded sqrtf; dist2_hi is -1 when no distance passes. */
+int kf_dist_gate_threshold(float icp_dist_mm, float* dist2_hi);
 /* The exp used by the bilateral filter and its spatial kernel. */
 float kf_expf_abi(float x);
 /* FP64 helpers rounded to FP32, as the kernels use them. */

@@ -11,7 +11,7 @@ from ._abi import (KF_ICP_STATUS_GATED, KF_MEM_DEVICE, KF_MEM_HOST,  # noqa: F40
                    KF_RAYCAST_TRILINEAR, KF_TSDF_F16, KF_TSDF_F32, PRESET_REFERENCE, PRESET_ROOM,
                    PRESET_ROOM_5MM_F16, PRESET_ROOM_CPU, PRESET_ROOM_HD_F16, PRESET_SPHERES_128,
                    PRESETS, STAGES, KinfuContext, KinfuError, Params, SynthScene,
-                   VolumeDesc, exported_symbols, gate_thresholds, ldlt6, lib, mat3_inverse,
+                   VolumeDesc, dist_gate_threshold, exported_symbols, gate_thresholds, ldlt6, lib, mat3_inverse,
                    mat4_inverse, orbit_pose, preset, preset_scene, read_ply, render,
                    spatial_kernel, stream, write_ply)
 

@@ -119,6 +119,7 @@ _SIGNATURES = {
     "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
     "kf_adjoint_used_saved": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
+    "kf_dist_gate_threshold": (C.c_int, [C.c_float, _F]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
     "kf_mat4_inverse": (C.c_int, [_F, _F]),
     "kf_mat3_inverse": (C.c_int, [_F, _F]),
@@ -184,6 +185,13 @@ def gate_thresholds(angle_deg):
     lo, hi = C.c_float(), C.c_float()
     check(lib.kf_gate_thresholds(angle_deg, C.byref(lo), C.byref(hi)))
     return lo.value, hi.value
+
+
+def dist_gate_threshold(dist_mm):
+    """Largest squared distance the ICP distance gate passes (-1: none)."""
+    v = C.c_float()
+    check(lib.kf_dist_gate_threshold(dist_mm, C.byref(v)))
+    return v.value
 
 
 def mat4_inverse(T):

@@ -89,6 +89,29 @@ float icp_cos_threshold(float gate_deg) {
   return ord2f(hi);
 }
 
+// Largest float s with the reference's distance gate passing for a squared
+// distance s: sqrtf(s) <= gate && sqrtf(s) >= 0 (sensor_pose_estimation.cu:
+// 47-48) for a correctly rounded sqrtf.  sqrtf is monotone, so the gate holds
+// exactly on [0, s]; -1 when no distance passes.
+float icp_dist2_threshold(float gate_mm) {
+  auto pass = [&](float s) {
+    const float d = sqrtf(s);
+    return d <= gate_mm && d >= 0.0f;
+  };
+  const float inf = INFINITY;
+  if (!pass(0.0f)) return -1.0f;
+  if (pass(inf)) return inf;
+  int64_t lo = f2ord(0.0f), hi = f2ord(inf);  // pass(lo) true, pass(hi) false
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (pass(ord2f(mid)))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return ord2f(lo);
+}
+
 // Smallest float c with !(acos((double)c) * kDegF > 90): the normal is flipped
 // (voxel_traversal.cu:361-369) exactly for -1 <= c < this value.
 float flip_cos_threshold() {
@@ -339,6 +362,12 @@ int kf_gate_thresholds(float icp_angle_deg, float* icp_lo, float* flip_hi) {
   return KF_OK;
 }
 
+int kf_dist_gate_threshold(float icp_dist_mm, float* dist2_hi) {
+  KF_REQUIRE(dist2_hi, KF_ERR_INVALID, "null pointer");
+  *dist2_hi = icp_dist2_threshold(icp_dist_mm);
+  return KF_OK;
+}
+
 float kf_expf_abi(float x) { return kf_expf(x); }
 
 int kf_mat4_inverse(const float T[16], float Tinv[16]) {
@@ -375,7 +404,7 @@ int kf_destroy(kf_context* c) {
                   c->spatial, c->range_lut, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
                   c->adj_partials, c->gn_partials, c->adj_touched, c->adj_map_v,
-                  c->adj_map_n, c->adj_map_hit};
+                  c->adj_map_n, c->adj_map_hit, c->model_aux, c->adj_map_aux};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -411,6 +440,7 @@ static int apply_params(kf_context* c, const kf_params* prm) {
       KF_HIP_CHECK(hipMemcpy(c->range_lut, lut.data(), sizeof(float) * n, hipMemcpyHostToDevice));
   }
   c->icp_cos_lo = icp_cos_threshold(prm->icp_angle_deg);
+  c->icp_dist2_hi = icp_dist2_threshold(prm->icp_dist_mm);
   c->flip_cos_hi = flip_cos_threshold();
   if (prm->record_icp && !c->icp_masks) {
     const size_t bytes = (size_t)KF_MAX_ICP_ITERS * c->H * c->W;
@@ -502,9 +532,10 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
     size_t bytes;
   } allocs[] = {
       {(void**)&c->depth, n * 4},       {(void**)&c->depth_tmp, n * 4},
-      {(void**)&c->depth_u16, n * 2},   {(void**)&c->vmap, n * 16},
-      {(void**)&c->nmap, n * 16},       {(void**)&c->mask, n},
-      {(void**)&c->model_v, n * 16},    {(void**)&c->model_n, n * 16},
+      {(void**)&c->depth_u16, n * 2},   {(void**)&c->vmap, n * 12},
+      {(void**)&c->nmap, n * 12},       {(void**)&c->mask, n},
+      {(void**)&c->model_v, n * 12},    {(void**)&c->model_n, n * 12},
+      {(void**)&c->model_aux, n * 8},
       {(void**)&c->hit, n * 4},         {(void**)&c->integ_tab, n * 8},
       {(void**)&c->integ_tiles, (size_t)((width + 15) / 16) * ((height + 15) / 16) * 8},
       {&c->ray_queue, kf_ray_queue_entries(height, width) * kf_ray_state_bytes()},
@@ -531,9 +562,10 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
     const std::string m = g_last_error;
     return fail(rc, m);
   }
-  void* zero_bufs[] = {c->depth, c->depth_tmp, c->vmap, c->nmap, c->mask, c->model_v, c->model_n};
-  size_t zero_sizes[] = {n * 4, n * 4, n * 16, n * 16, n, n * 16, n * 16};
-  for (int i = 0; i < 7; ++i)
+  void* zero_bufs[] = {c->depth, c->depth_tmp, c->vmap,    c->nmap,
+                       c->mask,  c->model_v,   c->model_n, c->model_aux};
+  size_t zero_sizes[] = {n * 4, n * 4, n * 12, n * 12, n, n * 12, n * 12, n * 8};
+  for (int i = 0; i < 8; ++i)
     if (hipMemsetAsync(zero_bufs[i], 0, zero_sizes[i], c->stream) != hipSuccess)
       return fail(KF_ERR_HIP, "hipMemsetAsync failed");
   if (hipMemsetAsync(c->hit, 0xff, n * 4, c->stream) != hipSuccess ||
@@ -593,7 +625,8 @@ int kf_synchronize(kf_context* c) {
 int kf_device_bytes(const kf_context* c, size_t* bytes) {
   KF_REQUIRE(c && bytes, KF_ERR_INVALID, "null argument");
   const size_t n = (size_t)c->H * c->W;
-  size_t b = c->volume_bytes + n * (4 + 4 + 2 + 16 + 16 + 1 + 16 + 16 + 4 + 8) + sizeof(kf_dev_state) +
+  size_t b = c->volume_bytes + n * (4 + 4 + 2 + 12 + 12 + 1 + 12 + 12 + 8 + 4 + 8) +
+             sizeof(kf_dev_state) +
              (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 8 +
              kf_ray_queue_entries(c->H, c->W) * kf_ray_state_bytes() +
              kf_integ_group_entries(c->vg) * 4 +
@@ -684,16 +717,17 @@ int kf_download_vertex_maps(kf_context* c, float* vx, float* vy, float* vz, floa
                             float* nz, uint8_t* mask) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
   const size_t n = (size_t)c->H * c->W;
-  std::vector<float4> V(n), N(n);
-  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->vmap, n * 16, hipMemcpyDeviceToHost, c->stream));
-  KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->nmap, n * 16, hipMemcpyDeviceToHost, c->stream));
+  // the vertex map's z is 0 where the mask is 0; Vz proper is the depth
+  std::vector<kf_p3> V(n), N(n);
+  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->vmap, n * 12, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->nmap, n * 12, hipMemcpyDeviceToHost, c->stream));
   if (mask)
     KF_HIP_CHECK(hipMemcpyAsync(mask, c->mask, n, hipMemcpyDeviceToHost, c->stream));
+  if (vz) KF_HIP_CHECK(hipMemcpyAsync(vz, c->depth, n * 4, hipMemcpyDeviceToHost, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < n; ++i) {
     if (vx) vx[i] = V[i].x;
     if (vy) vy[i] = V[i].y;
-    if (vz) vz[i] = V[i].z;
     if (nx) nx[i] = N[i].x;
     if (ny) ny[i] = N[i].y;
     if (nz) nz[i] = N[i].z;
@@ -707,13 +741,13 @@ int kf_upload_vertex_maps(kf_context* c, const float* vx, const float* vy, const
   KF_REQUIRE(c && vx && vy && vz && nx && ny && nz && mask, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
   const size_t n = (size_t)c->H * c->W;
-  std::vector<float4> V(n), N(n);
+  std::vector<kf_p3> V(n), N(n);
   for (size_t i = 0; i < n; ++i) {
-    V[i] = make_float4(vx[i], vy[i], vz[i], mask[i] ? 1.0f : 0.0f);
-    N[i] = make_float4(nx[i], ny[i], nz[i], 0.0f);
+    V[i] = kf_p3{vx[i], vy[i], mask[i] ? vz[i] : 0.0f};
+    N[i] = kf_p3{nx[i], ny[i], nz[i]};
   }
-  KF_HIP_CHECK(hipMemcpyAsync(c->vmap, V.data(), n * 16, hipMemcpyHostToDevice, c->stream));
-  KF_HIP_CHECK(hipMemcpyAsync(c->nmap, N.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->vmap, V.data(), n * 12, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->nmap, N.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipMemcpyAsync(c->mask, mask, n, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipMemcpyAsync(c->depth, vz, n * 4, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -950,9 +984,9 @@ int kf_download_model_maps(kf_context* c, float* x, float* y, float* z, float* n
                            float* nz, int32_t* hit_index) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
   const size_t n = (size_t)c->H * c->W;
-  std::vector<float4> V(n), N(n);
-  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->model_v, n * 16, hipMemcpyDeviceToHost, c->stream));
-  KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->model_n, n * 16, hipMemcpyDeviceToHost, c->stream));
+  std::vector<kf_p3> V(n), N(n);
+  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->model_v, n * 12, hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->model_n, n * 12, hipMemcpyDeviceToHost, c->stream));
   if (hit_index)
     KF_HIP_CHECK(hipMemcpyAsync(hit_index, c->hit, n * 4, hipMemcpyDeviceToHost, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1017,10 +1051,15 @@ int kf_export_model_ply(kf_context* c, const char* path, const uint8_t* bgr, int
 int kf_download_rendered_depth(kf_context* c, float* depth_mm) {
   KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
   const size_t n = (size_t)c->H * c->W;
-  std::vector<float4> V(n);
-  KF_HIP_CHECK(hipMemcpyAsync(V.data(), c->model_v, n * 16, hipMemcpyDeviceToHost, c->stream));
+  if (!c->maps_trilinear) {
+    // the last raycast into the model maps was a REFERENCE one
+    memset(depth_mm, 0, n * 4);
+    return KF_OK;
+  }
+  std::vector<float2> A(n);
+  KF_HIP_CHECK(hipMemcpyAsync(A.data(), c->model_aux, n * 8, hipMemcpyDeviceToHost, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < n; ++i) depth_mm[i] = V[i].w;
+  for (size_t i = 0; i < n; ++i) depth_mm[i] = A[i].x;
   return KF_OK;
 }
 
@@ -1041,7 +1080,8 @@ static int adjoint_buffers(kf_context* c) {
   } bufs[] = {
       {(void**)&c->adj_touched, touched_bytes},  {(void**)&c->adj_depth, n * 4},
       {(void**)&c->adj_normal, n * 16},           {(void**)&c->adj_partials, kf_adjoint_partials(c) * 8},
-      {(void**)&c->adj_map_v, n * 16},            {(void**)&c->adj_map_n, n * 16},
+      {(void**)&c->adj_map_v, n * 12},            {(void**)&c->adj_map_n, n * 12},
+      {(void**)&c->adj_map_aux, n * 8},
       {(void**)&c->adj_map_hit, n * 4},           {(void**)&grad, (size_t)c->n_store * 4},
   };
   bool ok = true;
@@ -1121,13 +1161,14 @@ int kf_upload_model_maps(kf_context* c, const float* x, const float* y, const fl
   KF_REQUIRE(c && x && y && z && nx && ny && nz, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
   const size_t n = (size_t)c->H * c->W;
-  std::vector<float4> V(n), N(n);
+  std::vector<kf_p3> V(n), N(n);
   for (size_t i = 0; i < n; ++i) {
-    V[i] = make_float4(x[i], y[i], z[i], 0.0f);
-    N[i] = make_float4(nx[i], ny[i], nz[i], 0.0f);
+    V[i] = kf_p3{x[i], y[i], z[i]};
+    N[i] = kf_p3{nx[i], ny[i], nz[i]};
   }
-  KF_HIP_CHECK(hipMemcpyAsync(c->model_v, V.data(), n * 16, hipMemcpyHostToDevice, c->stream));
-  KF_HIP_CHECK(hipMemcpyAsync(c->model_n, N.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+  c->maps_trilinear = false;
+  KF_HIP_CHECK(hipMemcpyAsync(c->model_v, V.data(), n * 12, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->model_n, N.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   return KF_OK;
 }

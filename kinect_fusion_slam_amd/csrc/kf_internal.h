@@ -31,6 +31,13 @@ struct kf_dev_state {
   int32_t iter_status[KF_MAX_ICP_ITERS];
 };
 
+// A packed 3-vector of an image map (12 bytes, 4-byte aligned): a wave's 64
+// consecutive records are 768 contiguous bytes, read as one dwordx3 load per
+// lane, with no unused fourth lane as in a float4.
+struct kf_p3 {
+  float x, y, z;
+};
+
 // Camera constants passed by value to every kernel.
 struct kf_cam {
   float K[9];
@@ -100,11 +107,13 @@ struct kf_context {
   float* depth = nullptr;       // working depth, float mm
   float* depth_tmp = nullptr;   // bilateral output
   uint16_t* depth_u16 = nullptr;// staging for host uint16 uploads
-  float4* vmap = nullptr;       // {Vx, Vy, Vz(=depth), mask}
-  float4* nmap = nullptr;       // {Nx, Ny, Nz, 0}
+  kf_p3* vmap = nullptr;        // {Vx, Vy, Vz}, Vz = depth where the mask is 1, else 0
+  kf_p3* nmap = nullptr;        // {Nx, Ny, Nz}
   uint8_t* mask = nullptr;      // vertex mask (reference layout)
-  float4* model_v = nullptr;    // raycast vertices {x, y, z, 0}
-  float4* model_n = nullptr;    // raycast normals
+  kf_p3* model_v = nullptr;     // raycast vertices {x, y, z}
+  kf_p3* model_n = nullptr;     // raycast normals {nx, ny, nz}
+  float2* model_aux = nullptr;  // trilinear raycasts only: {camera depth, crossing te}
+  bool maps_trilinear = false;  // model_aux belongs to the maps (last raycast trilinear)
   int32_t* hit = nullptr;       // raycast hit voxel index
   void* ray_queue = nullptr;    // KF_RAY_PARTS x kf_ray_part_capacity ray states
   uint32_t* ray_counts = nullptr; // [2][KF_RAY_PARTS * KF_RAY_COUNT_STRIDE] append counters
@@ -135,6 +144,7 @@ struct kf_context {
   int icp_px = 4;                  // pixels per thread
   uint8_t* icp_masks = nullptr;    // [KF_MAX_ICP_ITERS][H*W] when record_icp
   float icp_cos_lo = 0.f;          // ICP normal gate as a cosine threshold
+  float icp_dist2_hi = 0.f;        // ICP distance gate as a squared-distance threshold
   float flip_cos_hi = 0.f;         // raycast normal flip threshold
 
   // stream of frames
@@ -154,12 +164,13 @@ struct kf_context {
                                    // (zeroed before the next adjoint)
   float* adj_depth = nullptr;   // dL/d(rendered depth), H*W
   float4* adj_normal = nullptr; // dL/d(normal), H*W
-  float4* adj_map_v = nullptr;  // the adjoint's own trilinear raycast maps (when the
-  float4* adj_map_n = nullptr;  //   model maps' saved crossings do not apply), so
-  int32_t* adj_map_hit = nullptr; // that the context's model maps stay as they are
+  kf_p3* adj_map_v = nullptr;   // the adjoint's own trilinear raycast maps (when the
+  kf_p3* adj_map_n = nullptr;   //   model maps' saved crossings do not apply), so
+  float2* adj_map_aux = nullptr;//   that the context's model maps stay as they are
+  int32_t* adj_map_hit = nullptr;
   bool adj_normal_set = false;
   bool adj_want_tsdf = false;   // the current adjoint call returns dL/dTSDF
-  // The model maps hold a trilinear raycast's crossings (model_n.w = te where
+  // The model maps hold a trilinear raycast's crossings (model_aux.y = te where
   // hit >= 0) made at saved_K / saved_T with the current depth, volume and
   // parameters: every call that changes one of those clears saved_ok.
   bool saved_ok = false;

@@ -26,6 +26,15 @@ namespace {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Pointer qualifier for the global address space: loads through it are
+// global_load (vmcnt only) rather than flat_load (vmcnt and lgkmcnt).  The
+// host compilation pass of the kernels has no such address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KF_GLOBAL __attribute__((address_space(1)))
+#else
+#define KF_GLOBAL
+#endif
+
 // Uniform load of a small float array that the previous kernel wrote.
 template <int N>
 __device__ __forceinline__ void load_uniform(const float* __restrict__ src, float* dst) {
@@ -404,15 +413,19 @@ __global__ __launch_bounds__(256) void k_bilateral(const void* __restrict__ in_,
 // Vertex, normal and mask of pixel (x, y) from its depth di and the depths
 // of the pixels below (dr) and to the right (dcn); dr and dcn are read only
 // where the reference reads them (di != 0, not the last row or column).
+// The stored Vz is the depth where the mask is 1 and 0 elsewhere: ICP uses a
+// pixel where Vz != 0 and the mask is set (sensor_pose_estimation.cu:20),
+// and a set mask implies di != 0, so "stored Vz != 0" is that test, with the
+// same Vz wherever it passes.
 __device__ __forceinline__ void vertex_at(int x, int y, int H, int W, const float* Ki, float di,
-                                          float dr, float dcn, float4& V, float4& N,
+                                          float dr, float dcn, kf_p3& V, kf_p3& N,
                                           uint8_t& m) {
-  V = make_float4(0.f, 0.f, di, 0.f);
-  N = make_float4(0.f, 0.f, 0.f, 0.f);
+  V = kf_p3{0.f, 0.f, 0.f};
+  N = kf_p3{0.f, 0.f, 0.f};
   m = 0;
   if (di != 0 && y != (H - 1) && x != (W - 1)) {
     m = 1;
-    V.w = 1.0f;
+    V.z = di;
     const int nr = (y + 1) * W + x;
     const int nc = y * W + (x + 1);
     if (nr > 0 && nc > 0 && dr != 0 && dcn != 0) {
@@ -438,8 +451,8 @@ __device__ __forceinline__ void vertex_at(int x, int y, int H, int W, const floa
 }
 
 __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_cam cam, int H,
-                                                int W, float4* __restrict__ vmap,
-                                                float4* __restrict__ nmap,
+                                                int W, kf_p3* __restrict__ vmap,
+                                                kf_p3* __restrict__ nmap,
                                                 uint8_t* __restrict__ mask) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= H * W) return;
@@ -450,7 +463,7 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_
   const float di = d[i];
   const float dr = d[min((y + 1) * W + x, n - 1)];
   const float dcn = d[min(y * W + (x + 1), n - 1)];
-  float4 V, N;
+  kf_p3 V, N;
   uint8_t m;
   vertex_at(x, y, H, W, cam.Kinv, di, dr, dcn, V, N, m);
   vmap[i] = V;
@@ -488,34 +501,55 @@ struct IcpShared {
 // Sums each of the ROWS rows of nb partials into sh.rowsum (all threads
 // take part).  Wave w takes rows w, w + NW, ...; each lane loads its PPL
 // entries of those rows (nb <= 64 PPL) with unconditional (clamped) loads,
-// so that they are all in flight together, adds them in FP64, and the 64
-// lane sums of a row are added by a DPP tree (a fixed order).  Ends with a
-// block barrier; icp_update_pose forms the sums.
+// so that they are all in flight together (icp_load_partials), adds them in
+// FP64, and the 64 lane sums of a row are added by a DPP tree, a fixed order
+// (icp_sum_partials, which ends with a block barrier; icp_update_pose forms
+// the sums).
 template <int ROWS, int NT, int PPL, typename PT>
-__device__ __forceinline__ void icp_reduce_partials(const PT* __restrict__ P, int nb,
-                                                    IcpShared<ROWS>& sh) {
-  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
-  constexpr int NW = NT / 64;
-  constexpr int RPW = (ROWS + NW - 1) / NW;  // rows per wave
+struct IcpPartialRegs {
+  static constexpr int NW = NT / 64;
+  static constexpr int RPW = (ROWS + NW - 1) / NW;  // rows per wave
   PT v[RPW][PPL];
+};
+
+template <int ROWS, int NT, int PPL, typename PT, typename PP>
+__device__ __forceinline__ void icp_load_partials(PP P, int nb,
+                                                  IcpPartialRegs<ROWS, NT, PPL, PT>& R) {
+  using Regs = IcpPartialRegs<ROWS, NT, PPL, PT>;
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) {
-    const int row = min(wave + r * NW, ROWS - 1);
+  for (int r = 0; r < Regs::RPW; ++r) {
+    const int row = min(wave + r * Regs::NW, ROWS - 1);
 #pragma unroll
     for (int j = 0; j < PPL; ++j)
-      v[r][j] = P[(size_t)row * nb + min(ln + 64 * j, nb - 1)];
+      R.v[r][j] = P[(size_t)row * nb + min(ln + 64 * j, nb - 1)];
   }
+}
+
+template <int ROWS, int NT, int PPL, typename PT>
+__device__ __forceinline__ void icp_sum_partials(const IcpPartialRegs<ROWS, NT, PPL, PT>& R,
+                                                 int nb, IcpShared<ROWS>& sh) {
+  using Regs = IcpPartialRegs<ROWS, NT, PPL, PT>;
+  const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
-  for (int r = 0; r < RPW; ++r) {
+  for (int r = 0; r < Regs::RPW; ++r) {
     double t = 0.0;
 #pragma unroll
     for (int j = 0; j < PPL; ++j)
-      t += ln + 64 * j < nb ? (double)v[r][j] : 0.0;
+      t += ln + 64 * j < nb ? (double)R.v[r][j] : 0.0;
     t = wave_total(t);
-    const int row = wave + r * NW;
+    const int row = wave + r * Regs::NW;
     if (ln == 63 && row < ROWS) sh.rowsum[row] = t;
   }
   __syncthreads();
+}
+
+template <int ROWS, int NT, int PPL, typename PT>
+__device__ __forceinline__ void icp_reduce_partials(const PT* __restrict__ P, int nb,
+                                                    IcpShared<ROWS>& sh) {
+  IcpPartialRegs<ROWS, NT, PPL, PT> R;
+  icp_load_partials<ROWS, NT, PPL, PT>(P, nb, R);  // NOLINT
+  icp_sum_partials<ROWS, NT, PPL, PT>(R, nb, sh);
 }
 
 // LDS visibility between the lanes of one wave.
@@ -566,9 +600,9 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 // block reduction, for iteration `iter`.
 template <bool KAHAN, int PX, int PPL>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
-    const float4* __restrict__ vmap, const float4* __restrict__ nmap,
-    const float4* __restrict__ mv, const float4* __restrict__ mn, kf_cam cam, int H, int W,
-    float dist_gate, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
+    const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
+    const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, kf_cam cam, int H, int W,
+    float dist2_hi, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
     const void* __restrict__ part_in_, void* __restrict__ part_out_,
     uint8_t* __restrict__ mask_out, int iter) {
   constexpr int NS = KF_ICP_SUMS;
@@ -584,15 +618,38 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 
   const int n = H * W;
   const int base = blockIdx.x * (KF_ICP_BLOCK * PX) + threadIdx.x;
-  // phase 1: this thread's vertex and normal loads, issued before the
-  // previous iteration is finished (they do not depend on the pose); the
-  // index is clamped so that the loads are unconditional
-  float4 V[PX], Nc[PX];
+  // phase 1: the previous iteration's block partials and the two poses it
+  // needs, then this thread's vertex and normal loads, all issued before
+  // the previous iteration is finished (the maps do not depend on the pose).
+  // Loads complete in issue order, so the prologue's loads go first: it
+  // waits for them and not for the maps.  Every load is unconditional, with
+  // in-bounds indices (iteration 0 loads the other parity's partials and
+  // ignores them), so that no branch makes the compiler wait at its join.
+  // The order is pinned for the compiler: the pointers are passed through
+  // empty asm statements (which drops what their __restrict__ lets it
+  // assume), and an asm memory clobber separates the two groups of loads.
+  const PT* pin_ = part_in;
+  const kf_dev_state* stp_ = st;
+  const kf_p3* vm_ = vmap;
+  const kf_p3* nm_ = nmap;
+  asm volatile("" : "+s"(pin_), "+s"(stp_), "+s"(vm_), "+s"(nm_));
+  // back in the global address space (global, not flat, loads)
+  const KF_GLOBAL PT* pin = (const KF_GLOBAL PT*)pin_;
+  const KF_GLOBAL kf_dev_state* stp = (const KF_GLOBAL kf_dev_state*)stp_;
+  const KF_GLOBAL kf_p3* vm = (const KF_GLOBAL kf_p3*)vm_;
+  const KF_GLOBAL kf_p3* nm = (const KF_GLOBAL kf_p3*)nm_;
+  IcpPartialRegs<ROWS, KF_ICP_BLOCK, PPL, PT> R;
+  icp_load_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(pin, nb, R);
+  const int e16 = threadIdx.x & 15;
+  const float T0e = iter <= 1 ? stp->Tprev[e16] : stp->iter_T[iter - 2][e16];
+  const float Tpe = stp->Tprev_inv[e16];
+  asm volatile("" ::: "memory");
+  kf_p3 V[PX], Nc[PX];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
     const int x = min(base + k * KF_ICP_BLOCK, n - 1);
-    V[k] = vmap[x];
-    Nc[k] = nmap[x];
+    V[k] = vm[x];
+    Nc[k] = nm[x];
   }
   if (iter == 0) {
     // frame start
@@ -615,10 +672,10 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     // finish iteration iter-1, which started from the pose recorded for
     // iter-2 (or from T_prev for iter-1 == 0)
     if (threadIdx.x < 16) {
-      sh.T0[threadIdx.x] = iter == 1 ? st->Tprev[threadIdx.x] : st->iter_T[iter - 2][threadIdx.x];
-      sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
+      sh.T0[threadIdx.x] = T0e;
+      sh.Tp[threadIdx.x] = Tpe;
     }
-    icp_reduce_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(part_in, nb, sh);
+    icp_sum_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(R, nb, sh);
     icp_update_pose<ROWS>(sh, min_inliers);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
   }
@@ -633,13 +690,15 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 
 #pragma unroll
   for (int k = 0; k < PX; ++k)
-    if (base + k * KF_ICP_BLOCK >= n) V[k] = Nc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-  // phase 2: projective association, then the model gathers, issued together
+    if (base + k * KF_ICP_BLOCK >= n) V[k] = Nc[k] = kf_p3{0.f, 0.f, 0.f};
+  // phase 2: projective association, then the model gathers, issued together.
+  // Vz != 0 is the reference's "Vz != 0 and mask" (the vertex map stores 0
+  // where the mask is 0).
   int jx[PX];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
     jx[k] = -1;
-    if (V[k].z != 0 && V[k].w != 0) {
+    if (V[k].z != 0) {
       const float cx = F[0] * V[k].x + F[1] * V[k].y + F[2] * V[k].z + F[3];
       const float cy = F[4] * V[k].x + F[5] * V[k].y + F[6] * V[k].z + F[7];
       const float cz = F[8] * V[k].x + F[9] * V[k].y + F[10] * V[k].z + F[11];
@@ -652,13 +711,22 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     }
   }
   // the model gathers, unconditional (pixel 0 for unassociated pixels, whose
-  // values are not used)
-  float4 S[PX], Sn[PX];
+  // values are not used).  Each empty asm statement takes a pixel's six
+  // gathered values as register inputs, so all 2 PX loads are issued before
+  // the first of them is waited for; without it the compiler moves each load
+  // into the branch that uses it, and the gathers of a thread's pixels
+  // (vertex, then normal after the distance gate) become 2 PX dependent
+  // memory round trips.
+  kf_p3 S[PX], Sn[PX];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
     S[k] = mv[max(jx[k], 0)];
     Sn[k] = mn[max(jx[k], 0)];
   }
+#pragma unroll
+  for (int k = 0; k < PX; ++k)
+    asm volatile("" ::"v"(S[k].x), "v"(S[k].y), "v"(S[k].z), "v"(Sn[k].x), "v"(Sn[k].y),
+                 "v"(Sn[k].z));
   // phase 3: gates, Jacobian rows, accumulation
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
@@ -666,20 +734,23 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     bool ok = false;
     float a[6], b = 0.f;
     if (jx[k] >= 0) {
-      const float4 Vk = V[k];
+      const kf_p3 Vk = V[k];
       const float ux = Te[0] * Vk.x + Te[1] * Vk.y + Te[2] * Vk.z + Te[3];
       const float uy = Te[4] * Vk.x + Te[5] * Vk.y + Te[6] * Vk.z + Te[7];
       const float uz = Te[8] * Vk.x + Te[9] * Vk.y + Te[10] * Vk.z + Te[11];
-      const float4 Sk = S[k];
+      const kf_p3 Sk = S[k];
       const float ddx = ux - Sk.x, ddy = uy - Sk.y, ddz = uz - Sk.z;
-      const float dist = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-      if (dist <= dist_gate && dist >= 0.0f) {
-        const float4 Nk = Nc[k];
+      // sqrtf(d2) <= gate (sensor_pose_estimation.cu:44-48) as d2 <= the
+      // largest float whose correctly rounded square root passes; NaN fails
+      // both forms
+      const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (d2 <= dist2_hi) {
+        const kf_p3 Nk = Nc[k];
         float nnx = Te[0] * Nk.x + Te[1] * Nk.y + Te[2] * Nk.z;
         float nny = Te[4] * Nk.x + Te[5] * Nk.y + Te[6] * Nk.z;
         float nnz = Te[8] * Nk.x + Te[9] * Nk.y + Te[10] * Nk.z;
         normalize3(nnx, nny, nnz);
-        const float4 Snk = Sn[k];
+        const kf_p3 Snk = Sn[k];
         float ax = Snk.x, ay = Snk.y, az = Snk.z;
         normalize3(ax, ay, az);
         const float c = (nnx * ax + nny * ay + nnz * az);
@@ -2454,11 +2525,25 @@ __device__ __forceinline__ void table_flush(const int* key, const float* val, in
   }
 }
 
+// One pixel of the model maps: vertex and normal as packed 3-vectors, the
+// hit index, and in trilinear mode the camera depth and crossing parameter
+// the walk keeps in the w lanes (the adjoint starts from that te).
+template <int MODE>
+__device__ __forceinline__ void store_maps(kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n,
+                                           float2* __restrict__ out_aux,
+                                           int32_t* __restrict__ out_hit, int p, const float4& ov,
+                                           const float4& on, int hit) {
+  out_v[p] = kf_p3{ov.x, ov.y, ov.z};
+  out_n[p] = kf_p3{on.x, on.y, on.z};
+  out_hit[p] = hit;
+  if constexpr (MODE == kModeTrilinear) out_aux[p] = make_float2(ov.w, on.w);
+}
+
 template <int PREC, bool COUNT, int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_raycast(
     const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
-    float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
-    RayState* __restrict__ queue, int tiles_x, int n_blocks) {
+    kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n, float2* __restrict__ out_aux,
+    int32_t* __restrict__ out_hit, RayState* __restrict__ queue, int tiles_x, int n_blocks) {
   const int bid = xcd_contiguous(blockIdx.x, n_blocks);
   const int W = io.W;
   const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -2510,9 +2595,7 @@ __global__ __launch_bounds__(256) void k_raycast(
       queue[(size_t)part * io.qcap + base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
   if (!active || MODE == kModeRefine) return;
-  out_v[p] = ov;
-  out_n[p] = on;
-  out_hit[p] = hit;
+  store_maps<MODE>(out_v, out_n, out_aux, out_hit, p, ov, on, hit);
 }
 
 __device__ __forceinline__ void uniform_ray(RayState& r) {
@@ -2539,8 +2622,8 @@ __device__ __forceinline__ void uniform_ray(RayState& r) {
 template <int PREC, bool COUNT, int MODE>
 __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
-    float4* __restrict__ out_v, float4* __restrict__ out_n, int32_t* __restrict__ out_hit,
-    const RayState* __restrict__ queue) {
+    kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n, float2* __restrict__ out_aux,
+    int32_t* __restrict__ out_hit, const RayState* __restrict__ queue) {
   __shared__ WaveWalkLds lds[4];
   static_assert(KF_RAY_PARTS == 64, "one queue partition per lane");
   const int ln = threadIdx.x & 63;
@@ -2586,11 +2669,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
         wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, px);
     if constexpr (MODE == kModeRefine) gn_add_pixel(px, gn);
     if constexpr (MODE != kModeRefine) {
-      if (lane == owner) {
-        out_v[r.pix] = ov;
-        out_n[r.pix] = on;
-        out_hit[r.pix] = hit;
-      }
+      if (lane == owner) store_maps<MODE>(out_v, out_n, out_aux, out_hit, r.pix, ov, on, hit);
     }
     if constexpr (COUNT) add_counts(st, visited, hit >= 0);
   }
@@ -2607,7 +2686,7 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
 template <int PREC>
 __global__ __launch_bounds__(256) void k_adjoint_saved(
     const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
-    const float4* __restrict__ model_n, const int32_t* __restrict__ hit_map, int tiles_x,
+    const float2* __restrict__ model_aux, const int32_t* __restrict__ hit_map, int tiles_x,
     int n_blocks) {
   const int bid = xcd_contiguous(blockIdx.x, n_blocks);
   const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -2628,7 +2707,7 @@ __global__ __launch_bounds__(256) void k_adjoint_saved(
   float T[12];
   load_uniform<12>(st->T, T);
   const int h = active ? hit_map[p] : -1;
-  const float te = active ? model_n[p].w : 0.0f;
+  const float te = active ? model_aux[p].y : 0.0f;
   if (h >= 0) {
     const float* Ki = io.Ki;
     const float X = (float)x + 1.0f, Y = (float)y + 1.0f;
@@ -2885,7 +2964,7 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
     const void* pin = pbase + (size_t)((it + 1) & 1) * half;
     void* pout = pbase + (size_t)(it & 1) * half;
     k_icp_iter<KAHAN, PX, PPL><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
-        c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->prm.icp_dist_mm,
+        c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->icp_dist2_hi,
         c->icp_cos_lo, min_inl, c->state, pin, pout, m, it);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -3123,8 +3202,9 @@ static int refine_pass1_batches(const kf_context* c, float tmax) {
 
 // Model-map outputs of a raycast (the context's, or the adjoint's own).
 struct RayMaps {
-  float4* v;
-  float4* n;
+  kf_p3* v;
+  kf_p3* n;
+  float2* aux;
   int32_t* hit;
 };
 
@@ -3134,14 +3214,14 @@ static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb
   RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
   if (MODE != kModeRefine && io.pass1_batches > kPass1Batches)
     k_raycast<PREC, COUNT, MODE, true><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, io, c->H, c->state, m.v, m.n, m.hit, q, tiles_x, nb);
+        c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, q, tiles_x, nb);
   else
     k_raycast<PREC, COUNT, MODE, false><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, io, c->H, c->state, m.v, m.n, m.hit, q, tiles_x, nb);
+        c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, q, tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
   k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(
-      c->tsdf, io2, c->state, m.v, m.n, m.hit, q);
+      c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, q);
 }
 
 template <int PREC>
@@ -3161,7 +3241,8 @@ static void launch_raycast_p(kf_context* c, const RayIO& io, int tiles_x, int nb
 // switch (ignored by the modes that do not count).
 static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode,
                                       const RayMaps* maps = nullptr, bool count = true) {
-  const RayMaps m = maps ? *maps : RayMaps{c->model_v, c->model_n, c->hit};
+  const RayMaps m = maps ? *maps : RayMaps{c->model_v, c->model_n, c->model_aux, c->hit};
+  if (!maps && mode != kModeRefine) c->maps_trilinear = mode == kModeTrilinear;
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
   RayIO io = make_ray_io(c, cam);
@@ -3215,20 +3296,20 @@ hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam, bool save
   // Without saved crossings for these inputs, the trilinear raycast first
   // walks every ray into the adjoint's own maps (the context's model maps
   // stay as they are); the adjoint then starts from those crossings.
-  RayMaps m{c->model_v, c->model_n, c->hit};
+  RayMaps m{c->model_v, c->model_n, c->model_aux, c->hit};
   if (!saved) {
-    m = RayMaps{c->adj_map_v, c->adj_map_n, c->adj_map_hit};
+    m = RayMaps{c->adj_map_v, c->adj_map_n, c->adj_map_aux, c->adj_map_hit};
     hipError_t e = launch_raycast_mode(c, cam, kModeTrilinear, &m, false);
     if (e != hipSuccess) return e;
   }
   const int tiles_x = (c->W + 15) / 16, nb = tiles_x * ((c->H + 15) / 16);
   const RayIO io = make_ray_io(c, cam);
   if (c->vol.precision == KF_TSDF_F32)
-    k_adjoint_saved<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state, m.n,
-                                                             m.hit, tiles_x, nb);
+    k_adjoint_saved<KF_TSDF_F32><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
+                                                             m.aux, m.hit, tiles_x, nb);
   else
-    k_adjoint_saved<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state, m.n,
-                                                             m.hit, tiles_x, nb);
+    k_adjoint_saved<KF_TSDF_F16><<<nb, 256, 0, c->stream>>>(c->tsdf, io, c->H, c->state,
+                                                             m.aux, m.hit, tiles_x, nb);
   k_adjoint_finish<<<1, kAdjFinishThreads, 0, c->stream>>>(c->adj_partials, nb, c->state);
   return hipGetLastError();
 }

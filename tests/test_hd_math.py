@@ -139,3 +139,36 @@ def test_gate_thresholds_equal_the_acos_tests(kf, oracle, gate):
         assert oracle.icp_angle_gate(c, gate) == int(lo32 <= c <= 1), c
         assert oracle.normal_flip(c) == int(-1 <= c < hi32), c
         assert bool(icp_pass(c)) == bool(lo32 <= c <= 1), c
+
+
+@pytest.mark.parametrize("gate", [20.0, 100.0, 0.5, 4000.0, 1e-30, 0.0, -1.0, np.inf, np.nan])
+def test_dist_gate_threshold_equals_the_sqrt_test(kf, gate):
+    """The ICP distance gate, sqrtf(d2) <= gate && sqrtf(d2) >= 0
+    (sensor_pose_estimation.cu:47-48), against d2 <= threshold, with numpy's
+    float32 sqrt (IEEE, correctly rounded) as the reference side: at the
+    threshold, one float either side, random squares of distances around
+    the gate, and the special values."""
+    hi = np.float32(kf.dist_gate_threshold(gate))
+    g = np.float32(gate)
+
+    def passes(d2):
+        with np.errstate(invalid="ignore"):
+            d = np.sqrt(np.float32(d2))
+        return bool(d <= g and d >= np.float32(0))
+
+    if not passes(np.float32(0)):
+        assert hi == np.float32(-1)
+    else:
+        assert passes(hi)
+        if np.isfinite(hi):
+            assert not passes(np.nextafter(hi, np.float32(np.inf)))
+    rng = np.random.default_rng(7)
+    scale = np.float32(gate) if np.isfinite(gate) and gate > 0 else np.float32(10.0)
+    d = (rng.uniform(0, 2, 4000) * scale).astype(np.float32)
+    cand = np.concatenate([d * d, [np.float32(0), np.float32(np.inf), np.float32(np.nan)]])
+    if np.isfinite(hi) and hi >= 0:
+        cand = np.concatenate([cand, hi + np.arange(-40, 41) * np.spacing(hi)])
+    cand = cand.astype(np.float32)
+    cand = cand[~(cand < 0)]  # a float sum of squares is >= +0 or NaN
+    for s in cand:
+        assert passes(s) == bool(s <= hi), (gate, s)
