@@ -493,11 +493,11 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   c->n_store = (int64_t)c->n_bricks * 512;
 
   const size_t n = (size_t)height * width;
-  auto icp_blocks_for = [&](int px) {
-    return (int)((n + (size_t)KF_ICP_BLOCK * px - 1) / ((size_t)KF_ICP_BLOCK * px));
-  };
+  // ICP grid: blocks of KF_ICP_BLOCK threads with up to 4 pixels per thread
+  // (KF_ICP_MAX_BLOCKS blocks cover 2^21 pixels at most)
   c->icp_px = 4;
-  c->icp_blocks = icp_blocks_for(4);
+  c->icp_bpx = KF_ICP_BLOCK * 4;
+  c->icp_blocks = (int)((n + c->icp_bpx - 1) / c->icp_bpx);
   c->volume_bytes = (size_t)c->n_store * (vol->precision == KF_TSDF_F32 ? 8 : 4);
   if (c->icp_blocks > KF_ICP_MAX_BLOCKS) {
     delete c;
@@ -510,20 +510,23 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   };
   if (hipGetDevice(&c->device) != hipSuccess) return fail(KF_ERR_HIP, "no HIP device");
   {
-    // ICP: the fewest pixels per thread (1 to 4) that keep the grid within
-    // one block per compute unit; a CU with two blocks would set the time of
-    // every iteration.  Larger images take 4 and up to KF_ICP_MAX_BLOCKS blocks.
+    // ICP: one block per compute unit, the image split evenly over all of
+    // them (a CU with two blocks would set the time of every iteration, and
+    // each block's two waves per SIMD hide each other's VALU latency).  Up to
+    // 4 pixels per thread; larger images keep 4 and take more blocks.
     int n_cu = 0;
     if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device) !=
             hipSuccess ||
         n_cu <= 0)
       n_cu = KF_ICP_MAX_BLOCKS;
-    for (int px = 1; px <= 4; ++px)
-      if (icp_blocks_for(px) <= n_cu) {
-        c->icp_px = px;
-        c->icp_blocks = icp_blocks_for(px);
-        break;
-      }
+    const int nb = (int)std::min<size_t>((size_t)std::min(n_cu, KF_ICP_MAX_BLOCKS),
+                                         (n + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK);
+    const int bpx = (int)((n + nb - 1) / nb);
+    if (bpx <= 4 * KF_ICP_BLOCK) {
+      c->icp_blocks = (int)((n + bpx - 1) / bpx);
+      c->icp_bpx = bpx;
+      c->icp_px = (bpx + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK;
+    }
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(KF_ERR_HIP, "hipStreamCreate failed");

@@ -141,7 +141,8 @@ struct kf_context {
   void* icp_partials = nullptr;    // 2 parities x [KF_ICP_SUMS][blocks] doubles (or
                                    // [2 * KF_ICP_SUMS][blocks] floats, compensated)
   int icp_blocks = 0;
-  int icp_px = 4;                  // pixels per thread
+  int icp_bpx = 0;                 // pixels per block
+  int icp_px = 4;                  // pixels per thread, at most
   uint8_t* icp_masks = nullptr;    // [KF_MAX_ICP_ITERS][H*W] when record_icp
   float icp_cos_lo = 0.f;          // ICP normal gate as a cosine threshold
   float icp_dist2_hi = 0.f;        // ICP distance gate as a squared-distance threshold

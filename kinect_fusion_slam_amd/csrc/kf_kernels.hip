@@ -604,7 +604,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, kf_cam cam, int H, int W,
     float dist2_hi, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
     const void* __restrict__ part_in_, void* __restrict__ part_out_,
-    uint8_t* __restrict__ mask_out, int iter) {
+    uint8_t* __restrict__ mask_out, int iter, int bpx) {
   constexpr int NS = KF_ICP_SUMS;
   constexpr int ROWS = KAHAN ? 2 * NS : NS;
   // block partials: FP64 sums, or FP32 (sum, compensation) pairs when KAHAN
@@ -616,8 +616,11 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   __shared__ PT red_c[KAHAN ? KF_ICP_BLOCK / 64 : 1][NS];
   const int nb = gridDim.x;
 
+  // block b takes pixels [b bpx, (b + 1) bpx) of the image, thread t the
+  // pixels t, t + KF_ICP_BLOCK, ... below bpx (bpx <= PX KF_ICP_BLOCK)
   const int n = H * W;
-  const int base = blockIdx.x * (KF_ICP_BLOCK * PX) + threadIdx.x;
+  const int base = blockIdx.x * bpx + threadIdx.x;
+  const int lim = min(bpx, n - blockIdx.x * bpx) - (int)threadIdx.x;  // k valid: k * KF_ICP_BLOCK < lim
   // phase 1: the previous iteration's block partials and the two poses it
   // needs, then this thread's vertex and normal loads, all issued before
   // the previous iteration is finished (the maps do not depend on the pose).
@@ -690,7 +693,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 
 #pragma unroll
   for (int k = 0; k < PX; ++k)
-    if (base + k * KF_ICP_BLOCK >= n) V[k] = Nc[k] = kf_p3{0.f, 0.f, 0.f};
+    if (k * KF_ICP_BLOCK >= lim) V[k] = Nc[k] = kf_p3{0.f, 0.f, 0.f};
   // phase 2: projective association, then the model gathers, issued together.
   // Vz != 0 is the reference's "Vz != 0 and mask" (the vertex map stores 0
   // where the mask is 0).
@@ -766,7 +769,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
         }
       }
     }
-    if (mask_out && x < n) mask_out[x] = ok ? 1 : 0;
+    if (mask_out && k * KF_ICP_BLOCK < lim) mask_out[x] = ok ? 1 : 0;
     if (ok) {
       int q = 0;
 #pragma unroll
@@ -2965,7 +2968,7 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
     void* pout = pbase + (size_t)(it & 1) * half;
     k_icp_iter<KAHAN, PX, PPL><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
         c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->icp_dist2_hi,
-        c->icp_cos_lo, min_inl, c->state, pin, pout, m, it);
+        c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, c->icp_bpx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
