@@ -202,6 +202,33 @@ int kf_estimate_pose(kf_context* ctx, const float K[9], float T_inout[16]);
 int kf_icp_history(kf_context* ctx, int32_t* n_iters, float* T_iters,
                    double* sums, int32_t* status, uint8_t* masks);
 
+/* ICP without a context and without device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory, such
+ * as code under HIP graph capture or a tight serving loop.  It runs the same
+ * kernels as kf_estimate_pose, on the caller's device arrays of height*width
+ * packed {x, y, z} float triplets (12 bytes per pixel):
+ *   vmap     frame vertices, with z = 0 where the vertex mask is 0 (so a pixel
+ *            takes part exactly where the reference's Vz != 0 && mask holds),
+ *   nmap     frame normals,
+ *   model_v, model_n   the model vertices and normals raycast at T_prev.
+ * T_prev (host, read during the call) is the pose the model maps were
+ * raycast at and the starting estimate; the refined pose (16 floats,
+ * row-major) is written to the device array T_out.  Only the ICP fields of
+ * prm are used (icp_iterations, icp_dist_mm, icp_angle_deg,
+ * icp_compensated, icp_robust, icp_min_inliers).  The work is enqueued on
+ * `stream` (a hipStream_t; NULL for the null stream) and the call returns
+ * without waiting.  The block partials and the pose record live in a
+ * static device array of KF_ICP_STATIC_SCRATCH_BYTES on each GPU, shared by
+ * every call on that GPU and kept until the process ends: calls that
+ * overlap on one GPU race (enqueue them on one stream).  Nothing in it has
+ * to start at zero.  Images of more than 2^20 pixels are rejected
+ * (KF_ERR_INVALID). */
+#define KF_ICP_STATIC_SCRATCH_BYTES (256 * 1024)
+int kf_estimate_pose_static(const float* vmap, const float* nmap, const float* model_v,
+                            const float* model_n, int32_t height, int32_t width,
+                            const float K[9], const kf_params* prm, const float T_prev[16],
+                            float* T_out, void* stream);
+
 /* ---- A8: projective TSDF integration (projective_tsdf.cu:5-214) ---------- */
 int kf_integrate(kf_context* ctx, const float K[9], const float T[16],
                  int32_t first_frame);

@@ -120,6 +120,8 @@ _SIGNATURES = {
     "kf_adjoint_used_saved": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_dist_gate_threshold": (C.c_int, [C.c_float, _F]),
+    "kf_estimate_pose_static": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.c_int32, _F,
+                                          C.POINTER(Params), _F, _P, _P]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
     "kf_mat4_inverse": (C.c_int, [_F, _F]),
     "kf_mat3_inverse": (C.c_int, [_F, _F]),

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -154,6 +155,34 @@ bool params_valid(const kf_params* p, std::string* why) {
     return false;
   }
   return true;
+}
+
+// ICP grid: one block per compute unit, the image split evenly over all of
+// them (a CU with two blocks would set the time of every iteration, and each
+// block's two waves per SIMD hide each other's VALU latency).  Up to 4 pixels
+// per thread; larger images keep 4 and take more blocks (KF_ICP_MAX_BLOCKS
+// blocks cover 2^20 pixels).  The grid fixes the order of the ICP sums.
+void icp_grid(size_t n, int n_cu, int* blocks, int* bpx, int* px) {
+  *px = 4;
+  *bpx = KF_ICP_BLOCK * 4;
+  *blocks = (int)((n + *bpx - 1) / *bpx);
+  const int nb = (int)std::min<size_t>((size_t)std::min(n_cu, KF_ICP_MAX_BLOCKS),
+                                       (n + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK);
+  const int b = (int)((n + nb - 1) / nb);
+  if (b <= 4 * KF_ICP_BLOCK) {
+    *blocks = (int)((n + b - 1) / b);
+    *bpx = b;
+    *px = (b + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK;
+  }
+}
+
+int device_cus(int device) {
+  int n_cu = 0;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+          hipSuccess ||
+      n_cu <= 0)
+    n_cu = KF_ICP_MAX_BLOCKS;
+  return n_cu;
 }
 
 void host_spatial_kernel(int r, float sigma, float* w) {
@@ -493,15 +522,12 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
   c->n_store = (int64_t)c->n_bricks * 512;
 
   const size_t n = (size_t)height * width;
-  // ICP grid: blocks of KF_ICP_BLOCK threads with up to 4 pixels per thread
-  // (KF_ICP_MAX_BLOCKS blocks cover 2^21 pixels at most)
-  c->icp_px = 4;
-  c->icp_bpx = KF_ICP_BLOCK * 4;
-  c->icp_blocks = (int)((n + c->icp_bpx - 1) / c->icp_bpx);
+  // ICP grid before the device is known (the block-count check)
+  icp_grid(n, KF_ICP_MAX_BLOCKS, &c->icp_blocks, &c->icp_bpx, &c->icp_px);
   c->volume_bytes = (size_t)c->n_store * (vol->precision == KF_TSDF_F32 ? 8 : 4);
   if (c->icp_blocks > KF_ICP_MAX_BLOCKS) {
     delete c;
-    return kf_fail(KF_ERR_INVALID, "image too large for the ICP reduction (max 2^21 pixels)");
+    return kf_fail(KF_ERR_INVALID, "image too large for the ICP reduction (max 2^20 pixels)");
   }
 
   auto fail = [&](int code, const std::string& m) {
@@ -509,25 +535,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
     return kf_fail(code, m);
   };
   if (hipGetDevice(&c->device) != hipSuccess) return fail(KF_ERR_HIP, "no HIP device");
-  {
-    // ICP: one block per compute unit, the image split evenly over all of
-    // them (a CU with two blocks would set the time of every iteration, and
-    // each block's two waves per SIMD hide each other's VALU latency).  Up to
-    // 4 pixels per thread; larger images keep 4 and take more blocks.
-    int n_cu = 0;
-    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, c->device) !=
-            hipSuccess ||
-        n_cu <= 0)
-      n_cu = KF_ICP_MAX_BLOCKS;
-    const int nb = (int)std::min<size_t>((size_t)std::min(n_cu, KF_ICP_MAX_BLOCKS),
-                                         (n + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK);
-    const int bpx = (int)((n + nb - 1) / nb);
-    if (bpx <= 4 * KF_ICP_BLOCK) {
-      c->icp_blocks = (int)((n + bpx - 1) / bpx);
-      c->icp_bpx = bpx;
-      c->icp_px = (bpx + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK;
-    }
-  }
+  icp_grid(n, device_cus(c->device), &c->icp_blocks, &c->icp_bpx, &c->icp_px);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(KF_ERR_HIP, "hipStreamCreate failed");
   struct Alloc {
@@ -796,6 +804,97 @@ int kf_icp_history(kf_context* c, int32_t* n_iters, float* T_iters, double* sums
     KF_REQUIRE(c->icp_masks && c->prm.record_icp, KF_ERR_STATE, "record_icp is not enabled");
     KF_HIP_CHECK(hipMemcpy(masks, c->icp_masks, (size_t)n * c->H * c->W, hipMemcpyDeviceToHost));
   }
+  return KF_OK;
+}
+
+}  // extern "C"
+
+// ---- ICP with no device allocation (kf_estimate_pose_static) ---------------
+// Scratch of the allocation-free ICP entry point: the block partials of two
+// iteration parities, then the pose record, one copy per GPU that the
+// library runs on, held until the process ends.  In host code `scratch` is
+// a stand-in for the symbol, not a device address: host code reaches the
+// array only through the per-device address hipGetSymbolAddress reports.
+static constexpr size_t kIcpStaticPartialBytes = (size_t)2 * KF_ICP_MAX_BLOCKS * KF_ICP_SUMS * 8;
+static_assert(kIcpStaticPartialBytes % 256 == 0, "the pose record starts on a 256-byte line");
+static_assert(kIcpStaticPartialBytes + sizeof(kf_dev_state) <= KF_ICP_STATIC_SCRATCH_BYTES,
+              "partials and pose record fit the static scratch");
+static __device__ __align__(256) unsigned char scratch[KF_ICP_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[kMaxDevices] = {};  // the array's address on each device
+
+__global__ void k_pose_out(const kf_dev_state* st, float* T_out) {
+  if (threadIdx.x < 16) T_out[threadIdx.x] = st->T[threadIdx.x];
+}
+
+// The current device's copy of `scratch`, looked up once per device with
+// that device current.
+int scratch_of_current_device(int* device, char** ptr) {
+  KF_HIP_CHECK(hipGetDevice(device));
+  KF_REQUIRE(*device >= 0 && *device < kMaxDevices, KF_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[*device]) {
+    void* p = nullptr;
+    KF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    KF_REQUIRE(p, KF_ERR_HIP, "hipGetSymbolAddress returned no address for the ICP scratch");
+    g_scratch_dev[*device] = static_cast<char*>(p);
+  }
+  *ptr = g_scratch_dev[*device];
+  return KF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int kf_estimate_pose_static(const float* vmap, const float* nmap, const float* model_v,
+                            const float* model_n, int32_t height, int32_t width,
+                            const float K[9], const kf_params* prm, const float T_prev[16],
+                            float* T_out, void* stream) {
+  KF_REQUIRE(vmap && nmap && model_v && model_n && K && prm && T_prev && T_out, KF_ERR_INVALID,
+             "null argument");
+  KF_REQUIRE(height > 1 && width > 1 && (int64_t)height * width < (1ll << 28), KF_ERR_INVALID,
+             "bad image size");
+  KF_REQUIRE(prm->icp_iterations >= 0 && prm->icp_iterations <= KF_MAX_ICP_ITERS,
+             KF_ERR_INVALID, "icp_iterations must be in [0, 64]");
+  KF_REQUIRE((prm->icp_robust == 0 || prm->icp_robust == 1) && prm->icp_min_inliers >= 0,
+             KF_ERR_INVALID, "icp_robust must be 0 or 1 and icp_min_inliers non-negative");
+  int device = 0;
+  char* base = nullptr;
+  int rc = scratch_of_current_device(&device, &base);
+  if (rc != KF_OK) return rc;
+  // a host-side view of the ICP's buffers: the caller's maps, the static
+  // scratch and the caller's stream (nothing here owns device memory)
+  kf_context c;
+  c.device = device;
+  c.H = height;
+  c.W = width;
+  c.prm = *prm;
+  c.prm.record_icp = 0;
+  c.stream = static_cast<hipStream_t>(stream);
+  icp_grid((size_t)height * width, device_cus(device), &c.icp_blocks, &c.icp_bpx, &c.icp_px);
+  KF_REQUIRE(c.icp_blocks <= KF_ICP_MAX_BLOCKS, KF_ERR_INVALID,
+             "image too large for the static ICP scratch (at most 2^20 pixels)");
+  c.vmap = reinterpret_cast<kf_p3*>(const_cast<float*>(vmap));
+  c.nmap = reinterpret_cast<kf_p3*>(const_cast<float*>(nmap));
+  c.model_v = reinterpret_cast<kf_p3*>(const_cast<float*>(model_v));
+  c.model_n = reinterpret_cast<kf_p3*>(const_cast<float*>(model_n));
+  c.icp_partials = base;
+  c.state = reinterpret_cast<kf_dev_state*>(base + kIcpStaticPartialBytes);
+  c.icp_cos_lo = icp_cos_threshold(prm->icp_angle_deg);
+  c.icp_dist2_hi = icp_dist2_threshold(prm->icp_dist_mm);
+  kf_pose_arg a;
+  memcpy(a.T, T_prev, sizeof a.T);
+  KF_REQUIRE(kf_m4_inverse(a.T, a.Tinv) == 0, KF_ERR_INVALID, "pose is singular");
+  kf_cam cam;
+  kf_make_cam(K, &cam);
+  k_set_pose<<<1, 64, 0, c.stream>>>(c.state, a);
+  KF_HIP_CHECK(hipGetLastError());
+  KF_LAUNCH(kf_launch_icp(&c, cam));
+  k_pose_out<<<1, 64, 0, c.stream>>>(c.state, T_out);
+  KF_HIP_CHECK(hipGetLastError());
   return KF_OK;
 }
 

@@ -91,7 +91,7 @@ __host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int
 #endif
 // Block partials one lane reads per row when an iteration is finished: the
 // pixels per thread (1 to 4) are chosen so that at most 64 * 8 blocks cover
-// the image (up to 2^21 pixels).
+// the image (up to 2^20 pixels).
 #define KF_ICP_MAX_PART_PER_LANE 8
 #define KF_ICP_MAX_BLOCKS (64 * KF_ICP_MAX_PART_PER_LANE)
 
