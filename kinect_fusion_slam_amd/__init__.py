@@ -12,7 +12,7 @@ from ._abi import (KF_ICP_STATUS_GATED, KF_MEM_DEVICE, KF_MEM_HOST,  # noqa: F40
                    PRESET_ROOM_5MM_F16, PRESET_ROOM_CPU, PRESET_ROOM_HD_F16, PRESET_SPHERES_128,
                    PRESETS, STAGES, KinfuContext, KinfuError, Params, SynthScene,
                    VolumeDesc, dist_gate_threshold, exported_symbols, gate_thresholds, ldlt6, lib, mat3_inverse,
-                   mat4_inverse, orbit_pose, preset, preset_scene, read_ply, render,
+                   mat4_inverse, orbit_pose, preset, preset_scene, read_depth_file, read_ply, render,
                    spatial_kernel, stream, write_ply)
 
 __all__ = ["KinfuContext", "KinfuError", "VolumeDesc", "Params", "SynthScene", "preset",

@@ -189,6 +189,13 @@ def gate_thresholds(angle_deg):
     return lo.value, hi.value
 
 
+def read_depth_file(path, h, w):
+    """A raw uint16 little-endian depth frame (system_utility.cpp:58-68)."""
+    out = np.zeros((h, w), np.uint16)
+    check(lib.kf_read_depth_file(str(path).encode(), h, w, out.ctypes.data_as(_U16)))
+    return out
+
+
 def dist_gate_threshold(dist_mm):
     """Largest squared distance the ICP distance gate passes (-1: none)."""
     v = C.c_float()

@@ -39,6 +39,7 @@ int KinfuPipeline::init(const kf_volume_desc& vol, const kf_params& prm, const f
   h_ = h;
   w_ = w;
   prm_ = prm;
+  vol_ = vol;
   memcpy(intrinsic_matrix, K, sizeof intrinsic_matrix);
   const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   memcpy(extrinsic_matrix, I, sizeof extrinsic_matrix);
@@ -88,21 +89,11 @@ int KinfuPipeline::IncomingFrameProcessing(int current_frame, int /*per_nth_fram
 }
 
 int KinfuPipeline::download_volume(std::vector<float>& tsdf, std::vector<float>& weight) {
-  kf_volume_desc v;
-  kf_params p;
-  (void)p;
-  (void)v;
-  size_t bytes = 0;
-  void* ptr = nullptr;
-  int rc = kf_volume_device_ptr(ctx_, &ptr, &bytes);
-  if (rc != KF_OK) return rc;
-  kf_params prm;
-  kf_get_params(ctx_, &prm);
-  const size_t n = bytes / 4;  // upper bound; F32 stores 8 B per voxel
+  if (status_ != KF_OK) return status_;
+  const size_t n = (size_t)vol_.dims[0] * vol_.dims[1] * vol_.dims[2];
   tsdf.resize(n);
   weight.resize(n);
-  rc = kf_download_volume(ctx_, tsdf.data(), weight.data());
-  return rc;
+  return kf_download_volume(ctx_, tsdf.data(), weight.data());
 }
 
 }  // namespace Kinfu

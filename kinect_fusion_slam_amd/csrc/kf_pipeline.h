@@ -48,7 +48,11 @@ class KinfuPipeline {
   int width() const { return w_; }
   kf_context* context() { return ctx_; }
   int status() const { return status_; }
+  // The volume in the reference's layout: Nx*Ny*Nz floats each (index
+  // ix*Ny*Nz + iy*Nz + iz), NaN TSDF where never observed, as the
+  // reference's global_tsdf / global_weight_tsdf (kinfu_pipeline.cpp:61-64).
   int download_volume(std::vector<float>& tsdf, std::vector<float>& weight);
+  const kf_volume_desc& volume_desc() const { return vol_; }
   double last_frame_ms() const { return last_ms_; }
 
  private:
@@ -59,6 +63,7 @@ class KinfuPipeline {
   int status_ = KF_OK;
   double last_ms_ = 0.0;
   kf_params prm_{};
+  kf_volume_desc vol_{};
   std::vector<uint16_t> frame_;
 };
 

@@ -124,3 +124,19 @@ def test_ply_writer_roundtrip(kf, tmp_path):
     assert kf.lib.kf_write_ply(str(tmp_path / "no" / "dir.ply").encode(), n,
                                kf._abi.fptr(x), kf._abi.fptr(y), kf._abi.fptr(z),
                                None, None, None, None, 1) == -5
+
+
+def test_host_mirror_shim_loads_and_rejects_bad_arguments(kf):
+    """tests/shim/libkinfu_shim.so (the C entry of the C++ KinfuPipeline
+    mirror, driven on the GPU by test_gpu_host_mirror.py) links the library
+    and rejects a null data directory before creating a context."""
+    import ctypes as C
+    import os
+
+    from conftest import ROOT
+    shim = C.CDLL(os.path.join(ROOT, "tests", "shim", "libkinfu_shim.so"))
+    fn = shim.shim_run_reference_pipeline
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int,
+                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+    assert fn(None, 480, 640, 2000.0, 300.0, 25, 3, None, None, None, 0) == -1
