@@ -11,26 +11,38 @@ timed region.
 Runs, each on a fresh context over the same frames:
   1. FP32 volume, timed                         -> value, ms_per_step
   2. FP16 TSDF + u16 weight, timed              -> fp16
-  3. FP32 with per-stage hipEvents               -> stage_ms, roofline
-  4. FP32 with the raycast work counters on      -> algorithmic bytes
-  5. FP32 with the trilinear raycast, timed      -> trilinear.value, ATE
-  6. the same with stage events, then on the final state, after one untimed
+  3. FP32 with per-stage hipEvents, including the ICP iteration kernels
+     without the finishing kernel              -> stage_ms, roofline
+  4. FP16 with per-stage hipEvents              -> fp16.stage_ms
+  5. FP32 with the raycast work counters on, and the bricks each integration
+     kept read after every frame               -> algorithmic bytes
+  6. FP32 fed from host memory frame by frame (kf_upload_depth_u16 from host
+     then kf_pipeline_incoming on that depth, as the reference loads each
+     frame), timed                             -> host_fed
+  7. FP32 with the trilinear raycast, timed    -> trilinear.value, ATE
+  8. the same with stage events, then on the final state, after one untimed
      pair each, 20 raycast + pose-only adjoint pairs, 10 raycast +
      pose-and-TSDF adjoint pairs (the adjoint from the raycast's saved
      crossings) and 10 such pairs with the depth uploaded again in between
-     (the adjoint walks every ray)               -> adjoint cost
+     (no saved crossings: the adjoint first runs its own trilinear raycast)
+                                                -> adjoint cost
 and the CPU oracle on a bounded sample of the same workload.
 
-Prints one JSON line.  The roofline object is for the raycast stage (the
-largest share of a frame): k_raycast + k_raycast_rest, timed with hipEvents
-on the context stream, against its algorithmic bytes (voxels visited x
-sizeof(TSDF) + hits x 13 taps x sizeof(TSDF) + depth in + 6 maps and the hit
-index out).
+Prints one JSON line.  The roofline object is for the dominant kernel,
+k_icp_iter (10 launches per frame): its algorithmic bytes per launch (every
+pixel's vertex, normal and mask read, and the model vertex and normal
+gathered: 6 x 4 + 1 + 6 x 4 = 49 B per pixel) over its mean launch time,
+measured with one hipEvent pair around a frame's 10 launches on the context
+stream.  Its
+traffic is the HBM bytes per dispatch that rocprofv3's FETCH_SIZE and
+WRITE_SIZE report (profiles/icp_room_640x480_pmc.txt), when that summary is
+present.
 """
 import argparse
 import json
-import statistics
 import os
+import re
+import statistics
 import sys
 import time
 
@@ -41,13 +53,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s full pipeline 640x480 38.9M voxels, fp16 vs fp32 TSDF at ATE<=1mm"
 HBM_PEAK_GBS = 8000.0
-
-# HBM bytes per frame of the raycast stage (k_raycast + k_raycast_rest), from
-# two rocprofv3 --pmc passes over `python3 bench.py --steps 30 --warmup 5
-# --no-cpu` (FETCH_SIZE x 2 + WRITE_SIZE x 1; the factors are what the same
-# counters read for a device-to-device copy of 256 MiB).  The passes and the
-# per-kernel figures are in profiles/raycast_room_640x480_pmc.txt.
-RAYCAST_TRAFFIC_BYTES = 26294477
+ICP_PMC_SUMMARY = os.path.join(ROOT, "profiles", "icp_room_640x480_pmc.txt")
 
 
 def ensure_built():
@@ -63,17 +69,36 @@ def ate_mm(poses, gt):
     return float(np.sqrt((err ** 2).mean()))
 
 
-def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, mode=0,
-            adjoint_reps=0):
-    """One pass over frames[0 .. warmup+steps]; times frames warmup+1 ...
-    With adjoint_reps, then times that many (raycast, pose-only raycast
-    adjoint) pairs and half as many (raycast, adjoint with dL/dTSDF) pairs on
-    the final volume, pose and depth, each after one untimed pair."""
+def icp_traffic():
+    """HBM bytes per k_icp_iter dispatch from the committed rocprofv3 summary
+    (its 'bench-traffic' line: FETCH_SIZE and WRITE_SIZE already scaled by the
+    copy calibration), or None."""
+    try:
+        text = open(ICP_PMC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"^bench-traffic k_icp_iter read_bytes=(\d+) write_bytes=(\d+)", text, re.M)
+    return (int(m.group(1)), int(m.group(2))) if m else None
+
+
+def new_context(kf, precision, timing=False, mode=0):
     vol, prm, _, H, W = kf.preset(kf.PRESET_ROOM)
     vol.precision = precision
     prm.stage_timing = 1 if timing else 0
     prm.raycast_mode = mode
-    ctx = kf.KinfuContext(vol, prm, H, W)
+    return kf.KinfuContext(vol, prm, H, W)
+
+
+def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, mode=0,
+            adjoint_reps=0):
+    """One pass over frames[0 .. warmup+steps]; times frames warmup+1 ...
+    count: raycast work counters on, and the integration's kept bricks read
+    after every timed frame (a host round trip per frame: not timed).  With
+    adjoint_reps, then times that many (raycast, pose-only raycast adjoint)
+    pairs and half as many (raycast, adjoint with dL/dTSDF) pairs on the
+    final volume, pose and depth, each after one untimed pair."""
+    ctx = new_context(kf, precision, timing, mode)
+    _, _, _, H, W = kf.preset(kf.PRESET_ROOM)
     try:
         ctx.stream_upload(frames)
         ctx.pipeline_initial(K, 0)
@@ -81,36 +106,46 @@ def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, 
             ctx.pipeline_incoming(K, f)
         ctx.synchronize()
         ctx.reset_stage_times()
+        work = None
         if count:
             ctx.count_raycast_work(True)
-        t0 = time.perf_counter()
-        for f in range(warmup + 1, warmup + 1 + steps):
-            ctx.pipeline_incoming(K, f)
-        ctx.synchronize()
-        elapsed = time.perf_counter() - t0
+            bricks = groups = 0
+            for f in range(warmup + 1, warmup + 1 + steps):
+                ctx.pipeline_incoming(K, f)
+                g, b = ctx.integration_work()
+                groups += g
+                bricks += b
+            visited, hits = ctx.raycast_work()
+            work = {"visited": visited, "hits": hits, "groups": groups, "bricks": bricks}
+            elapsed = None
+        else:
+            t0 = time.perf_counter()
+            for f in range(warmup + 1, warmup + 1 + steps):
+                ctx.pipeline_incoming(K, f)
+            ctx.synchronize()
+            elapsed = time.perf_counter() - t0
         stages = ctx.stage_times()
-        work = ctx.raycast_work() if count else None
         poses = ctx.poses(warmup + 1 + steps)
         if adjoint_reps:
             T = ctx.pose()
             d = ctx.depth()
             ones = np.ones((H, W), np.float32)
             work = {}
-            for key in ("pose_only", "full", "full_walk"):
+            for key in ("pose_only", "full", "full_no_saved"):
                 # one untimed pair first, then the timed pairs (a full adjoint
                 # also copies dL/dTSDF to the host, outside the timed stage);
-                # full_walk uploads the same depth again between the two,
+                # full_no_saved uploads the same depth again between the two,
                 # which drops the raycast's saved crossings
                 full = key != "pose_only"
                 for rep in range(1 + (adjoint_reps if not full else max(1, adjoint_reps // 2))):
                     if rep == 1:
                         ctx.reset_stage_times()
                     ctx.raycast(K, T)
-                    if key == "full_walk":
+                    if key == "full_no_saved":
                         ctx.upload_depth_f32(d)
                     ctx.raycast_adjoint(K, T, ones, want_tsdf=full)
                 adj = ctx.stage_times()
-                if key != "full_walk":
+                if key != "full_no_saved":
                     work["raycast"] = adj["raycast"][0] / max(adj["raycast"][1], 1)
                 work[key] = adj["raycast_adjoint"][0] / max(adj["raycast_adjoint"][1], 1)
     finally:
@@ -118,24 +153,82 @@ def run_gpu(kf, precision, frames, K, warmup, steps, timing=False, count=False, 
     return elapsed, stages, poses, work
 
 
+def run_host_fed(kf, frames, K, warmup, steps):
+    """FP32, each frame handed over from host memory: kf_upload_depth_u16
+    (pageable host array, KF_MEM_HOST) then kf_pipeline_incoming on that
+    depth (KF_FRAME_CURRENT), as IncomingFrameProcessing loads and uploads
+    every frame (kinfu_pipeline.cpp:128-131).  Returns the elapsed time of
+    the timed frames."""
+    ctx = new_context(kf, kf.KF_TSDF_F32)
+    try:
+        ctx.upload_depth_u16(frames[0])
+        ctx.pipeline_initial(K, kf.KF_FRAME_CURRENT)
+        for f in range(1, warmup + 1):
+            ctx.upload_depth_u16(frames[f])
+            ctx.pipeline_incoming(K, kf.KF_FRAME_CURRENT)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for f in range(warmup + 1, warmup + 1 + steps):
+            ctx.upload_depth_u16(frames[f])
+            ctx.pipeline_incoming(K, kf.KF_FRAME_CURRENT)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+    finally:
+        ctx.close()
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "model": model}
+
+
 def cpu_baseline(kf, repeats=3):
+    """The oracle's frame sequence (C++/OpenMP restatement of the reference
+    kernels) on config 0: the first 10 frames of the same stream into the
+    same volume, in the -O3 x86-64-v3 timing build, at all OpenMP threads and
+    at one thread; median of `repeats` runs each."""
     from oracle import kf_oracle as oracle
     vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM_CPU)
     scene = kf.preset_scene(kf.PRESET_ROOM_CPU)
-    n = 40
+    n = 10
     frames, _ = kf.stream(scene, K, H, W, 0, n)
-    times = []
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        oracle.pipeline(frames, K, vol, prm, reduce_mode=0)
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": round(n / med, 3), "unit": "frames/s", "cores": oracle.max_threads(),
+    n_threads = oracle.max_threads()
+
+    def timed(threads):
+        oracle.set_threads(threads)
+        times, stage = [], np.zeros(6)
+        for _ in range(repeats):
+            ms = np.zeros(6)
+            t0 = time.perf_counter()
+            oracle.pipeline(frames, K, vol, prm, reduce_mode=0, stage_ms=ms, timing_build=True)
+            times.append(time.perf_counter() - t0)
+            stage = ms
+        return statistics.median(times), stage
+
+    med_n, stage_n = timed(n_threads)
+    med_1, _ = timed(1)
+    oracle.set_threads(n_threads)
+    names = ("depth_gate", "bilateral", "vertex", "icp", "integrate", "raycast")
+    return {"value": round(n / med_n, 3), "unit": "frames/s", "cores": n_threads,
             "kind": "port",
-            "sample": f"CPU oracle (C++/OpenMP restatement of the reference kernels, g++ -O2, "
-                      f"FP32): the same stream's first {n} frames, 640x480 into 360x360x300 "
-                      f"@10mm, full frame sequence incl. initial frame; median of {repeats} "
-                      f"runs of {med:.2f} s"}
+            "sample": f"CPU oracle (C++/OpenMP restatement of the reference kernels, FP32): "
+                      f"the stream's first {n} frames (config 0: 640x480 into 360x360x300 "
+                      f"@10mm, full frame sequence incl. the initial frame), median of "
+                      f"{repeats} runs: {med_n:.2f} s at {n_threads} threads, {med_1:.2f} s "
+                      f"at 1 thread",
+            "single_thread_value": round(n / med_1, 3),
+            "flags": "g++ -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off",
+            "stage_ms_per_frame": {k: round(v / n, 2) for k, v in zip(names, stage_n)},
+            "host": host_info()}
 
 
 def copy_calibration():
@@ -189,6 +282,8 @@ def main():
     _, st16, _, _ = run_gpu(kf, kf.KF_TSDF_F16, frames, K, args.warmup, args.steps, timing=True)
     _, _, _, work = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps, count=True)
     print("[bench] stage timing and work counters done", file=sys.stderr, flush=True)
+    elh = run_host_fed(kf, frames, K, args.warmup, args.steps)
+    print(f"[bench] host-fed fp32 {args.steps / elh:.1f} frames/s", file=sys.stderr, flush=True)
     elt, _, ptri, _ = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps,
                               mode=kf.KF_RAYCAST_TRILINEAR)
     _, sttri, _, adj = run_gpu(kf, kf.KF_TSDF_F32, frames, K, args.warmup, args.steps,
@@ -203,41 +298,57 @@ def main():
         return ms / max(cnt, 1) / 1e3
 
     npx = H * W
-    nvox = int(vol.dims[0]) * int(vol.dims[1]) * int(vol.dims[2])
-    visited, hits = work[0] / args.steps, work[1] / args.steps
-    ray_bytes = visited * 4 + hits * 13 * 4 + 8 * npx * 4
-    ray_s = avg_s(st32, "raycast")
-    achieved = ray_bytes / ray_s / 1e9
+    steps = args.steps
+    # dominant kernel: one ICP iteration (10 per frame)
+    icp_iter_bytes = npx * (6 * 4 + 1 + 6 * 4)
+    # the icp_iter stage brackets the iteration kernels of one frame's ICP
+    # with one event pair (events between the kernels would add their own
+    # gaps): its mean over frames divided by the iteration count
+    icp_iter_s = avg_s(st32, "icp_iter") / prm.icp_iterations
+    achieved = icp_iter_bytes / icp_iter_s / 1e9
+    traffic = icp_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": RAYCAST_TRAFFIC_BYTES,
-                "kernel": "raycast stage: k_raycast<F32> + k_raycast_rest<F32>",
-                "algorithmic_bytes": int(ray_bytes), "avg_launch_us": round(ray_s * 1e6, 2),
-                "voxels_visited_per_frame": int(visited), "hits_per_frame": int(hits)}
+                "traffic": (traffic[0] + traffic[1]) if traffic else None,
+                "kernel": "k_icp_iter (FP32 sums, 640x480, 10 launches per frame)",
+                "algorithmic_bytes": icp_iter_bytes, "avg_launch_us": round(icp_iter_s * 1e6, 3),
+                "launches_per_frame": prm.icp_iterations}
+    if traffic:
+        roofline["traffic_read_write"] = list(traffic)
 
     # the other stages against the same peak (algorithmic bytes per launch)
-    icp_iters = prm.icp_iterations
+    visited, hits = work["visited"] / steps, work["hits"] / steps
+    bricks = work["bricks"] / steps
+    ray_bytes = visited * 4 + hits * 13 * 4 + 8 * npx * 4
     other = {
-        "integrate_dense_model": {
-            "bytes": nvox * 16, "us": round(avg_s(st32, "integrate") * 1e6, 2),
-            "note": "dense read-modify-write of every voxel (16 B); the kernel culls bricks "
-                    "outside the view frustum, so it can run above this figure"},
-        "icp": {"bytes": icp_iters * 13 * npx * 4, "us": round(avg_s(st32, "icp") * 1e6, 2)},
+        "raycast": {"bytes": int(ray_bytes), "us": round(avg_s(st32, "raycast") * 1e6, 2),
+                    "voxels_visited_per_frame": int(visited), "hits_per_frame": int(hits),
+                    "note": "k_raycast + k_raycast_rest: voxels visited x 4 B + hits x 13 "
+                            "taps x 4 B + depth in and 6 maps + hit index out"},
+        "integrate": {"bytes": int(bricks * 512 * 16),
+                      "us": round(avg_s(st32, "integrate") * 1e6, 2),
+                      "bricks_kept_per_frame": round(bricks, 1),
+                      "groups_kept_per_frame": round(work["groups"] / steps, 1),
+                      "note": "k_integ_cull + k_integrate: TSDF and weight (4 + 4 B) read and "
+                              "written for every voxel of the bricks that pass the culls"},
+        "icp_stage": {"bytes": prm.icp_iterations * icp_iter_bytes,
+                      "us": round(avg_s(st32, "icp") * 1e6, 2)},
         "vertex": {"bytes": 6 * npx * 4 + npx, "us": round(avg_s(st32, "vertex") * 1e6, 2)},
         "bilateral": {"bytes": 2 * npx * 4, "us": round(avg_s(st32, "bilateral") * 1e6, 2)},
     }
     for v in other.values():
         v["GBs"] = round(v["bytes"] / (v["us"] * 1e-6) / 1e9, 1) if v["us"] else None
+        v["frac"] = round(v["GBs"] / HBM_PEAK_GBS, 5) if v["GBs"] else None
 
     a32, a16 = ate_mm(p32, gt), ate_mm(p16, gt)
     out = {
         "metric": METRIC,
-        "value": round(args.steps / el32, 2),
+        "value": round(steps / el32, 2),
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(el32 / args.steps * 1e3, 4),
+        "ms_per_step": round(el32 / steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -246,17 +357,21 @@ def main():
         "config": {"workload": "640x480 synthetic room stream into 360x360x300 voxels @10mm "
                                "(38.88M), FP32 TSDF+weight, trunc 30mm, bilateral 5x5, "
                                "10 ICP iterations, raycast per frame",
-                   "frames_timed": args.steps, "image": [H, W], "volume": list(vol.dims),
+                   "frames_timed": steps, "image": [H, W], "volume": list(vol.dims),
                    "parallelism": "single GPU"},
-        "fp16": {"value": round(args.steps / el16, 2), "unit": "frames/s",
+        "fp16": {"value": round(steps / el16, 2), "unit": "frames/s",
                  "ratio_vs_fp32": round(el32 / el16, 3), "stage_ms": per_stage(st16)},
         "ate_mm": {"fp32": round(a32, 3), "fp16": round(a16, 3),
                    "fp16_minus_fp32": round(a16 - a32, 3)},
         "stage_ms": per_stage(st32),
+        "host_fed": {"value": round(steps / elh, 2), "unit": "frames/s",
+                     "note": "FP32, each frame uploaded from pageable host memory "
+                             "(kf_upload_depth_u16) before kf_pipeline_incoming on it, as the "
+                             "reference loads every frame; PCIe-inclusive, not the metric"},
         "trilinear": {
             "note": "same stream and precision with the trilinear zero-crossing raycast "
                     "(differentiable mode) instead of the reference interpolation",
-            "value": round(args.steps / elt, 2), "unit": "frames/s",
+            "value": round(steps / elt, 2), "unit": "frames/s",
             "ate_mm": round(ate_mm(ptri, gt), 3), "stage_ms": per_stage(sttri),
             "raycast_forward_ms": round(adj["raycast"], 4),
             "adjoint_pose_only_ms": round(adj["pose_only"], 4),
@@ -264,14 +379,16 @@ def main():
             "adjoint_vs_forward": round(adj["full"] / adj["raycast"], 3),
             "adjoint_pose_only_vs_forward": round(adj["pose_only"] / adj["raycast"], 3),
             "adjoint_note": "the adjoint right after the raycast at the same inputs starts from "
-                            "the raycast's saved per-pixel crossings; with any input changed "
-                            "in between it walks every ray again (walk figures)",
-            "adjoint_walk_pose_and_tsdf_ms": round(adj["full_walk"], 4),
-            "adjoint_walk_vs_forward": round(adj["full_walk"] / adj["raycast"], 3)},
+                            "the raycast's saved per-pixel crossings; with an input changed in "
+                            "between it first runs the trilinear raycast into buffers of its "
+                            "own and starts from those crossings (no_saved figures)",
+            "adjoint_no_saved_pose_and_tsdf_ms": round(adj["full_no_saved"], 4),
+            "adjoint_no_saved_vs_forward": round(adj["full_no_saved"] / adj["raycast"], 3)},
         "roofline": roofline,
         "stage_roofline": other,
     }
     if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(kf)
     print(json.dumps(out), flush=True)
 

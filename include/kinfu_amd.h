@@ -69,7 +69,9 @@ extern "C" {
 #define KF_STAGE_INTEG_PREP 6 /* integration per-pixel table                  */
 #define KF_STAGE_RAYCAST_ADJOINT 7 /* kf_raycast_adjoint kernels              */
 #define KF_STAGE_REFINE 8    /* pose refinement through the trilinear raycast  */
-#define KF_STAGE_COUNT 9
+#define KF_STAGE_ICP_ITER 9  /* the iteration kernels of one ICP, back to back: KF_STAGE_ICP
+                                without its last (finishing) kernel; one count per ICP */
+#define KF_STAGE_COUNT 10
 
 /* Parameter presets for kf_preset(). */
 #define KF_PRESET_REFERENCE 100 /* the reference's compiled-in defaults          */
@@ -326,7 +328,11 @@ int kf_stream_upload_u16(kf_context* ctx, const uint16_t* frames, int32_t n_fram
 int kf_pipeline_initial(kf_context* ctx, const float K[9], int32_t frame);
 /* IncomingFrameProcessing: depth(frame) -> [bilateral] -> vertex/normal ->
  * ICP (updates the pose on the device) -> integrate -> raycast.  Enqueued
- * only; nothing is copied to the host. */
+ * only; nothing is copied to the host.  frame = KF_FRAME_CURRENT (here and in
+ * kf_pipeline_initial) takes the depth last loaded by kf_upload_depth_u16 /
+ * _f32 instead of a stream frame, as the reference loads each frame from the
+ * host (kinfu_pipeline.cpp:128-131); the pose is then not recorded. */
+#define KF_FRAME_CURRENT (-1)
 int kf_pipeline_incoming(kf_context* ctx, const float K[9], int32_t frame);
 /* initial(first) then incoming(first+1 .. first+count-1); enqueued only. */
 int kf_pipeline_run(kf_context* ctx, const float K[9], int32_t first, int32_t count);
@@ -347,6 +353,12 @@ int kf_reset_stage_times(kf_context* ctx);
  * instantiations, so it is left off when timing. */
 int kf_count_raycast_work(kf_context* ctx, int32_t enable);
 int kf_raycast_work(kf_context* ctx, int64_t* voxels_visited, int64_t* hits);
+/* Work of the last integration (no reference counterpart; a measurement
+ * aid): groups of 2x2x2 bricks and 8^3-voxel bricks that passed its culls
+ * (projective_tsdf.cu:5-140 visits every voxel; the culled bricks hold no
+ * voxel that passes the reference's per-voxel tests).  Valid until the next
+ * integration. */
+int kf_integration_work(kf_context* ctx, int64_t* groups_kept, int64_t* bricks_kept);
 
 /* ---- conventions exposed for tests ---------------------------------------
  * The angle gates are evaluated as cosine thresholds that are exactly

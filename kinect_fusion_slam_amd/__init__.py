@@ -6,7 +6,8 @@ as hand-written HIP kernels behind the C ABI in include/kinfu_amd.h.
 This package is the Python view of that ABI (ctypes).  Every stage runs in
 libkinfu_amd.so; importing without the built library fails.
 """
-from ._abi import (KF_ICP_STATUS_GATED, KF_MEM_DEVICE, KF_MEM_HOST,  # noqa: F401
+from ._abi import (KF_FRAME_CURRENT, KF_ICP_STATUS_GATED, KF_MEM_DEVICE,  # noqa: F401
+                   KF_MEM_HOST,
                    KF_RAYCAST_REFERENCE,
                    KF_RAYCAST_TRILINEAR, KF_TSDF_F16, KF_TSDF_F32, PRESET_REFERENCE, PRESET_ROOM,
                    PRESET_ROOM_5MM_F16, PRESET_ROOM_CPU, PRESET_ROOM_HD_F16, PRESET_SPHERES_128,

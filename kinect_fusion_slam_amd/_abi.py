@@ -19,7 +19,8 @@ KF_MAX_ICP_ITERS = 64
 KF_ICP_SUMS = 28
 KF_ICP_STATUS_GATED = 2
 STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep",
-          "raycast_adjoint", "refine")
+          "raycast_adjoint", "refine", "icp_iter")
+KF_FRAME_CURRENT = -1
 KF_RAYCAST_REFERENCE, KF_RAYCAST_TRILINEAR = 0, 1
 PRESET_REFERENCE = 100
 PRESET_ROOM_CPU = 0       # 640x480, 360x360x300 @10mm FP32, 10-frame CPU case
@@ -120,6 +121,7 @@ _SIGNATURES = {
     "kf_adjoint_used_saved": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_dist_gate_threshold": (C.c_int, [C.c_float, _F]),
+    "kf_integration_work": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "kf_estimate_pose_static": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.c_int32, _F,
                                           C.POINTER(Params), _F, _P, _P]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
@@ -561,6 +563,12 @@ class KinfuContext:
 
     def count_raycast_work(self, enable=True):
         check(lib.kf_count_raycast_work(self.h, 1 if enable else 0))
+
+    def integration_work(self):
+        """(groups, bricks) that passed the last integration's culls."""
+        g, b = C.c_int64(), C.c_int64()
+        check(lib.kf_integration_work(self.h, C.byref(g), C.byref(b)))
+        return g.value, b.value
 
     def raycast_work(self):
         v, h = C.c_int64(), C.c_int64()

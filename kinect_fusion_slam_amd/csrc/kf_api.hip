@@ -1305,6 +1305,16 @@ int kf_stream_upload_u16(kf_context* c, const uint16_t* frames, int32_t n_frames
 }
 
 static int load_stream_frame(kf_context* c, int32_t frame) {
+  if (frame == KF_FRAME_CURRENT) {
+    // the depth the caller uploaded (already range-gated)
+    KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+    if (c->prm.bilateral) {
+      kf_stage_begin(c, KF_STAGE_BILATERAL);
+      KF_LAUNCH(kf_launch_bilateral(c));
+      kf_stage_end(c, KF_STAGE_BILATERAL);
+    }
+    return KF_OK;
+  }
   KF_REQUIRE(c->frames && frame >= 0 && frame < c->n_frames, KF_ERR_INVALID,
              "frame outside the uploaded stream");
   const uint16_t* src = c->frames + (size_t)frame * c->H * c->W;
@@ -1323,6 +1333,7 @@ static int load_stream_frame(kf_context* c, int32_t frame) {
 }
 
 static int record_pose(kf_context* c, int32_t frame) {
+  if (frame == KF_FRAME_CURRENT) return KF_OK;
   KF_HIP_CHECK(hipMemcpyAsync(c->poses + (size_t)frame * 16, c->state->T, 64,
                               hipMemcpyDeviceToDevice, c->stream));
   return KF_OK;
@@ -1365,7 +1376,7 @@ int kf_pipeline_incoming(kf_context* c, const float K[9], int32_t frame) {
 }
 
 int kf_pipeline_run(kf_context* c, const float K[9], int32_t first, int32_t count) {
-  KF_REQUIRE(c && K && count > 0, KF_ERR_INVALID, "bad argument");
+  KF_REQUIRE(c && K && count > 0 && first >= 0, KF_ERR_INVALID, "bad argument");
   c->saved_ok = false;
   int rc = kf_pipeline_initial(c, K, first);
   for (int32_t f = first + 1; rc == KF_OK && f < first + count; ++f)
@@ -1428,6 +1439,28 @@ int kf_count_raycast_work(kf_context* c, int32_t enable) {
   KF_HIP_CHECK(hipMemsetAsync(&c->state->ray_steps, 0, 2 * sizeof(unsigned long long), c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   c->count_rays = enable != 0;
+  return KF_OK;
+}
+
+int kf_integration_work(kf_context* c, int64_t* groups_kept, int64_t* bricks_kept) {
+  KF_REQUIRE(c && groups_kept && bricks_kept, KF_ERR_INVALID, "null argument");
+  // the last integration appended to slot (integ_slot ^ 1) of the group and
+  // brick counters; the next integration clears that slot
+  const int s = c->integ_slot ^ 1;
+  const size_t lines = (size_t)KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
+  std::vector<uint32_t> g(lines), b(lines);
+  KF_HIP_CHECK(hipMemcpyAsync(g.data(), c->integ_counts + (size_t)s * lines, lines * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(b.data(), c->integ_counts + (size_t)(2 + s) * lines, lines * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  int64_t ng = 0, nb = 0;
+  for (int p = 0; p < KF_RAY_PARTS; ++p) {
+    ng += g[(size_t)p * KF_RAY_COUNT_STRIDE];
+    nb += b[(size_t)p * KF_RAY_COUNT_STRIDE];
+  }
+  *groups_kept = ng;
+  *bricks_kept = nb;
   return KF_OK;
 }
 

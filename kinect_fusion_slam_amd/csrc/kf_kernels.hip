@@ -2966,11 +2966,13 @@ static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
     uint8_t* m = (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
     const void* pin = pbase + (size_t)((it + 1) & 1) * half;
     void* pout = pbase + (size_t)(it & 1) * half;
+    if (it == 0) kf_stage_begin(c, KF_STAGE_ICP_ITER);
     k_icp_iter<KAHAN, PX, PPL><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
         c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->icp_dist2_hi,
         c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, c->icp_bpx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (it == iters - 1) kf_stage_end(c, KF_STAGE_ICP_ITER);
   }
   if (iters > 0) {
     const void* pin = pbase + (size_t)((iters - 1) & 1) * half;

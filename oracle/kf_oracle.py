@@ -11,6 +11,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkf_oracle.so")
+# the same source built -O3 for x86-64-v3, for the benchmark's CPU timing only
+FAST_LIB_PATH = os.path.join(_HERE, "libkf_oracle_fast.so")
 
 _F = C.POINTER(C.c_float)
 _D = C.POINTER(C.c_double)
@@ -21,10 +23,10 @@ _U8 = C.POINTER(C.c_uint8)
 _U16 = C.POINTER(C.c_uint16)
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is not built (make -C oracle)")
-    lib = C.CDLL(LIB_PATH)
+def _load(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built (make -C oracle)")
+    lib = C.CDLL(path)
     sig = {
         "oracle_range_gate": (None, [_U16, _F, C.c_int64, C.c_float, C.c_float]),
         "oracle_spatial_kernel": (None, [C.c_int, C.c_float, _F]),
@@ -71,6 +73,15 @@ def _load():
 
 
 lib = _load()
+_fast = None
+
+
+def timing_lib():
+    """The -O3 x86-64-v3 build of the same oracle (the CPU baseline's timing)."""
+    global _fast
+    if _fast is None:
+        _fast = _load(FAST_LIB_PATH)
+    return _fast
 
 
 def _f(a):
@@ -90,6 +101,8 @@ def _ivec(v):
 
 def set_threads(n):
     lib.oracle_set_threads(int(n))
+    if _fast is not None:
+        _fast.oracle_set_threads(int(n))
 
 
 def max_threads():
@@ -272,14 +285,17 @@ def normal_flip(c):
     return lib.oracle_normal_flip(float(c))
 
 
-def pipeline(frames, K, vol, prm, reduce_mode=0, stage_ms=None):
+def pipeline(frames, K, vol, prm, reduce_mode=0, stage_ms=None, timing_build=False):
+    """The whole frame sequence (kinfu_pipeline.cpp:91-193) on the CPU;
+    timing_build runs it in the -O3 x86-64-v3 build of the same source."""
+    lib_ = timing_lib() if timing_build else lib
     f = np.ascontiguousarray(frames, np.uint16)
     n, H, W = f.shape
     tsdf, weight = new_volume(vol.dims, "f32")
     poses = np.zeros((n, 16), np.float32)
     ms = np.zeros(6, np.float64)
     r = prm.bilateral_radius if prm.bilateral else 0
-    lib.oracle_pipeline(f.ctypes.data_as(_U16), n, H, W, _f(_c32(K, 9)), _ivec(vol.dims),
+    lib_.oracle_pipeline(f.ctypes.data_as(_U16), n, H, W, _f(_c32(K, 9)), _ivec(vol.dims),
                         _ivec(vol.start), vol.voxel_mm, prm.truncation_mm, prm.sdf_min_mm,
                         prm.sdf_max_mm, prm.depth_min_mm, prm.depth_max_mm, r, prm.sigma_s_px,
                         prm.sigma_r_mm, prm.icp_iterations, prm.icp_dist_mm, prm.icp_angle_deg,
