@@ -68,3 +68,34 @@ def test_host_mirror_from_dat_files_equals_pipeline_run(kf, tmp_path):
     assert_same(tsdf, t, "tsdf")
     assert np.isfinite(w).sum() > 100_000
     assert not np.array_equal(poses[-1], np.eye(4, dtype=np.float32).ravel())
+
+
+def test_host_fed_frames_equal_the_resident_stream(kf):
+    """kf_pipeline_initial / kf_pipeline_incoming with KF_FRAME_CURRENT take
+    the depth last uploaded from host memory (range gate at upload, then the
+    bilateral filter on the float depth) instead of a resident stream frame
+    (gate fused into the filter): the same values, so the pose after every
+    frame and the final volume are bit-exact (room preset, bilateral 5x5)."""
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
+    n = 5
+    frames, gt = kf.stream(kf.preset_scene(kf.PRESET_ROOM), K, H, W, 0, n, threads=16)
+    say(f"[gpu] host-fed pipeline, {n} frames")
+    fed = []
+    with kf.KinfuContext(vol, prm, H, W) as ctx:
+        for f in range(n):
+            ctx.upload_depth_u16(frames[f])
+            if f == 0:
+                ctx.pipeline_initial(K, kf.KF_FRAME_CURRENT)
+            else:
+                ctx.pipeline_incoming(K, kf.KF_FRAME_CURRENT)
+            fed.append(ctx.pose().ravel())
+        t_fed, w_fed = ctx.volume()
+    say("[gpu] resident-stream pipeline")
+    with kf.KinfuContext(vol, prm, H, W) as ctx:
+        ctx.stream_upload(frames)
+        ctx.pipeline_run(K, 0, n)
+        want = ctx.poses(n).reshape(n, 16)
+        t, w = ctx.volume()
+    assert_same(np.array(fed), want, "poses")
+    assert_same(w_fed, w, "weight")
+    assert_same(t_fed, t, "tsdf")
