@@ -58,6 +58,7 @@ extern "C" {
 #define KF_MAX_BILATERAL_RADIUS 8
 #define KF_ICP_SUMS 28 /* 21 unique J^TJ, 6 J^Tr, inlier count          */
 #define KF_MAX_REFINE_STEPS 64
+#define KF_MAX_ICP_LEVELS 3
 
 /* Stage indices for kf_stage_times(). */
 #define KF_STAGE_DEPTH 0     /* uint16 -> float mm + range gate                */
@@ -125,6 +126,16 @@ typedef struct kf_params {
                                unchanged (no reference counterpart)       */
   int32_t icp_min_inliers;  /* icp_robust: inlier count below which an
                                iteration is not applied                    */
+  int32_t icp_levels;       /* 1: the reference's ICP, every iteration at full
+                               resolution (sensor_pose_estimation.cu:339);
+                               2 or 3: coarse to fine, icp_coarse_iters at
+                               half (and quarter) resolution first, then
+                               icp_iterations at full resolution (no
+                               reference counterpart)                      */
+  int32_t icp_coarse_iters[2]; /* iterations at half and quarter resolution */
+  float pyramid_max_step_mm; /* a coarse depth pixel averages its 2x2 block
+                                only where the four depths differ by at most
+                                this, else it is 0                          */
 } kf_params;
 
 /* Raycast hit handling.  REFERENCE: the reference's per-axis interpolation
@@ -203,6 +214,23 @@ int kf_estimate_pose(kf_context* ctx, const float K[9], float T_inout[16]);
 #define KF_ICP_STATUS_GATED 2
 int kf_icp_history(kf_context* ctx, int32_t* n_iters, float* T_iters,
                    double* sums, int32_t* status, uint8_t* masks);
+/* Coarse-to-fine ICP (icp_levels > 1; no reference counterpart).  Level l
+ * has floor(H/2^l) x floor(W/2^l) pixels, each covering a 2x2 block of level
+ * l-1, and intrinsics fx/2, skew/2, (cx + 0.5)/2, fy/2, (cy + 0.5)/2 of level
+ * l-1 (1-indexed pixels).  Its depth is the block mean ((d00 + d01) +
+ * (d10 + d11)) * 0.25 where all four are > 0 and within pyramid_max_step_mm,
+ * else 0; its vertex, normal and mask maps follow raw_vertices.cu:3-51 on
+ * that depth; its model maps are the block mean of the vertices and the
+ * normalised sum of the normals where all four pixels hold a surface
+ * (model_ok), else 0.  The iterations of the levels run coarsest first and
+ * are numbered across levels in kf_icp_history (masks of level l use its
+ * first H_l*W_l bytes).  kf_pyramid_level returns level l (1 or 2) as the
+ * last ICP built it; K (nullable) holds the full-resolution intrinsics on
+ * entry and level l's on return; any pointer may be NULL. */
+int kf_pyramid_level(kf_context* ctx, int32_t level, int32_t* height, int32_t* width,
+                     float K[9], float* depth, float* vx, float* vy, float* vz, float* nx,
+                     float* ny, float* nz, uint8_t* mask, float* mx, float* my, float* mz,
+                     float* mnx, float* mny, float* mnz, uint8_t* model_ok);
 
 /* ICP without a context and without device allocation.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory, such

@@ -52,7 +52,8 @@ class Params(C.Structure):
                 ("stage_timing", C.c_int32), ("raycast_fix_bounds", C.c_int32),
                 ("raycast_mode", C.c_int32), ("refine_steps", C.c_int32),
                 ("refine_max_residual_mm", C.c_float), ("icp_robust", C.c_int32),
-                ("icp_min_inliers", C.c_int32)]
+                ("icp_min_inliers", C.c_int32), ("icp_levels", C.c_int32),
+                ("icp_coarse_iters", C.c_int32 * 2), ("pyramid_max_step_mm", C.c_float)]
 
 
 class SynthScene(C.Structure):
@@ -122,6 +123,9 @@ _SIGNATURES = {
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_dist_gate_threshold": (C.c_int, [C.c_float, _F]),
     "kf_integration_work": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "kf_pyramid_level": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _F,
+                                   _F, _F, _F, _F, _F, _F, _F, _U8, _F, _F, _F, _F, _F, _F,
+                                   _U8]),
     "kf_estimate_pose_static": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.c_int32, _F,
                                           C.POINTER(Params), _F, _P, _P]),
     "kf_expf_abi": (C.c_float, [C.c_float]),
@@ -298,6 +302,16 @@ def stream(scene, K, h, w, first, n, noise_scale=1.0, hole_frac=0.02, max_depth=
     return frames, poses
 
 
+def set_field(struct, name, value):
+    """Sets a structure field; array fields take a sequence."""
+    cur = getattr(struct, name)
+    if isinstance(cur, C.Array):
+        for i, v in enumerate(value):
+            cur[i] = v
+    else:
+        setattr(struct, name, value)
+
+
 # ---------------------------------------------------------------- context
 
 class KinfuContext:
@@ -318,7 +332,7 @@ class KinfuContext:
             if hasattr(vol, k) and k in ("precision", "voxel_mm"):
                 setattr(vol, k, v)
             else:
-                setattr(prm, k, v)
+                set_field(prm, k, v)
         ctx = cls(vol, prm, h, w)
         ctx.K = K
         return ctx
@@ -350,7 +364,7 @@ class KinfuContext:
     def set_params(self, **kw):
         p = self.params
         for k, v in kw.items():
-            setattr(p, k, v)
+            set_field(p, k, v)
         check(lib.kf_set_params(self.h, C.byref(p)))
 
     def synchronize(self):
@@ -563,6 +577,31 @@ class KinfuContext:
 
     def count_raycast_work(self, enable=True):
         check(lib.kf_count_raycast_work(self.h, 1 if enable else 0))
+
+    def pyramid_level(self, level, K):
+        """Level `level` (1 or 2) of the last coarse-to-fine ICP: a dict with
+        H, W, K (that level's intrinsics), the frame's depth, vx .. nz and mask,
+        and the model maps x .. nz with their model_ok flags."""
+        h, w = C.c_int32(), C.c_int32()
+        Kl = _f32(K, 9).copy()
+        check(lib.kf_pyramid_level(self.h, level, C.byref(h), C.byref(w), None, *([None] * 7),
+                                   None, *([None] * 6), None))
+        shape = (h.value, w.value)
+        f = {k: np.zeros(shape, np.float32) for k in ("depth", "vx", "vy", "vz", "nx", "ny",
+                                                      "nz", "x", "y", "z", "mnx", "mny",
+                                                      "mnz")}
+        mask, ok = np.zeros(shape, np.uint8), np.zeros(shape, np.uint8)
+        check(lib.kf_pyramid_level(self.h, level, None, None, fptr(Kl), fptr(f["depth"]),
+                                   *(fptr(f[k]) for k in ("vx", "vy", "vz", "nx", "ny", "nz")),
+                                   mask.ctypes.data_as(_U8),
+                                   *(fptr(f[k]) for k in ("x", "y", "z", "mnx", "mny", "mnz")),
+                                   ok.ctypes.data_as(_U8)))
+        model = {"x": f["x"], "y": f["y"], "z": f["z"], "nx": f["mnx"], "ny": f["mny"],
+                 "nz": f["mnz"], "ok": ok}
+        frame = {k: f[k] for k in ("vx", "vy", "vz", "nx", "ny", "nz")}
+        frame["mask"] = mask
+        return {"H": h.value, "W": w.value, "K": Kl, "depth": f["depth"], "frame": frame,
+                "model": model}
 
     def integration_work(self):
         """(groups, bricks) that passed the last integration's culls."""

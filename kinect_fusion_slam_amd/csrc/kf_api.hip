@@ -154,6 +154,17 @@ bool params_valid(const kf_params* p, std::string* why) {
     *why = "icp_robust must be 0 or 1 and icp_min_inliers non-negative";
     return false;
   }
+  if (p->icp_levels < 1 || p->icp_levels > KF_MAX_ICP_LEVELS || p->icp_coarse_iters[0] < 0 ||
+      p->icp_coarse_iters[1] < 0 || !(p->pyramid_max_step_mm >= 0)) {
+    *why = "icp_levels must be in [1, 3], icp_coarse_iters and pyramid_max_step_mm non-negative";
+    return false;
+  }
+  int total = p->icp_iterations;
+  for (int l = 1; l < p->icp_levels; ++l) total += p->icp_coarse_iters[l - 1];
+  if (total > KF_MAX_ICP_ITERS) {
+    *why = "ICP iterations over all pyramid levels must be at most 64";
+    return false;
+  }
   return true;
 }
 
@@ -323,6 +334,10 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   p.refine_max_residual_mm = 50.0f;
   p.icp_robust = 0;
   p.icp_min_inliers = 0;
+  p.icp_levels = 1;  // the reference's single-resolution ICP
+  p.icp_coarse_iters[0] = 5;
+  p.icp_coarse_iters[1] = 4;
+  p.pyramid_max_step_mm = 30.0f;
   switch (which) {
     case KF_PRESET_REFERENCE: {
       // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,
@@ -436,8 +451,45 @@ int kf_destroy(kf_context* c) {
                   c->adj_map_n, c->adj_map_hit, c->model_aux, c->adj_map_aux};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (kf_pyr_level& P : c->pyr) {
+    void* pb[] = {P.depth, P.vmap, P.nmap, P.mask, P.mv, P.mn, P.ok};
+    for (void* b : pb)
+      if (b) (void)hipFree(b);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return KF_OK;
+}
+
+// Buffers of the coarse ICP levels 1 .. levels-1 (kept once allocated).
+static int alloc_pyramid(kf_context* c, int levels) {
+  int H = c->H, W = c->W;
+  for (int l = 1; l < levels; ++l) {
+    H /= 2;
+    W /= 2;
+    kf_pyr_level& P = c->pyr[l];
+    if (P.depth) continue;
+    KF_REQUIRE(H >= 2 && W >= 2, KF_ERR_INVALID, "image too small for this many ICP levels");
+    const size_t n = (size_t)H * W;
+    P.H = H;
+    P.W = W;
+    icp_grid(n, device_cus(c->device), &P.blocks, &P.bpx, &P.px);
+    struct {
+      void** p;
+      size_t bytes;
+    } a[] = {{(void**)&P.depth, n * 4}, {(void**)&P.vmap, n * 12}, {(void**)&P.nmap, n * 12},
+             {(void**)&P.mask, n},      {(void**)&P.mv, n * 12},   {(void**)&P.mn, n * 12},
+             {(void**)&P.ok, n}};
+    for (auto& b : a)
+      if (hipMalloc(b.p, b.bytes) != hipSuccess) {
+        *b.p = nullptr;
+        for (auto& f : a) {
+          if (*f.p) (void)hipFree(*f.p);
+          *f.p = nullptr;
+        }
+        return kf_fail(KF_ERR_OOM, "cannot allocate the ICP pyramid");
+      }
+  }
   return KF_OK;
 }
 
@@ -479,7 +531,7 @@ static int apply_params(kf_context* c, const kf_params* prm) {
     }
     KF_HIP_CHECK(hipMemset(c->icp_masks, 0, bytes));
   }
-  return KF_OK;
+  return alloc_pyramid(c, prm->icp_levels);
 }
 
 int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf_params* prm,
@@ -643,6 +695,8 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
              kf_integ_group_entries(c->vg) * 4 +
              (size_t)6 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4;
   if (c->icp_masks) b += (size_t)KF_MAX_ICP_ITERS * n;
+  for (const kf_pyr_level& P : c->pyr)
+    if (P.depth) b += (size_t)P.H * P.W * (4 + 12 + 12 + 1 + 12 + 12 + 1);
   if (c->frames) b += (size_t)c->n_frames * n * 2 + (size_t)c->n_frames * 64;
   *bytes = b;
   return KF_OK;
@@ -861,6 +915,8 @@ int kf_estimate_pose_static(const float* vmap, const float* nmap, const float* m
              KF_ERR_INVALID, "icp_iterations must be in [0, 64]");
   KF_REQUIRE((prm->icp_robust == 0 || prm->icp_robust == 1) && prm->icp_min_inliers >= 0,
              KF_ERR_INVALID, "icp_robust must be 0 or 1 and icp_min_inliers non-negative");
+  KF_REQUIRE(prm->icp_levels == 1, KF_ERR_INVALID,
+             "the allocation-free ICP runs one level (icp_levels = 1)");
   int device = 0;
   char* base = nullptr;
   int rc = scratch_of_current_device(&device, &base);
@@ -895,6 +951,49 @@ int kf_estimate_pose_static(const float* vmap, const float* nmap, const float* m
   KF_LAUNCH(kf_launch_icp(&c, cam));
   k_pose_out<<<1, 64, 0, c.stream>>>(c.state, T_out);
   KF_HIP_CHECK(hipGetLastError());
+  return KF_OK;
+}
+
+int kf_pyramid_level(kf_context* c, int32_t level, int32_t* height, int32_t* width, float K[9],
+                     float* depth, float* vx, float* vy, float* vz, float* nx, float* ny,
+                     float* nz, uint8_t* mask, float* mx, float* my, float* mz, float* mnx,
+                     float* mny, float* mnz, uint8_t* model_ok) {
+  KF_REQUIRE(c && level >= 1 && level < KF_MAX_ICP_LEVELS, KF_ERR_INVALID, "bad level");
+  const kf_pyr_level& P = c->pyr[level];
+  KF_REQUIRE(P.depth, KF_ERR_STATE, "the pyramid level is not allocated (icp_levels)");
+  if (height) *height = P.H;
+  if (width) *width = P.W;
+  const size_t n = (size_t)P.H * P.W;
+  std::vector<kf_p3> V(n), N(n), MV(n), MN(n);
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  KF_HIP_CHECK(hipMemcpy(V.data(), P.vmap, n * 12, hipMemcpyDeviceToHost));
+  KF_HIP_CHECK(hipMemcpy(N.data(), P.nmap, n * 12, hipMemcpyDeviceToHost));
+  KF_HIP_CHECK(hipMemcpy(MV.data(), P.mv, n * 12, hipMemcpyDeviceToHost));
+  KF_HIP_CHECK(hipMemcpy(MN.data(), P.mn, n * 12, hipMemcpyDeviceToHost));
+  if (depth) KF_HIP_CHECK(hipMemcpy(depth, P.depth, n * 4, hipMemcpyDeviceToHost));
+  if (vz) KF_HIP_CHECK(hipMemcpy(vz, P.depth, n * 4, hipMemcpyDeviceToHost));
+  if (mask) KF_HIP_CHECK(hipMemcpy(mask, P.mask, n, hipMemcpyDeviceToHost));
+  if (model_ok) KF_HIP_CHECK(hipMemcpy(model_ok, P.ok, n, hipMemcpyDeviceToHost));
+  float* out[10] = {vx, vy, nx, ny, nz, mx, my, mz, mnx, mny};
+  for (size_t i = 0; i < n; ++i) {
+    const float val[10] = {V[i].x, V[i].y, N[i].x, N[i].y, N[i].z,
+                           MV[i].x, MV[i].y, MV[i].z, MN[i].x, MN[i].y};
+    for (int k = 0; k < 10; ++k)
+      if (out[k]) out[k][i] = val[k];
+    if (mnz) mnz[i] = MN[i].z;
+  }
+  if (K) {
+    // the intrinsics of the last ICP's level 0 are not kept: K is the level's
+    // scaling of the K passed in
+    float Kl[9];
+    for (int i = 0; i < 9; ++i) Kl[i] = K[i];
+    for (int l = 0; l < level; ++l) {
+      float Kc[9];
+      kf_pyramid_intrinsics(Kl, Kc);
+      for (int i = 0; i < 9; ++i) Kl[i] = Kc[i];
+    }
+    for (int i = 0; i < 9; ++i) K[i] = Kl[i];
+  }
   return KF_OK;
 }
 

@@ -95,6 +95,20 @@ __host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int
 #define KF_ICP_MAX_PART_PER_LANE 8
 #define KF_ICP_MAX_BLOCKS (64 * KF_ICP_MAX_PART_PER_LANE)
 
+// A coarse level of the opt-in ICP pyramid (levels 1 .. icp_levels-1): its
+// image and ICP grid, the frame's depth, vertex, normal and mask maps, and
+// the model maps with their surface flags.
+struct kf_pyr_level {
+  int H = 0, W = 0, blocks = 0, bpx = 0, px = 0;
+  float* depth = nullptr;
+  kf_p3* vmap = nullptr;
+  kf_p3* nmap = nullptr;
+  uint8_t* mask = nullptr;
+  kf_p3* mv = nullptr;
+  kf_p3* mn = nullptr;
+  uint8_t* ok = nullptr;
+};
+
 struct kf_context {
   int device = 0;
   int H = 0, W = 0;
@@ -146,6 +160,7 @@ struct kf_context {
   uint8_t* icp_masks = nullptr;    // [KF_MAX_ICP_ITERS][H*W] when record_icp
   float icp_cos_lo = 0.f;          // ICP normal gate as a cosine threshold
   float icp_dist2_hi = 0.f;        // ICP distance gate as a squared-distance threshold
+  kf_pyr_level pyr[KF_MAX_ICP_LEVELS]; // [1 ..] coarse levels (allocated when enabled)
   float flip_cos_hi = 0.f;         // raycast normal flip threshold
 
   // stream of frames
@@ -199,6 +214,8 @@ hipError_t kf_launch_bilateral(kf_context* c);
 hipError_t kf_launch_depth_bilateral(kf_context* c, const uint16_t* src);
 hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam);
+// Intrinsics of the next coarser pyramid level (K and Kc: 3x3 row-major).
+void kf_pyramid_intrinsics(const float* K, float* Kc);
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first);
 hipError_t kf_launch_raycast(kf_context* c, const kf_cam& cam);

@@ -472,6 +472,61 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ d, kf_
 }
 
 // ---------------------------------------------------------------------------
+// Coarse-to-fine pyramid for ICP (opt-in; the reference runs every iteration
+// at full resolution, sensor_pose_estimation.cu:339).  Level l+1 has
+// floor(H/2) x floor(W/2) pixels; its pixel (x, y) covers pixels (2x + i,
+// 2y + j), i, j in {0, 1}, of level l.
+
+// Depth: the mean of the block, ((d00 + d01) + (d10 + d11)) * 0.25, where all
+// four are > 0 and max - min <= max_step; 0 elsewhere (NaN fails both tests).
+__global__ __launch_bounds__(256) void k_depth_down(const float* __restrict__ d_in, int Wi,
+                                                    float* __restrict__ d_out, int Ho, int Wo,
+                                                    float max_step) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= Ho * Wo) return;
+  const int y = i / Wo, x = i - y * Wo;
+  const float* r0 = d_in + (size_t)(2 * y) * Wi + 2 * x;
+  const float* r1 = r0 + Wi;
+  const float a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+  const float lo = fminf(fminf(a, b), fminf(c, d)), hi = fmaxf(fmaxf(a, b), fmaxf(c, d));
+  const bool ok = a > 0.0f && b > 0.0f && c > 0.0f && d > 0.0f && hi - lo <= max_step;
+  d_out[i] = ok ? ((a + b) + (c + d)) * 0.25f : 0.0f;
+}
+
+// Model maps: where all four pixels hold a surface (hit index >= 0 at level
+// 0, ok set at coarser levels), the mean vertex ((v00 + v01) + (v10 + v11)) *
+// 0.25 and the normalised sum of the normals (normalize3); elsewhere zeros
+// and ok = 0, as a raycast pixel without a hit.
+__global__ __launch_bounds__(256) void k_model_down(
+    const kf_p3* __restrict__ v_in, const kf_p3* __restrict__ n_in,
+    const int32_t* __restrict__ hit_in, const uint8_t* __restrict__ ok_in, int Wi,
+    kf_p3* __restrict__ v_out, kf_p3* __restrict__ n_out, uint8_t* __restrict__ ok_out, int Ho,
+    int Wo) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= Ho * Wo) return;
+  const int y = i / Wo, x = i - y * Wo;
+  const size_t p[4] = {(size_t)(2 * y) * Wi + 2 * x, (size_t)(2 * y) * Wi + 2 * x + 1,
+                       (size_t)(2 * y + 1) * Wi + 2 * x, (size_t)(2 * y + 1) * Wi + 2 * x + 1};
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) ok = ok && (hit_in ? hit_in[p[k]] >= 0 : ok_in[p[k]] != 0);
+  kf_p3 V{0.f, 0.f, 0.f}, N{0.f, 0.f, 0.f};
+  if (ok) {
+    const kf_p3 v0 = v_in[p[0]], v1 = v_in[p[1]], v2 = v_in[p[2]], v3 = v_in[p[3]];
+    const kf_p3 n0 = n_in[p[0]], n1 = n_in[p[1]], n2 = n_in[p[2]], n3 = n_in[p[3]];
+    V = kf_p3{((v0.x + v1.x) + (v2.x + v3.x)) * 0.25f, ((v0.y + v1.y) + (v2.y + v3.y)) * 0.25f,
+              ((v0.z + v1.z) + (v2.z + v3.z)) * 0.25f};
+    float nx = (n0.x + n1.x) + (n2.x + n3.x), ny = (n0.y + n1.y) + (n2.y + n3.y),
+          nz = (n0.z + n1.z) + (n2.z + n3.z);
+    normalize3(nx, ny, nz);
+    N = kf_p3{nx, ny, nz};
+  }
+  v_out[i] = V;
+  n_out[i] = N;
+  ok_out[i] = ok ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
 // A5-A7  ICP.
 
 // One kernel per iteration, and no cross-block synchronisation inside a
@@ -597,14 +652,16 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 }
 
 // Per-pixel association and gates (sensor_pose_estimation.cu:16-136) plus the
-// block reduction, for iteration `iter`.
+// block reduction, for iteration `iter`.  nb_in is the grid of the iteration
+// before (the row stride of its partials): the levels of the opt-in
+// coarse-to-fine pyramid have grids of their own.
 template <bool KAHAN, int PX, int PPL>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
     const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, kf_cam cam, int H, int W,
     float dist2_hi, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
     const void* __restrict__ part_in_, void* __restrict__ part_out_,
-    uint8_t* __restrict__ mask_out, int iter, int bpx) {
+    uint8_t* __restrict__ mask_out, int iter, int bpx, int nb_in) {
   constexpr int NS = KF_ICP_SUMS;
   constexpr int ROWS = KAHAN ? 2 * NS : NS;
   // block partials: FP64 sums, or FP32 (sum, compensation) pairs when KAHAN
@@ -642,7 +699,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   const KF_GLOBAL kf_p3* vm = (const KF_GLOBAL kf_p3*)vm_;
   const KF_GLOBAL kf_p3* nm = (const KF_GLOBAL kf_p3*)nm_;
   IcpPartialRegs<ROWS, KF_ICP_BLOCK, PPL, PT> R;
-  icp_load_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(pin, nb, R);
+  icp_load_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(pin, nb_in, R);
   const int e16 = threadIdx.x & 15;
   const float T0e = iter <= 1 ? stp->Tprev[e16] : stp->iter_T[iter - 2][e16];
   const float Tpe = stp->Tprev_inv[e16];
@@ -678,7 +735,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       sh.T0[threadIdx.x] = T0e;
       sh.Tp[threadIdx.x] = Tpe;
     }
-    icp_sum_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(R, nb, sh);
+    icp_sum_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(R, nb_in, sh);
     icp_update_pose<ROWS>(sh, min_inliers);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
   }
@@ -2951,59 +3008,134 @@ hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam) {
   return hipGetLastError();
 }
 
+// One level of the ICP: its image, grid, camera and maps, and iterations.
+struct IcpLevel {
+  int H, W, blocks, bpx, px, iters;
+  kf_cam cam;
+  const kf_p3 *vmap, *nmap, *mv, *mn;
+};
+
 template <bool KAHAN, int PX, int PPL>
-static hipError_t launch_icp_t(kf_context* c, const kf_cam& cam) {
-  const int iters = c->prm.icp_iterations;
+static void launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin, void* pout,
+                               uint8_t* m, int it, int nb_in, float min_inl) {
+  k_icp_iter<KAHAN, PX, PPL><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(
+      L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W, c->icp_dist2_hi, c->icp_cos_lo, min_inl,
+      c->state, pin, pout, m, it, L.bpx, nb_in);
+}
+
+// The iterations of every level, coarsest first (lv[n_levels - 1] ... lv[0]),
+// numbered 0, 1, ... across levels (the per-iteration record), then the
+// finishing kernel.  Each iteration finishes the one before, whatever its
+// level, from that one's partials (grid nb_in).
+template <bool KAHAN, int PPL>
+static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_levels) {
   const int n = c->H * c->W;
-  const int nb = c->icp_blocks;
   // one parity of partials: KF_ICP_SUMS doubles, or 2 * KF_ICP_SUMS floats
-  // (compensated), per block
-  const size_t half = (size_t)KF_ICP_SUMS * 8 * nb;
+  // (compensated), per block of the largest grid (level 0)
+  const size_t half = (size_t)KF_ICP_SUMS * 8 * c->icp_blocks;
   char* const pbase = reinterpret_cast<char*>(c->icp_partials);
   // robust update: the inlier threshold (>= 0); the reference's update: -1
   const float min_inl = c->prm.icp_robust ? (float)c->prm.icp_min_inliers : -1.0f;
-  for (int it = 0; it < iters; ++it) {
-    uint8_t* m = (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
-    const void* pin = pbase + (size_t)((it + 1) & 1) * half;
-    void* pout = pbase + (size_t)(it & 1) * half;
-    if (it == 0) kf_stage_begin(c, KF_STAGE_ICP_ITER);
-    k_icp_iter<KAHAN, PX, PPL><<<nb, KF_ICP_BLOCK, 0, c->stream>>>(
-        c->vmap, c->nmap, c->model_v, c->model_n, cam, c->H, c->W, c->icp_dist2_hi,
-        c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, c->icp_bpx);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (it == iters - 1) kf_stage_end(c, KF_STAGE_ICP_ITER);
+  int total = 0;
+  for (int l = 0; l < n_levels; ++l) total += lv[l].iters;
+  int it = 0, nb_prev = lv[n_levels - 1].blocks;
+  for (int l = n_levels - 1; l >= 0; --l) {
+    const IcpLevel& L = lv[l];
+    for (int k = 0; k < L.iters; ++k, ++it) {
+      uint8_t* m =
+          (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
+      const void* pin = pbase + (size_t)((it + 1) & 1) * half;
+      void* pout = pbase + (size_t)(it & 1) * half;
+      if (it == 0) kf_stage_begin(c, KF_STAGE_ICP_ITER);
+      switch (L.px) {
+        case 1:
+          launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          break;
+        case 2:
+          launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          break;
+        case 3:
+          launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          break;
+        default:
+          launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          break;
+      }
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      if (it == total - 1) kf_stage_end(c, KF_STAGE_ICP_ITER);
+      nb_prev = L.blocks;
+    }
   }
-  if (iters > 0) {
-    const void* pin = pbase + (size_t)((iters - 1) & 1) * half;
-    k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb,
-                                                                      iters - 1, min_inl);
+  if (total > 0) {
+    const void* pin = pbase + (size_t)((total - 1) & 1) * half;
+    k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb_prev,
+                                                                      total - 1, min_inl);
   }
   return hipGetLastError();
 }
 
-// Partials per lane of the iteration prologue: 4 covers up to 256 blocks
-// (one per compute unit on this GPU), 8 the maximum block count.
-template <int PX>
-static hipError_t launch_icp_px(kf_context* c, const kf_cam& cam) {
-  const bool kahan = c->prm.icp_compensated != 0;
-  if (c->icp_blocks <= 4 * 64)
-    return kahan ? launch_icp_t<true, PX, 4>(c, cam) : launch_icp_t<false, PX, 4>(c, cam);
-  return kahan ? launch_icp_t<true, PX, KF_ICP_MAX_PART_PER_LANE>(c, cam)
-               : launch_icp_t<false, PX, KF_ICP_MAX_PART_PER_LANE>(c, cam);
+// Intrinsics of the next coarser level.  A pixel u (1-indexed) of the coarse
+// image covers the fine pixels 2u - 1 and 2u, whose centre is at fine
+// coordinate 2u - 0.5: u_coarse = (u_fine + 0.5) / 2, so fx and the skew
+// halve and cx becomes (cx + 0.5) / 2 (likewise in y).
+void kf_pyramid_intrinsics(const float* K, float* Kc) {
+  for (int i = 0; i < 9; ++i) Kc[i] = K[i];
+  Kc[0] = K[0] * 0.5f;
+  Kc[1] = K[1] * 0.5f;
+  Kc[2] = (K[2] + 0.5f) * 0.5f;
+  Kc[4] = K[4] * 0.5f;
+  Kc[5] = (K[5] + 0.5f) * 0.5f;
+}
+
+// Levels 1 .. icp_levels-1 from level 0: depth, vertex and normal maps of
+// the frame, and the model maps (from the raycast's maps and hit index).
+static hipError_t build_pyramid(kf_context* c, const kf_cam& cam0, IcpLevel* lv) {
+  const float step = c->prm.pyramid_max_step_mm;
+  float K[9];
+  for (int i = 0; i < 9; ++i) K[i] = cam0.K[i];
+  for (int l = 1; l < c->prm.icp_levels; ++l) {
+    kf_pyr_level& P = c->pyr[l];
+    const int Wi = l == 1 ? c->W : c->pyr[l - 1].W;
+    const float* d_in = l == 1 ? c->depth : c->pyr[l - 1].depth;
+    const kf_p3* v_in = l == 1 ? c->model_v : c->pyr[l - 1].mv;
+    const kf_p3* n_in = l == 1 ? c->model_n : c->pyr[l - 1].mn;
+    const int nl = P.H * P.W, grid = (nl + 255) / 256;
+    k_depth_down<<<grid, 256, 0, c->stream>>>(d_in, Wi, P.depth, P.H, P.W, step);
+    float Kl[9];
+    kf_pyramid_intrinsics(K, Kl);
+    for (int i = 0; i < 9; ++i) K[i] = Kl[i];
+    kf_cam cam;
+    kf_make_cam(Kl, &cam);
+    k_vertex<<<grid, 256, 0, c->stream>>>(P.depth, cam, P.H, P.W, P.vmap, P.nmap, P.mask);
+    k_model_down<<<grid, 256, 0, c->stream>>>(v_in, n_in, l == 1 ? c->hit : nullptr,
+                                              l == 1 ? nullptr : c->pyr[l - 1].ok, Wi, P.mv,
+                                              P.mn, P.ok, P.H, P.W);
+    lv[l] = IcpLevel{P.H, P.W, P.blocks, P.bpx, P.px, c->prm.icp_coarse_iters[l - 1],
+                     cam,  P.vmap, P.nmap, P.mv, P.mn};
+  }
+  return hipGetLastError();
 }
 
 hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
-  switch (c->icp_px) {
-    case 1:
-      return launch_icp_px<1>(c, cam);
-    case 2:
-      return launch_icp_px<2>(c, cam);
-    case 3:
-      return launch_icp_px<3>(c, cam);
-    default:
-      return launch_icp_px<4>(c, cam);
+  IcpLevel lv[KF_MAX_ICP_LEVELS];
+  lv[0] = IcpLevel{c->H, c->W, c->icp_blocks, c->icp_bpx, c->icp_px, c->prm.icp_iterations,
+                   cam,  c->vmap, c->nmap, c->model_v, c->model_n};
+  const int n_levels = c->prm.icp_levels > 1 ? c->prm.icp_levels : 1;
+  if (n_levels > 1) {
+    if (!c->pyr[n_levels - 1].depth) return hipErrorInvalidValue;
+    hipError_t e = build_pyramid(c, cam, lv);
+    if (e != hipSuccess) return e;
   }
+  // partials per lane of the iteration prologue: 4 covers up to 256 blocks
+  // (one per compute unit on this GPU), 8 the maximum block count; level 0
+  // has the most blocks
+  const bool kahan = c->prm.icp_compensated != 0;
+  if (c->icp_blocks <= 4 * 64)
+    return kahan ? launch_icp_levels<true, 4>(c, lv, n_levels)
+                 : launch_icp_levels<false, 4>(c, lv, n_levels);
+  return kahan ? launch_icp_levels<true, KF_ICP_MAX_PART_PER_LANE>(c, lv, n_levels)
+               : launch_icp_levels<false, KF_ICP_MAX_PART_PER_LANE>(c, lv, n_levels);
 }
 
 static int integ_groups(const kf_vol& vg, int* gx, int* gy, int* gz) {

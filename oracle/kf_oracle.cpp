@@ -164,6 +164,66 @@ void oracle_vertex_normals(const float* d, const float* Ki, int H, int W, float*
   }
 }
 
+// ---------------------------------------------------------------------------
+// Coarse-to-fine ICP pyramid.  No reference counterpart: the reference runs
+// every ICP iteration at full resolution (sensor_pose_estimation.cu:339).
+// Level l+1 has floor(H/2) x floor(W/2) pixels; its pixel (x, y) covers
+// pixels (2x + i, 2y + j), i, j in {0, 1}, of level l.  Vertex and normal
+// maps of a level are oracle_vertex_normals on its depth with its K^-1.
+
+// Depth: the block mean ((d00 + d01) + (d10 + d11)) * 0.25 where all four
+// depths are > 0 and max - min <= max_step, else 0.
+void oracle_pyramid_depth(const float* d, int H, int W, float max_step, float* out) {
+  const int Ho = H / 2, Wo = W / 2;
+  for (int y = 0; y < Ho; ++y)
+    for (int x = 0; x < Wo; ++x) {
+      const float a = d[(int64_t)(2 * y) * W + 2 * x], b = d[(int64_t)(2 * y) * W + 2 * x + 1];
+      const float c = d[(int64_t)(2 * y + 1) * W + 2 * x];
+      const float e = d[(int64_t)(2 * y + 1) * W + 2 * x + 1];
+      const float lo = std::min(std::min(a, b), std::min(c, e));
+      const float hi = std::max(std::max(a, b), std::max(c, e));
+      const bool ok = a > 0.0f && b > 0.0f && c > 0.0f && e > 0.0f && hi - lo <= max_step;
+      out[(int64_t)y * Wo + x] = ok ? ((a + b) + (c + e)) * 0.25f : 0.0f;
+    }
+}
+
+// Model maps: where all four pixels hold a surface (ok_in), the block mean
+// of the vertices and the sum of the normals scaled by 1 / sqrtf(|sum|^2)
+// (the normalisation of cuda_functions.cuh:2-10); elsewhere zeros, ok 0.
+void oracle_pyramid_model(const float* const in[6], const uint8_t* ok_in, int H, int W,
+                          float* const out[6], uint8_t* ok_out) {
+  const int Ho = H / 2, Wo = W / 2;
+  for (int y = 0; y < Ho; ++y)
+    for (int x = 0; x < Wo; ++x) {
+      const int64_t o = (int64_t)y * Wo + x;
+      const int64_t p[4] = {(int64_t)(2 * y) * W + 2 * x, (int64_t)(2 * y) * W + 2 * x + 1,
+                            (int64_t)(2 * y + 1) * W + 2 * x, (int64_t)(2 * y + 1) * W + 2 * x + 1};
+      const bool ok = ok_in[p[0]] && ok_in[p[1]] && ok_in[p[2]] && ok_in[p[3]];
+      ok_out[o] = ok ? 1 : 0;
+      for (int k = 0; k < 6; ++k) out[k][o] = 0.0f;
+      if (!ok) continue;
+      for (int k = 0; k < 3; ++k)
+        out[k][o] = ((in[k][p[0]] + in[k][p[1]]) + (in[k][p[2]] + in[k][p[3]])) * 0.25f;
+      float n[3];
+      for (int k = 0; k < 3; ++k)
+        n[k] = (in[3 + k][p[0]] + in[3 + k][p[1]]) + (in[3 + k][p[2]] + in[3 + k][p[3]]);
+      const float t = 1 / sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int k = 0; k < 3; ++k) out[3 + k][o] = n[k] * t;
+    }
+}
+
+// Intrinsics of the next coarser level: a coarse pixel u (1-indexed) covers
+// the fine pixels 2u - 1 and 2u, centred at 2u - 0.5, so u_coarse =
+// (u_fine + 0.5) / 2: fx and the skew halve, cx -> (cx + 0.5) / 2 (and y).
+void oracle_pyramid_intrinsics(const float* K, float* Kc) {
+  for (int i = 0; i < 9; ++i) Kc[i] = K[i];
+  Kc[0] = K[0] * 0.5f;
+  Kc[1] = K[1] * 0.5f;
+  Kc[2] = (K[2] + 0.5f) * 0.5f;
+  Kc[4] = K[4] * 0.5f;
+  Kc[5] = (K[5] + 0.5f) * 0.5f;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -63,6 +63,10 @@ def _load(path=LIB_PATH):
         "oracle_f32_to_f16": (None, [_F, _U16, C.c_int64]),
         "oracle_f16_to_f32": (None, [_U16, _F, C.c_int64]),
         "oracle_max_threads": (C.c_int, []),
+        "oracle_pyramid_depth": (None, [_F, C.c_int, C.c_int, C.c_float, _F]),
+        "oracle_pyramid_model": (None, [C.POINTER(_F), _U8, C.c_int, C.c_int, C.POINTER(_F),
+                                        _U8]),
+        "oracle_pyramid_intrinsics": (None, [_F, _F]),
         "oracle_set_threads": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -304,3 +308,72 @@ def pipeline(frames, K, vol, prm, reduce_mode=0, stage_ms=None, timing_build=Fal
     if stage_ms is not None:
         stage_ms[:] = ms
     return poses.reshape(n, 4, 4), tsdf, weight
+
+
+# ---------------------------------------------------------------- ICP pyramid
+# No reference counterpart: the reference's ICP runs at full resolution only
+# (sensor_pose_estimation.cu:339).
+
+def pyramid_depth(depth, max_step):
+    d = _c32(depth)
+    H, W = d.shape
+    out = np.zeros((H // 2, W // 2), np.float32)
+    lib.oracle_pyramid_depth(_f(d), H, W, max_step, _f(out))
+    return out
+
+
+def pyramid_model(model, ok):
+    """model: dict x, y, z, nx, ny, nz (H x W); ok: H x W (1 = surface)."""
+    keys = ("x", "y", "z", "nx", "ny", "nz")
+    ins = [_c32(model[k]) for k in keys]
+    H, W = ins[0].shape
+    o = np.ascontiguousarray(ok, np.uint8)
+    outs = [np.zeros((H // 2, W // 2), np.float32) for _ in keys]
+    ok_out = np.zeros((H // 2, W // 2), np.uint8)
+    lib.oracle_pyramid_model((_F * 6)(*(_f(a) for a in ins)), o.ctypes.data_as(_U8), H, W,
+                             (_F * 6)(*(_f(a) for a in outs)), ok_out.ctypes.data_as(_U8))
+    res = dict(zip(keys, outs))
+    res["ok"] = ok_out
+    return res
+
+
+def pyramid_intrinsics(K):
+    out = np.zeros(9, np.float32)
+    lib.oracle_pyramid_intrinsics(_f(_c32(K, 9)), _f(out))
+    return out
+
+
+def pyramid_levels(depth, model, hit, K, Kinv_of, levels, max_step):
+    """Levels 0 .. levels-1: [(K_l, frame maps dict incl. vz and mask, model
+    maps dict incl. ok)], level 0 from the full-resolution inputs."""
+    fr = vertex_normals(depth, Kinv_of(K))
+    fr["vz"] = _c32(depth)
+    mo = dict(model)
+    mo["ok"] = (np.asarray(hit) >= 0).astype(np.uint8)
+    out = [(np.asarray(K, np.float32).reshape(9), fr, mo)]
+    d = _c32(depth)
+    for _ in range(1, levels):
+        Kl = pyramid_intrinsics(out[-1][0])
+        d = pyramid_depth(d, max_step)
+        fr = vertex_normals(d, Kinv_of(Kl))
+        fr["vz"] = d
+        mo = pyramid_model(out[-1][2], out[-1][2]["ok"])
+        out.append((Kl, fr, mo))
+    return out
+
+
+def icp_pyramid(levels, T_prev, iters, dist_gate, angle_gate, reduce_mode=1):
+    """Coarse to fine: iters[l] iterations at level l (coarsest first), each
+    level starting from the pose the level before ended with; the model maps
+    of every level were raycast at T_prev."""
+    T = np.asarray(T_prev, np.float32).reshape(4, 4)
+    record = []
+    for l in range(len(levels) - 1, -1, -1):
+        K, fr, mo = levels[l]
+        if iters[l] == 0:
+            continue
+        r = icp(fr, mo, K, T_prev, iters[l], dist_gate, angle_gate, reduce_mode=reduce_mode,
+                T_start=T)
+        record.append((l, r))
+        T = r["T"]
+    return T, record
