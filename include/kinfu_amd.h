@@ -330,7 +330,7 @@ int kf_raycast_adjoint(kf_context* ctx, const float K[9], const float T[16],
                        const float* dl_ddepth, const float* dl_dnormal, float* dl_dtsdf,
                        double dl_dxi[6]);
 /* 1 if the last kf_raycast_adjoint started from a raycast's saved crossings,
- * 0 if it walked the rays (or no adjoint ran yet). */
+ * 0 if it ran its own trilinear raycast first (or no adjoint ran yet). */
 int kf_adjoint_used_saved(kf_context* ctx, int32_t* used_saved);
 /* Pose refinement through the differentiable raycast (no reference
  * counterpart).  Minimises sum_p (depth_p(T) - d_p)^2 over the pixels whose

@@ -4,9 +4,10 @@
   bit-exact against the oracle's FP32 transcription (F32 and F16 volumes);
 * adjoint: dL/dxi and dL/dTSDF against the oracle's FP64 adjoint of the same
   inputs, to 1e-4 relative (FP32 arithmetic and float atomics on the GPU),
-  from a raycast's saved crossings and by walking the rays again; the two
-  against each other on the room workload, and a changed volume dropping the
-  saved crossings;
+  from a raycast's saved crossings, and without them (the adjoint then runs
+  the trilinear raycast into buffers of its own first); the two against each
+  other on the room workload, and a changed volume dropping the saved
+  crossings;
 * the 128^3 three-sphere workload at 320x240 with 16 poses jittered 2 cm /
   2 deg: gradients of an L2 depth loss through a torch.autograd.Function
   against FP64 central finite differences (h = 1e-4) of the oracle, for the
@@ -72,10 +73,10 @@ def _small_scene():
 
 
 @pytest.mark.parametrize("precision", ["f32", "f16"])
-@pytest.mark.parametrize("path", ["saved", "walk"])
+@pytest.mark.parametrize("path", ["saved", "no_saved"])
 def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle, path, precision):
     """saved: the adjoint right after the raycast at the same inputs, which
-    starts from the raycast's saved crossings; walk: the depth uploaded again
+    starts from the raycast's saved crossings; no_saved: the depth uploaded again
     in between (same values), which drops them, so the adjoint first runs its
     own trilinear raycast and starts from that one's crossings.
 
@@ -120,7 +121,7 @@ def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle, path, precision):
         c.upload_depth_f32(guide)
         c.raycast(K, T0)
         assert_same(c.rendered_depth(), f32["depth"], "forward depth")
-        if path == "walk":
+        if path == "no_saved":
             c.upload_depth_f32(guide)
         gF, gxi = c.raycast_adjoint(K, T0, am.astype(np.float32), bm.astype(np.float32))
         assert c.adjoint_used_saved() == (path == "saved")
@@ -128,18 +129,19 @@ def test_adjoint_matches_fp64_oracle_adjoint(kf, oracle, path, precision):
     assert np.linalg.norm(gF - want["grad_tsdf"]) / np.linalg.norm(want["grad_tsdf"]) < 1e-4
 
 
-def test_adjoint_saved_crossings_match_walk(kf, room_stream):
+def test_adjoint_saved_crossings_match_no_saved(kf, room_stream):
     """Room workload after a 4-frame trilinear pipeline (rays finished by the
     second raycast pass included): the adjoint from a raycast's saved
-    crossings against the walking adjoint at the same inputs, with and
-    without dL/dTSDF, to the order of float additions; and a volume changed
+    crossings against the adjoint without them (its own raycast first) at
+    the same inputs, with and without dL/dTSDF, to the order of float
+    additions; and a volume changed
     after the raycast (TSDF moved 60 mm along z, twice the window tri_hit
-    searches around a crossing) must send the adjoint back to the walk."""
+    searches around a crossing) must make the adjoint run its own raycast."""
     vol, prm, K, H, W, frames, gt = room_stream
     rng = np.random.default_rng(8)
     a = rng.normal(size=(H, W)).astype(np.float32)
     b = rng.normal(size=(H, W, 3)).astype(np.float32)
-    say("[gpu] adjoint from saved crossings vs walk, room 640x480")
+    say("[gpu] adjoint from saved crossings vs its own raycast, room 640x480")
     with kf.KinfuContext.from_preset(kf.PRESET_ROOM, raycast_mode=kf.KF_RAYCAST_TRILINEAR,
                                      stage_timing=1) as c:
         c.stream_upload(frames[:4])
@@ -163,7 +165,8 @@ def test_adjoint_saved_crossings_match_walk(kf, room_stream):
         (gF_w, gxi_w), t_w = adjoint(False, True, 3)
         (_, gxi_sp), _ = adjoint(True, False)
         (_, gxi_wp), _ = adjoint(False, False)
-        say(f"[gpu] adjoint with dL/dTSDF: saved {t_s * 1e3:.1f} us, walk {t_w * 1e3:.1f} us")
+        say(f"[gpu] adjoint with dL/dTSDF: saved {t_s * 1e3:.1f} us, own raycast "
+            f"{t_w * 1e3:.1f} us")
         t, w = c.volume()
         n = vol.dims
         t2 = np.roll(t.reshape(n), 6, axis=2).ravel()
@@ -175,7 +178,7 @@ def test_adjoint_saved_crossings_match_walk(kf, room_stream):
         c.raycast(K, T)
         gF_y, gxi_y = c.raycast_adjoint(K, T, a, b)
         assert c.adjoint_used_saved()
-        # a different pose walks too
+        # a different pose has no saved crossings either
         T2 = T.copy()
         T2[0, 3] += 1.0
         c.raycast_adjoint(K, T2, a, b, want_tsdf=False)
@@ -185,7 +188,8 @@ def test_adjoint_saved_crossings_match_walk(kf, room_stream):
     assert np.linalg.norm(gxi_sp - gxi_wp) <= 1e-6 * np.linalg.norm(gxi_wp)
     assert np.linalg.norm(gxi_sp - gxi_s) <= 1e-6 * np.linalg.norm(gxi_s)
     assert np.abs(gF_s - gF_w).max() <= 1e-5 * np.abs(gF_w).max()
-    # changed volume: the walk (x) and a fresh raycast's crossings (y) agree,
+    # changed volume: the adjoint's own raycast (x) and a fresh raycast's
+    # crossings (y) agree,
     # and differ from the old volume's sums
     assert np.linalg.norm(gxi_x - gxi_y) <= 1e-6 * np.linalg.norm(gxi_y)
     assert np.abs(gF_x - gF_y).max() <= 1e-5 * np.abs(gF_y).max()
