@@ -1457,11 +1457,17 @@ int kf_pipeline_incoming(kf_context* c, const float K[9], int32_t frame) {
   if (rc != KF_OK) return rc;
   kf_cam cam;
   kf_make_cam(K, &cam);
-  kf_stage_begin(c, KF_STAGE_VERTEX);
-  KF_LAUNCH(kf_launch_vertex(c, cam));
-  kf_stage_end(c, KF_STAGE_VERTEX);
+  // the vertex and normal maps are computed by the first full-resolution ICP
+  // iteration (raw_vertices.cu:3-51 in that kernel); with no iteration, by
+  // the vertex kernel
+  const bool fused = c->prm.icp_iterations > 0;
+  if (!fused) {
+    kf_stage_begin(c, KF_STAGE_VERTEX);
+    KF_LAUNCH(kf_launch_vertex(c, cam));
+    kf_stage_end(c, KF_STAGE_VERTEX);
+  }
   kf_stage_begin(c, KF_STAGE_ICP);
-  KF_LAUNCH(kf_launch_icp(c, cam));
+  KF_LAUNCH(kf_launch_icp(c, cam, fused));
   kf_stage_end(c, KF_STAGE_ICP);
   if (c->prm.refine_steps > 0) {
     kf_stage_begin(c, KF_STAGE_REFINE);

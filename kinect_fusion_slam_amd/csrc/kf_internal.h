@@ -213,7 +213,10 @@ hipError_t kf_launch_depth_u16(kf_context* c, const uint16_t* src);
 hipError_t kf_launch_bilateral(kf_context* c);
 hipError_t kf_launch_depth_bilateral(kf_context* c, const uint16_t* src);
 hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam);
-hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam);
+// maps_from_depth: the first full-resolution iteration computes the frame's
+// vertex, normal and mask maps from the working depth (instead of
+// kf_launch_vertex) and writes them to the context's maps.
+hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam, bool maps_from_depth = false);
 // Intrinsics of the next coarser pyramid level (K and Kc: 3x3 row-major).
 void kf_pyramid_intrinsics(const float* K, float* Kc);
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);

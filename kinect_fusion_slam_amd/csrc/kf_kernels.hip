@@ -654,10 +654,15 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 // Per-pixel association and gates (sensor_pose_estimation.cu:16-136) plus the
 // block reduction, for iteration `iter`.  nb_in is the grid of the iteration
 // before (the row stride of its partials): the levels of the opt-in
-// coarse-to-fine pyramid have grids of their own.
-template <bool KAHAN, int PX, int PPL>
+// coarse-to-fine pyramid have grids of their own.  FROM_DEPTH: the frame's
+// vertex and normal maps are not read but computed from the depth (three
+// loads per pixel, raw_vertices.cu:3-51 through vertex_at) and written to
+// vmap_out / nmap_out / vmask_out for the later iterations, so that the
+// pipeline needs no separate vertex kernel.
+template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
-    const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
+    const float* __restrict__ depth, kf_p3* __restrict__ vmap_out, kf_p3* __restrict__ nmap_out,
+    uint8_t* __restrict__ vmask_out, const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
     const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, kf_cam cam, int H, int W,
     float dist2_hi, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
     const void* __restrict__ part_in_, void* __restrict__ part_out_,
@@ -692,12 +697,14 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   const kf_dev_state* stp_ = st;
   const kf_p3* vm_ = vmap;
   const kf_p3* nm_ = nmap;
-  asm volatile("" : "+s"(pin_), "+s"(stp_), "+s"(vm_), "+s"(nm_));
+  const float* dp_ = depth;
+  asm volatile("" : "+s"(pin_), "+s"(stp_), "+s"(vm_), "+s"(nm_), "+s"(dp_));
   // back in the global address space (global, not flat, loads)
   const KF_GLOBAL PT* pin = (const KF_GLOBAL PT*)pin_;
   const KF_GLOBAL kf_dev_state* stp = (const KF_GLOBAL kf_dev_state*)stp_;
   const KF_GLOBAL kf_p3* vm = (const KF_GLOBAL kf_p3*)vm_;
   const KF_GLOBAL kf_p3* nm = (const KF_GLOBAL kf_p3*)nm_;
+  const KF_GLOBAL float* dp = (const KF_GLOBAL float*)dp_;
   IcpPartialRegs<ROWS, KF_ICP_BLOCK, PPL, PT> R;
   icp_load_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(pin, nb_in, R);
   const int e16 = threadIdx.x & 15;
@@ -705,11 +712,21 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   const float Tpe = stp->Tprev_inv[e16];
   asm volatile("" ::: "memory");
   kf_p3 V[PX], Nc[PX];
+  // FROM_DEPTH: each pixel's depth and those below and to the right of it
+  constexpr int PD = FROM_DEPTH ? PX : 1;
+  float Di[PD], Dr[PD], Dc[PD];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
     const int x = min(base + k * KF_ICP_BLOCK, n - 1);
-    V[k] = vm[x];
-    Nc[k] = nm[x];
+    if constexpr (FROM_DEPTH) {
+      const int py = x / W, px = x - py * W;
+      Di[k] = dp[x];
+      Dr[k] = dp[min((py + 1) * W + px, n - 1)];
+      Dc[k] = dp[min(py * W + (px + 1), n - 1)];
+    } else {
+      V[k] = vm[x];
+      Nc[k] = nm[x];
+    }
   }
   if (iter == 0) {
     // frame start
@@ -738,6 +755,22 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     icp_sum_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(R, nb_in, sh);
     icp_update_pose<ROWS>(sh, min_inliers);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
+  }
+  if constexpr (FROM_DEPTH) {
+    // the frame's vertex, normal and mask (raw_vertices.cu:3-51), kept for
+    // the later iterations
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+      const int x = min(base + k * KF_ICP_BLOCK, n - 1);
+      const int py = x / W, px = x - py * W;
+      uint8_t m;
+      vertex_at(px, py, H, W, cam.Kinv, Di[k], Dr[k], Dc[k], V[k], Nc[k], m);
+      if (k * KF_ICP_BLOCK < lim) {
+        vmap_out[x] = V[k];
+        nmap_out[x] = Nc[k];
+        vmask_out[x] = m;
+      }
+    }
   }
   float F[12], Te[12];
 #pragma unroll
@@ -3009,18 +3042,28 @@ hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam) {
 }
 
 // One level of the ICP: its image, grid, camera and maps, and iterations.
+// depth set: the first iteration computes the frame's maps from it and
+// writes them to vmap_w / nmap_w / mask_w (the pipeline's level 0).
 struct IcpLevel {
   int H, W, blocks, bpx, px, iters;
   kf_cam cam;
   const kf_p3 *vmap, *nmap, *mv, *mn;
+  const float* depth;
+  kf_p3 *vmap_w, *nmap_w;
+  uint8_t* mask_w;
 };
 
 template <bool KAHAN, int PX, int PPL>
 static void launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin, void* pout,
-                               uint8_t* m, int it, int nb_in, float min_inl) {
-  k_icp_iter<KAHAN, PX, PPL><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(
-      L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W, c->icp_dist2_hi, c->icp_cos_lo, min_inl,
-      c->state, pin, pout, m, it, L.bpx, nb_in);
+                               uint8_t* m, int it, int nb_in, float min_inl, bool from_depth) {
+  if (from_depth)
+    k_icp_iter<KAHAN, PX, PPL, true><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(
+        L.depth, L.vmap_w, L.nmap_w, L.mask_w, L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W,
+        c->icp_dist2_hi, c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in);
+  else
+    k_icp_iter<KAHAN, PX, PPL, false><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(
+        nullptr, nullptr, nullptr, nullptr, L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W,
+        c->icp_dist2_hi, c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in);
 }
 
 // The iterations of every level, coarsest first (lv[n_levels - 1] ... lv[0]),
@@ -3042,6 +3085,7 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
   for (int l = n_levels - 1; l >= 0; --l) {
     const IcpLevel& L = lv[l];
     for (int k = 0; k < L.iters; ++k, ++it) {
+      const bool fd = k == 0 && L.depth != nullptr;  // maps from the depth
       uint8_t* m =
           (c->prm.record_icp && c->icp_masks) ? c->icp_masks + (size_t)it * n : nullptr;
       const void* pin = pbase + (size_t)((it + 1) & 1) * half;
@@ -3049,16 +3093,16 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
       if (it == 0) kf_stage_begin(c, KF_STAGE_ICP_ITER);
       switch (L.px) {
         case 1:
-          launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
           break;
         case 2:
-          launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
           break;
         case 3:
-          launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
           break;
         default:
-          launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl);
+          launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
           break;
       }
       hipError_t e = hipGetLastError();
@@ -3111,16 +3155,19 @@ static hipError_t build_pyramid(kf_context* c, const kf_cam& cam0, IcpLevel* lv)
     k_model_down<<<grid, 256, 0, c->stream>>>(v_in, n_in, l == 1 ? c->hit : nullptr,
                                               l == 1 ? nullptr : c->pyr[l - 1].ok, Wi, P.mv,
                                               P.mn, P.ok, P.H, P.W);
-    lv[l] = IcpLevel{P.H, P.W, P.blocks, P.bpx, P.px, c->prm.icp_coarse_iters[l - 1],
-                     cam,  P.vmap, P.nmap, P.mv, P.mn};
+    lv[l] = IcpLevel{P.H,   P.W,    P.blocks, P.bpx, P.px,    c->prm.icp_coarse_iters[l - 1],
+                     cam,   P.vmap, P.nmap,   P.mv,  P.mn,    nullptr,
+                     nullptr, nullptr, nullptr};
   }
   return hipGetLastError();
 }
 
-hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam) {
+hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam, bool maps_from_depth) {
   IcpLevel lv[KF_MAX_ICP_LEVELS];
-  lv[0] = IcpLevel{c->H, c->W, c->icp_blocks, c->icp_bpx, c->icp_px, c->prm.icp_iterations,
-                   cam,  c->vmap, c->nmap, c->model_v, c->model_n};
+  lv[0] = IcpLevel{c->H,    c->W,    c->icp_blocks, c->icp_bpx,
+                   c->icp_px, c->prm.icp_iterations, cam, c->vmap,
+                   c->nmap, c->model_v, c->model_n, maps_from_depth ? c->depth : nullptr,
+                   c->vmap, c->nmap, c->mask};
   const int n_levels = c->prm.icp_levels > 1 ? c->prm.icp_levels : 1;
   if (n_levels > 1) {
     if (!c->pyr[n_levels - 1].depth) return hipErrorInvalidValue;
