@@ -161,6 +161,13 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width,
               const kf_params* prm, kf_context** out);
 int kf_destroy(kf_context* ctx);
 const char* kf_last_error(void);
+/* sizes[0..2] = sizeof(kf_params), sizeof(kf_volume_desc) and
+ * sizeof(kf_synth_scene) as the library was built.  A binding or program
+ * built against another version of this header compares them with its own
+ * and refuses to run on a mismatch: the library would otherwise write a
+ * grown struct past the end of the caller's copy.  No reference counterpart
+ * (the reference builds its pipeline into one program). */
+int kf_struct_sizes(size_t sizes[3]);
 int kf_set_params(kf_context* ctx, const kf_params* prm);
 int kf_get_params(const kf_context* ctx, kf_params* prm);
 /* Re-fills the volume with "never observed" and the pose with identity. */

@@ -76,6 +76,7 @@ _SIGNATURES = {
                             C.POINTER(_P)]),
     "kf_destroy": (C.c_int, [_P]),
     "kf_last_error": (C.c_char_p, []),
+    "kf_struct_sizes": (C.c_int, [C.POINTER(C.c_size_t)]),
     "kf_set_params": (C.c_int, [_P, C.POINTER(Params)]),
     "kf_get_params": (C.c_int, [_P, C.POINTER(Params)]),
     "kf_reset": (C.c_int, [_P]),
@@ -154,6 +155,14 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    built = (C.c_size_t * 3)()
+    lib.kf_struct_sizes(built)
+    mine = (C.sizeof(Params), C.sizeof(VolumeDesc), C.sizeof(SynthScene))
+    if tuple(built) != mine:
+        raise ImportError(
+            f"{LIB_PATH} was built with struct sizes (kf_params, kf_volume_desc, "
+            f"kf_synth_scene) = {tuple(built)} but this binding declares {mine}; "
+            "rebuild the library or update the binding")
     return lib
 
 

@@ -302,6 +302,14 @@ extern "C" {
 
 const char* kf_last_error(void) { return g_last_error.c_str(); }
 
+int kf_struct_sizes(size_t sizes[3]) {
+  if (!sizes) return kf_fail(KF_ERR_INVALID, "kf_struct_sizes: sizes is NULL");
+  sizes[0] = sizeof(kf_params);
+  sizes[1] = sizeof(kf_volume_desc);
+  sizes[2] = sizeof(kf_synth_scene);
+  return KF_OK;
+}
+
 int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], int32_t* height,
               int32_t* width) {
   kf_volume_desc v{};

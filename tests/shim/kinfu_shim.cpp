@@ -22,6 +22,12 @@ int shim_run_reference_pipeline(const char* data_dir, int height, int width, flo
                                 float min_depth, int first, int count, float* poses,
                                 float* tsdf, float* weight, int64_t n_voxels) {
   if (!data_dir || !poses || !tsdf || !weight || count < 1) return KF_ERR_INVALID;
+  // KinfuPipeline holds a kf_params and a kf_volume_desc: a library built
+  // from another header would construct it past the end of this frame.
+  size_t sz[3];
+  if (kf_struct_sizes(sz) != KF_OK || sz[0] != sizeof(kf_params) ||
+      sz[1] != sizeof(kf_volume_desc))
+    return KF_ERR_INVALID;
   Kinfu::KinfuPipeline p(height, width, max_depth, min_depth);
   if (p.status() != KF_OK) return p.status();
   p.data_dir = data_dir;

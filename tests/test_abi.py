@@ -26,6 +26,17 @@ def test_library_exports_every_declared_symbol(kf):
     assert not unbound, f"declared but not bound in the Python view: {unbound}"
 
 
+def test_struct_sizes_match_the_binding(kf):
+    """kf_struct_sizes reports the library's sizeof of every public struct;
+    the ctypes view declares the same layouts (a stale build of either side
+    would otherwise write past the other's copy)."""
+    sizes = (ctypes.c_size_t * 3)()
+    assert kf._abi.lib.kf_struct_sizes(sizes) == kf._abi.KF_OK
+    assert tuple(sizes) == (ctypes.sizeof(kf._abi.Params), ctypes.sizeof(kf._abi.VolumeDesc),
+                            ctypes.sizeof(kf._abi.SynthScene))
+    assert kf._abi.lib.kf_struct_sizes(None) == -1  # KF_ERR_INVALID
+
+
 def test_reference_preset_matches_reference_constants(kf):
     vol, prm, K, H, W = kf.preset(kf.PRESET_REFERENCE)
     # kinfu_pipeline.cpp:14-24
