@@ -362,8 +362,9 @@ int kf_stream_upload_u16(kf_context* ctx, const uint16_t* frames, int32_t n_fram
  * raycast, at the context pose. */
 int kf_pipeline_initial(kf_context* ctx, const float K[9], int32_t frame);
 /* IncomingFrameProcessing: depth(frame) -> [bilateral] -> vertex/normal ->
- * ICP (updates the pose on the device) -> integrate -> raycast.  Enqueued
- * only; nothing is copied to the host.  frame = KF_FRAME_CURRENT (here and in
+ * ICP (updates the pose on the device; its first iteration computes the
+ * vertex and normal maps) -> integrate -> raycast.  Enqueued only; nothing
+ * is copied to the host.  frame = KF_FRAME_CURRENT (here and in
  * kf_pipeline_initial) takes the depth last loaded by kf_upload_depth_u16 /
  * _f32 instead of a stream frame, as the reference loads each frame from the
  * host (kinfu_pipeline.cpp:128-131); the pose is then not recorded. */
