@@ -72,7 +72,8 @@ extern "C" {
 #define KF_STAGE_REFINE 8    /* pose refinement through the trilinear raycast  */
 #define KF_STAGE_ICP_ITER 9  /* the iteration kernels of one ICP, back to back: KF_STAGE_ICP
                                 without its last (finishing) kernel; one count per ICP */
-#define KF_STAGE_COUNT 10
+#define KF_STAGE_COLOR 10     /* colour JPEG: inverse DCT + upsampling/colour kernels */
+#define KF_STAGE_COUNT 11
 
 /* Parameter presets for kf_preset(). */
 #define KF_PRESET_REFERENCE 100 /* the reference's compiled-in defaults          */
@@ -187,6 +188,50 @@ int kf_upload_depth_f32(kf_context* ctx, const float* depth_mm, int32_t mem_kind
  * (system_utility.cpp:58-68 reads "../data/dep%04d.dat") into host memory. */
 int kf_read_depth_file(const char* path, int32_t height, int32_t width, uint16_t* out);
 int kf_download_depth(kf_context* ctx, float* depth_mm);
+
+/* ---- colour (system_utility.cpp:70-75, threeD_viewer.cpp:20-55) ---------
+ * The reference loads each frame's colour image with cv::imread(path)
+ * (LoadColorData, 8-bit B, G, R) and only uses it for its viewer, which
+ * copies the raycast surface with each pixel's colour into a PCL cloud
+ * (ThreeDViewer::SetUpPointClouds).  Here the context keeps an H x W x 3
+ * B, G, R image on the device (zeros until one is loaded, as the
+ * reference's cv::Mat::zeros at system_utility.cpp:13).
+ *
+ * kf_upload_color_jpeg decodes a baseline JPEG (SOF0/SOF1, 8-bit, Huffman;
+ * 1 or 3 components; sampling 4:4:4, 4:2:2, 4:2:0 or other integral
+ * factors except 1:2 vertical-only; restart intervals) to the values
+ * cv::imread gives with libjpeg-turbo's default decompression: ISLOW
+ * inverse DCT, "fancy" chroma upsampling, 16-bit fixed-point YCbCr -> RGB.
+ * The markers and the Huffman decoding run on the host (the serial part);
+ * dequantisation, inverse DCT, upsampling and colour conversion run on the
+ * device.  The image must be H x W.  Progressive, arithmetic-coded,
+ * lossless and 12-bit JPEGs, and 4-component (CMYK) images, give
+ * KF_ERR_INVALID.  Returns once the host buffers may be reused; the device
+ * work is enqueued. */
+int kf_upload_color_jpeg(kf_context* ctx, const uint8_t* jpeg, size_t n_bytes);
+/* LoadColorData: the JPEG file at path (the reference's
+ * ../data/img%04d.jpg); KF_ERR_IO when it cannot be read. */
+int kf_load_color_file(kf_context* ctx, const char* path);
+/* Host only (no context): a baseline JPEG's size and component count. */
+int kf_jpeg_info(const uint8_t* jpeg, size_t n_bytes, int32_t* height, int32_t* width,
+                 int32_t* components);
+/* Host only: the Huffman-decoded coefficient blocks kf_upload_color_jpeg
+ * sends to the device, 64 int16 each in natural (row-major) order, not
+ * dequantised: component after component, each over whole MCUs in raster
+ * order (MCUs per row x horizontal sampling factor blocks per row).  coef
+ * NULL: *n_blocks only; else capacity_blocks must be at least that. */
+int kf_jpeg_coefficients(const uint8_t* jpeg, size_t n_bytes, int16_t* coef,
+                         int64_t capacity_blocks, int64_t* n_blocks);
+/* An already decoded H x W x 3 B, G, R image (mem_kind KF_MEM_*). */
+int kf_upload_color_bgr(kf_context* ctx, const uint8_t* bgr, int32_t mem_kind);
+int kf_download_color(kf_context* ctx, uint8_t* bgr);
+/* SetUpPointClouds: H*W records of 4 floats {x, y, z, rgb}, one per pixel
+ * (index = row * W + column): where the last raycast's vertex has z != 0,
+ * the vertex and the colour of the same pixel packed as r << 16 | g << 8 |
+ * b and stored as the bits of a float (PCL's PointXYZRGB rgb field);
+ * elsewhere four zeros.  mem_kind KF_MEM_DEVICE: xyzrgb is device memory,
+ * 16-byte aligned. */
+int kf_point_cloud(kf_context* ctx, float* xyzrgb, int32_t mem_kind);
 
 /* ---- A4: bilateral filter (bilateral_filtering.cu:3-108) -----------------
  * Replaces the working depth by its bilateral filtering with the context's

@@ -19,7 +19,7 @@ KF_MAX_ICP_ITERS = 64
 KF_ICP_SUMS = 28
 KF_ICP_STATUS_GATED = 2
 STAGES = ("depth", "bilateral", "vertex", "icp", "integrate", "raycast", "integrate_prep",
-          "raycast_adjoint", "refine", "icp_iter")
+          "raycast_adjoint", "refine", "icp_iter", "color")
 KF_FRAME_CURRENT = -1
 KF_RAYCAST_REFERENCE, KF_RAYCAST_TRILINEAR = 0, 1
 PRESET_REFERENCE = 100
@@ -101,6 +101,14 @@ _SIGNATURES = {
     "kf_volume_device_ptr": (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "kf_raycast": (C.c_int, [_P, _F, _F]),
     "kf_download_model_maps": (C.c_int, [_P, _F, _F, _F, _F, _F, _F, _I32]),
+    "kf_upload_color_jpeg": (C.c_int, [_P, _U8, C.c_size_t]),
+    "kf_jpeg_info": (C.c_int, [_U8, C.c_size_t, _I32, _I32, _I32]),
+    "kf_jpeg_coefficients": (C.c_int, [_U8, C.c_size_t, C.POINTER(C.c_int16), C.c_int64,
+                                       C.POINTER(C.c_int64)]),
+    "kf_load_color_file": (C.c_int, [_P, C.c_char_p]),
+    "kf_upload_color_bgr": (C.c_int, [_P, _U8, C.c_int32]),
+    "kf_download_color": (C.c_int, [_P, _U8]),
+    "kf_point_cloud": (C.c_int, [_P, _F, C.c_int32]),
     "kf_upload_model_maps": (C.c_int, [_P, _F, _F, _F, _F, _F, _F]),
     "kf_stream_upload_u16": (C.c_int, [_P, _P, C.c_int32, C.c_int32]),
     "kf_pipeline_initial": (C.c_int, [_P, _F, C.c_int32]),
@@ -188,6 +196,28 @@ def _f32(a, n=None):
     if n is not None and a.size != n:
         raise ValueError(f"expected {n} floats, got {a.size}")
     return a
+
+
+def jpeg_info(data):
+    """(height, width, components) of a baseline JPEG held in bytes."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    h, w, nc = C.c_int32(), C.c_int32(), C.c_int32()
+    check(lib.kf_jpeg_info(buf.ctypes.data_as(_U8), buf.size, C.byref(h), C.byref(w),
+                           C.byref(nc)))
+    return h.value, w.value, nc.value
+
+
+def jpeg_coefficients(data):
+    """The Huffman-decoded coefficient blocks (n_blocks x 64 int16, natural
+    order, not dequantised) that the colour upload sends to the device."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    nb = C.c_int64()
+    check(lib.kf_jpeg_coefficients(buf.ctypes.data_as(_U8), buf.size, None, 0, C.byref(nb)))
+    out = np.zeros((nb.value, 64), np.int16)
+    check(lib.kf_jpeg_coefficients(buf.ctypes.data_as(_U8), buf.size,
+                                   out.ctypes.data_as(C.POINTER(C.c_int16)), nb.value,
+                                   C.byref(nb)))
+    return out
 
 
 def preset(which):
@@ -480,6 +510,33 @@ class KinfuContext:
                                                    ("x", "y", "z", "nx", "ny", "nz")),
                                          hit.ctypes.data_as(_I32)))
         out["hit"] = hit
+        return out
+
+    # colour (system_utility.cpp:70-75, threeD_viewer.cpp:20-55)
+    def upload_color_jpeg(self, data):
+        """Decodes a baseline JPEG (bytes) into the context's B, G, R image."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        check(lib.kf_upload_color_jpeg(self.h, buf.ctypes.data_as(_U8), buf.size))
+
+    def load_color_file(self, path):
+        check(lib.kf_load_color_file(self.h, os.fsencode(path)))
+
+    def upload_color_bgr(self, bgr):
+        a = np.ascontiguousarray(bgr, np.uint8)
+        if a.size != self.n * 3:
+            raise ValueError(f"expected {self.height}x{self.width}x3 bytes, got {a.shape}")
+        check(lib.kf_upload_color_bgr(self.h, a.ctypes.data_as(_U8), KF_MEM_HOST))
+
+    def color(self):
+        out = np.zeros((self.height, self.width, 3), np.uint8)
+        check(lib.kf_download_color(self.h, out.ctypes.data_as(_U8)))
+        return out
+
+    def point_cloud(self):
+        """H x W x 4 float32 {x, y, z, rgb}; rgb holds the bits of
+        r << 16 | g << 8 | b (PCL's PointXYZRGB layout)."""
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        check(lib.kf_point_cloud(self.h, fptr(out), KF_MEM_HOST))
         return out
 
     def upload_model_maps(self, x, y, z, nx, ny, nz):

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "kf_internal.h"
+#include "kf_jpeg.h"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -456,9 +457,13 @@ int kf_destroy(kf_context* c) {
                   c->spatial, c->range_lut, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
                   c->adj_partials, c->gn_partials, c->adj_touched, c->adj_map_v,
-                  c->adj_map_n, c->adj_map_hit, c->model_aux, c->adj_map_aux};
+                  c->adj_map_n, c->adj_map_hit, c->model_aux, c->adj_map_aux,
+                  c->color, c->jpeg_coef, c->jpeg_planes, c->jpeg_q, c->cloud};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (c->jpeg_coef_host) (void)hipHostFree(c->jpeg_coef_host);
+  if (c->jpeg_q_host) (void)hipHostFree(c->jpeg_q_host);
+  if (c->jpeg_copied) (void)hipEventDestroy(c->jpeg_copied);
   for (kf_pyr_level& P : c->pyr) {
     void* pb[] = {P.depth, P.vmap, P.nmap, P.mask, P.mv, P.mn, P.ok};
     for (void* b : pb)
@@ -608,6 +613,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->model_v, n * 12},    {(void**)&c->model_n, n * 12},
       {(void**)&c->model_aux, n * 8},
       {(void**)&c->hit, n * 4},         {(void**)&c->integ_tab, n * 8},
+      {(void**)&c->color, n * 3},
       {(void**)&c->integ_tiles, (size_t)((width + 15) / 16) * ((height + 15) / 16) * 8},
       {&c->ray_queue, kf_ray_queue_entries(height, width) * kf_ray_state_bytes()},
       {(void**)&c->ray_counts, (size_t)2 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4},
@@ -633,10 +639,10 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
     const std::string m = g_last_error;
     return fail(rc, m);
   }
-  void* zero_bufs[] = {c->depth, c->depth_tmp, c->vmap,    c->nmap,
-                       c->mask,  c->model_v,   c->model_n, c->model_aux};
-  size_t zero_sizes[] = {n * 4, n * 4, n * 12, n * 12, n, n * 12, n * 12, n * 8};
-  for (int i = 0; i < 8; ++i)
+  void* zero_bufs[] = {c->depth,   c->depth_tmp, c->vmap,      c->nmap, c->mask,
+                       c->model_v, c->model_n,   c->model_aux, c->color};
+  size_t zero_sizes[] = {n * 4, n * 4, n * 12, n * 12, n, n * 12, n * 12, n * 8, n * 3};
+  for (int i = 0; i < 9; ++i)
     if (hipMemsetAsync(zero_bufs[i], 0, zero_sizes[i], c->stream) != hipSuccess)
       return fail(KF_ERR_HIP, "hipMemsetAsync failed");
   if (hipMemsetAsync(c->hit, 0xff, n * 4, c->stream) != hipSuccess ||
@@ -1379,6 +1385,183 @@ int kf_upload_model_maps(kf_context* c, const float* x, const float* y, const fl
   KF_HIP_CHECK(hipMemcpyAsync(c->model_v, V.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipMemcpyAsync(c->model_n, N.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+// ---- colour: LoadColorData and SetUpPointClouds ----------------------------
+
+// Grows the JPEG decode's buffers to n_blocks coefficient blocks and
+// plane_bytes of component samples (the staging buffers are idle: the
+// caller has waited for their last copy).
+static int jpeg_buffers(kf_context* c, int64_t n_blocks, int64_t plane_bytes) {
+  if (!c->jpeg_copied)
+    KF_HIP_CHECK(hipEventCreateWithFlags(&c->jpeg_copied, hipEventDisableTiming));
+  if (!c->jpeg_q) {
+    KF_REQUIRE(hipMalloc((void**)&c->jpeg_q, 4 * 64 * 2) == hipSuccess, KF_ERR_OOM,
+               "cannot allocate the quantisation tables");
+    if (hipHostMalloc((void**)&c->jpeg_q_host, 4 * 64 * 2, hipHostMallocDefault) != hipSuccess) {
+      c->jpeg_q_host = nullptr;
+      return kf_fail(KF_ERR_OOM, "cannot allocate pinned staging");
+    }
+  }
+  if (n_blocks > c->jpeg_blocks_cap) {
+    KF_HIP_CHECK(hipStreamSynchronize(c->stream));  // the old buffer's last reader
+    if (c->jpeg_coef) (void)hipFree(c->jpeg_coef);
+    if (c->jpeg_coef_host) (void)hipHostFree(c->jpeg_coef_host);
+    c->jpeg_coef = nullptr;
+    c->jpeg_coef_host = nullptr;
+    c->jpeg_blocks_cap = 0;
+    const size_t bytes = (size_t)n_blocks * 64 * 2;
+    if (hipMalloc((void**)&c->jpeg_coef, bytes) != hipSuccess) {
+      c->jpeg_coef = nullptr;
+      return kf_fail(KF_ERR_OOM, "cannot allocate the coefficient blocks");
+    }
+    if (hipHostMalloc((void**)&c->jpeg_coef_host, bytes, hipHostMallocDefault) != hipSuccess) {
+      c->jpeg_coef_host = nullptr;
+      return kf_fail(KF_ERR_OOM, "cannot allocate pinned staging");
+    }
+    c->jpeg_blocks_cap = n_blocks;
+  }
+  if (plane_bytes > c->jpeg_planes_cap) {
+    KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (c->jpeg_planes) (void)hipFree(c->jpeg_planes);
+    c->jpeg_planes_cap = 0;
+    if (hipMalloc((void**)&c->jpeg_planes, (size_t)plane_bytes) != hipSuccess) {
+      c->jpeg_planes = nullptr;
+      return kf_fail(KF_ERR_OOM, "cannot allocate the sample planes");
+    }
+    c->jpeg_planes_cap = plane_bytes;
+  }
+  return KF_OK;
+}
+
+int kf_upload_color_jpeg(kf_context* c, const uint8_t* jpeg, size_t n) {
+  KF_REQUIRE(c && jpeg && n > 0, KF_ERR_INVALID, "bad argument");
+  kf_jpeg_frame f;
+  std::string err;
+  if (kf_jpeg_decode_entropy(jpeg, n, &f, nullptr, 0, &err) != 0)
+    return kf_fail(KF_ERR_INVALID, "JPEG: " + err);
+  KF_REQUIRE(f.H == c->H && f.W == c->W, KF_ERR_INVALID,
+             "colour image is " + std::to_string(f.W) + "x" + std::to_string(f.H) +
+                 ", the context " + std::to_string(c->W) + "x" + std::to_string(c->H));
+  kf_jpeg_dev d;
+  d.H = f.H;
+  d.W = f.W;
+  d.nc = f.nc;
+  d.rgb = f.rgb ? 1 : 0;
+  d.n_blocks = f.n_blocks;
+  int64_t plane_bytes = 0;
+  for (int i = 0; i < f.nc; ++i) {
+    const kf_jpeg_component& s = f.comp[i];
+    kf_jpeg_dev_comp& o = d.c[i];
+    o.block0 = s.block0;
+    o.plane0 = plane_bytes;
+    o.bw = s.bw;
+    o.stride = s.bw * 8;
+    o.dw = s.dw;
+    o.dh = s.dh;
+    o.fh = f.hmax / s.h;
+    o.fv = f.vmax / s.v;
+    o.tq = s.tq;
+    plane_bytes += (int64_t)s.bw * 8 * s.bh * 8;
+  }
+  // the staging buffers may still be read by the previous upload's copies
+  if (c->jpeg_copied) KF_HIP_CHECK(hipEventSynchronize(c->jpeg_copied));
+  int rc = jpeg_buffers(c, f.n_blocks, plane_bytes);
+  if (rc != KF_OK) return rc;
+  if (kf_jpeg_decode_entropy(jpeg, n, &f, c->jpeg_coef_host, c->jpeg_blocks_cap, &err) != 0)
+    return kf_fail(KF_ERR_INVALID, "JPEG: " + err);
+  memcpy(c->jpeg_q_host, f.q, sizeof f.q);
+  KF_HIP_CHECK(hipMemcpyAsync(c->jpeg_coef, c->jpeg_coef_host, (size_t)f.n_blocks * 64 * 2,
+                              hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->jpeg_q, c->jpeg_q_host, sizeof f.q, hipMemcpyHostToDevice,
+                              c->stream));
+  KF_HIP_CHECK(hipEventRecord(c->jpeg_copied, c->stream));
+  kf_stage_begin(c, KF_STAGE_COLOR);
+  KF_LAUNCH(kf_launch_jpeg_decode(c, d));
+  kf_stage_end(c, KF_STAGE_COLOR);
+  return KF_OK;
+}
+
+int kf_jpeg_info(const uint8_t* jpeg, size_t n, int32_t* height, int32_t* width,
+                 int32_t* components) {
+  KF_REQUIRE(jpeg && n > 0, KF_ERR_INVALID, "bad argument");
+  kf_jpeg_frame f;
+  std::string err;
+  if (kf_jpeg_decode_entropy(jpeg, n, &f, nullptr, 0, &err) != 0)
+    return kf_fail(KF_ERR_INVALID, "JPEG: " + err);
+  if (height) *height = f.H;
+  if (width) *width = f.W;
+  if (components) *components = f.nc;
+  return KF_OK;
+}
+
+int kf_jpeg_coefficients(const uint8_t* jpeg, size_t n, int16_t* coef, int64_t capacity,
+                         int64_t* n_blocks) {
+  KF_REQUIRE(jpeg && n > 0, KF_ERR_INVALID, "bad argument");
+  kf_jpeg_frame f;
+  std::string err;
+  if (kf_jpeg_decode_entropy(jpeg, n, &f, nullptr, 0, &err) != 0)
+    return kf_fail(KF_ERR_INVALID, "JPEG: " + err);
+  if (n_blocks) *n_blocks = f.n_blocks;
+  if (!coef) return KF_OK;
+  KF_REQUIRE(capacity >= f.n_blocks, KF_ERR_INVALID, "coefficient buffer too small");
+  if (kf_jpeg_decode_entropy(jpeg, n, &f, coef, capacity, &err) != 0)
+    return kf_fail(KF_ERR_INVALID, "JPEG: " + err);
+  return KF_OK;
+}
+
+int kf_load_color_file(kf_context* c, const char* path) {
+  KF_REQUIRE(c && path, KF_ERR_INVALID, "null argument");
+  FILE* fp = fopen(path, "rb");
+  KF_REQUIRE(fp, KF_ERR_IO, std::string("cannot open ") + path);
+  std::vector<uint8_t> data;
+  uint8_t chunk[1 << 16];
+  size_t got;
+  while ((got = fread(chunk, 1, sizeof chunk, fp)) > 0) data.insert(data.end(), chunk, chunk + got);
+  const bool bad = ferror(fp) != 0;
+  fclose(fp);
+  KF_REQUIRE(!bad && !data.empty(), KF_ERR_IO, std::string("cannot read ") + path);
+  return kf_upload_color_jpeg(c, data.data(), data.size());
+}
+
+int kf_upload_color_bgr(kf_context* c, const uint8_t* bgr, int32_t mem_kind) {
+  KF_REQUIRE(c && bgr, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipMemcpyAsync(c->color, bgr, (size_t)c->H * c->W * 3,
+                              mem_kind == KF_MEM_HOST ? hipMemcpyHostToDevice
+                                                      : hipMemcpyDeviceToDevice,
+                              c->stream));
+  if (mem_kind == KF_MEM_HOST) KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+int kf_download_color(kf_context* c, uint8_t* bgr) {
+  KF_REQUIRE(c && bgr, KF_ERR_INVALID, "null argument");
+  KF_HIP_CHECK(hipMemcpyAsync(bgr, c->color, (size_t)c->H * c->W * 3, hipMemcpyDeviceToHost,
+                              c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return KF_OK;
+}
+
+int kf_point_cloud(kf_context* c, float* xyzrgb, int32_t mem_kind) {
+  KF_REQUIRE(c && xyzrgb, KF_ERR_INVALID, "null argument");
+  const size_t bytes = (size_t)c->H * c->W * 16;
+  float4* out = reinterpret_cast<float4*>(xyzrgb);
+  if (mem_kind == KF_MEM_DEVICE) {
+    KF_REQUIRE(((uintptr_t)xyzrgb & 15) == 0, KF_ERR_INVALID,
+               "device output must be 16-byte aligned");
+  } else {
+    if (!c->cloud && hipMalloc((void**)&c->cloud, bytes) != hipSuccess) {
+      c->cloud = nullptr;
+      return kf_fail(KF_ERR_OOM, "cannot allocate the point cloud");
+    }
+    out = c->cloud;
+  }
+  KF_LAUNCH(kf_launch_point_cloud(c, out));
+  if (mem_kind != KF_MEM_DEVICE) {
+    KF_HIP_CHECK(hipMemcpyAsync(xyzrgb, c->cloud, bytes, hipMemcpyDeviceToHost, c->stream));
+    KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
   return KF_OK;
 }
 

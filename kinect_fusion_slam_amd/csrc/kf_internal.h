@@ -109,6 +109,22 @@ struct kf_pyr_level {
   uint8_t* ok = nullptr;
 };
 
+// The colour decode's frame as the device kernels take it: per component
+// its first coefficient block, its sample plane (byte offset, row stride =
+// 8 x blocks per row), its real samples and its upsampling factors.
+struct kf_jpeg_dev_comp {
+  int64_t block0 = 0, plane0 = 0;
+  int bw = 0, stride = 0;
+  int dw = 0, dh = 0;
+  int fh = 1, fv = 1;
+  int tq = 0;
+};
+struct kf_jpeg_dev {
+  int H = 0, W = 0, nc = 0, rgb = 0;
+  int64_t n_blocks = 0;
+  kf_jpeg_dev_comp c[4];
+};
+
 struct kf_context {
   int device = 0;
   int H = 0, W = 0;
@@ -193,6 +209,21 @@ struct kf_context {
   float saved_K[9] = {};
   float saved_T[16] = {};
   bool adj_used_saved = false;  // the last adjoint started from the saved crossings
+  // colour (LoadColorData / SetUpPointClouds): the H x W x 3 B, G, R image
+  // (zeros until one is loaded), and the JPEG decode's buffers, grown on
+  // demand: coefficient blocks (device, and pinned host staging), the
+  // component sample planes and the 4 quantisation tables; jpeg_copied marks
+  // the staging buffers' last copy to the device
+  uint8_t* color = nullptr;
+  int16_t* jpeg_coef = nullptr;
+  int16_t* jpeg_coef_host = nullptr;
+  int64_t jpeg_blocks_cap = 0;
+  uint8_t* jpeg_planes = nullptr;
+  int64_t jpeg_planes_cap = 0;
+  uint16_t* jpeg_q = nullptr;
+  uint16_t* jpeg_q_host = nullptr;
+  hipEvent_t jpeg_copied = nullptr;
+  float4* cloud = nullptr;        // point cloud staging for host output
   double* adj_partials = nullptr; // dL/dxi block partials
   float* gn_partials = nullptr;   // refinement block partials
   int refine_steps_done = 0;      // steps of the last refinement
@@ -229,6 +260,11 @@ size_t kf_adjoint_partials(const kf_context* c);  // doubles
 size_t kf_refine_partials(const kf_context* c);   // floats
 hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps);
 hipError_t kf_launch_volume_fill(kf_context* c);
+// dequantisation + inverse DCT of the coefficient blocks in jpeg_coef, then
+// upsampling and colour conversion into color
+hipError_t kf_launch_jpeg_decode(kf_context* c, const kf_jpeg_dev& f);
+// the point cloud of the model maps and the colour image into out (H*W)
+hipError_t kf_launch_point_cloud(kf_context* c, float4* out);
 hipError_t kf_launch_set_pose(kf_context* c, const float* T_host);
 hipError_t kf_launch_record_pose(kf_context* c, int frame);
 

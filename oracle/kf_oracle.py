@@ -68,6 +68,11 @@ def _load(path=LIB_PATH):
                                         _U8]),
         "oracle_pyramid_intrinsics": (None, [_F, _F]),
         "oracle_set_threads": (None, [C.c_int]),
+        "oracle_jpeg_decode": (C.c_int, [_U8, C.c_int64, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), _U8]),
+        "oracle_point_cloud": (None, [_F, _F, _F, _U8, C.c_int, C.c_int, _F]),
+        "oracle_jpeg_coefficients": (C.c_int, [_U8, C.c_int64, C.POINTER(C.c_int16),
+                                               C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -377,3 +382,44 @@ def icp_pyramid(levels, T_prev, iters, dist_gate, angle_gate, reduce_mode=1):
         record.append((l, r))
         T = r["T"]
     return T, record
+
+
+def jpeg_decode(data):
+    """cv::imread(path, IMREAD_COLOR) of a baseline JPEG held in `data`
+    (bytes): H x W x 3 uint8 in B, G, R order, or None where the oracle
+    rejects the stream (system_utility.cpp:70-75)."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    h, w = C.c_int(0), C.c_int(0)
+    if lib.oracle_jpeg_decode(buf.ctypes.data_as(_U8), buf.size, C.byref(h), C.byref(w),
+                              None) != 0:
+        return None
+    out = np.zeros((h.value, w.value, 3), np.uint8)
+    if lib.oracle_jpeg_decode(buf.ctypes.data_as(_U8), buf.size, C.byref(h), C.byref(w),
+                              out.ctypes.data_as(_U8)) != 0:
+        return None
+    return out
+
+
+def point_cloud(x, y, z, bgr):
+    """ThreeDViewer::SetUpPointClouds (threeD_viewer.cpp:20-55): H x W x 4
+    float32 records {x, y, z, rgb}, rgb the bits of r << 16 | g << 8 | b."""
+    z = _c32(z)
+    H, W = z.shape
+    out = np.zeros((H, W, 4), np.float32)
+    b = np.ascontiguousarray(bgr, np.uint8)
+    lib.oracle_point_cloud(_f(_c32(x)), _f(_c32(y)), _f(z), b.ctypes.data_as(_U8), H, W, _f(out))
+    return out
+
+
+def jpeg_coefficients(data):
+    """The entropy-decoded coefficient blocks (n x 64 int16, natural order,
+    not dequantised), component after component over whole MCUs."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    nb = C.c_int64(0)
+    if lib.oracle_jpeg_coefficients(buf.ctypes.data_as(_U8), buf.size, None, C.byref(nb)) != 0:
+        return None
+    out = np.zeros((nb.value, 64), np.int16)
+    if lib.oracle_jpeg_coefficients(buf.ctypes.data_as(_U8), buf.size,
+                                    out.ctypes.data_as(C.POINTER(C.c_int16)), C.byref(nb)) != 0:
+        return None
+    return out
