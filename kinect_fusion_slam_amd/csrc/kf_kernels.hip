@@ -526,6 +526,20 @@ __global__ __launch_bounds__(256) void k_model_down(
   ok_out[i] = ok ? 1 : 0;
 }
 
+// Consecutive block ids are dealt round-robin to the 8 XCDs, each with its
+// own L2.  xcd_contiguous maps a block id to a work index so that each XCD
+// takes a contiguous run of the n_blocks work items (neighbouring pixels
+// gather neighbouring model pixels; neighbouring tiles read neighbouring
+// bricks): XCD x receives hardware blocks x, x+8, ...: q+1 of them for
+// x < r, q otherwise.  It is a bijection for any n_blocks, so results do not
+// depend on how the hardware actually deals blocks.
+__device__ __forceinline__ int xcd_contiguous(int bid, int n_blocks) {
+  const int q = n_blocks >> 3, r = n_blocks & 7;
+  const int xcd = bid & 7, loc = bid >> 3;
+  const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return base + loc;
+}
+
 // ---------------------------------------------------------------------------
 // A5-A7  ICP.
 
@@ -678,11 +692,15 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   __shared__ PT red_c[KAHAN ? KF_ICP_BLOCK / 64 : 1][NS];
   const int nb = gridDim.x;
 
-  // block b takes pixels [b bpx, (b + 1) bpx) of the image, thread t the
-  // pixels t, t + KF_ICP_BLOCK, ... below bpx (bpx <= PX KF_ICP_BLOCK)
+  // chunk c takes pixels [c bpx, (c + 1) bpx) of the image, thread t the
+  // pixels t, t + KF_ICP_BLOCK, ... below bpx (bpx <= PX KF_ICP_BLOCK); each
+  // XCD takes a contiguous band of chunks, so the model rows its blocks
+  // gather are shared in its L2 instead of fetched by two XCDs at every
+  // chunk boundary.  The partials stay indexed by block.
   const int n = H * W;
-  const int base = blockIdx.x * bpx + threadIdx.x;
-  const int lim = min(bpx, n - blockIdx.x * bpx) - (int)threadIdx.x;  // k valid: k * KF_ICP_BLOCK < lim
+  const int chunk = xcd_contiguous(blockIdx.x, nb);
+  const int base = chunk * bpx + threadIdx.x;
+  const int lim = min(bpx, n - chunk * bpx) - (int)threadIdx.x;  // k valid: k * KF_ICP_BLOCK < lim
   // phase 1: the previous iteration's block partials and the two poses it
   // needs, then this thread's vertex and normal loads, all issued before
   // the previous iteration is finished (the maps do not depend on the pose).
@@ -2565,17 +2583,6 @@ constexpr int kPass1Steps = KF_PASS1_STEPS;  // steps per batch
 #define KF_REST_BLOCKS 2048
 #endif
 constexpr int kRestBlocks = KF_REST_BLOCKS;
-
-// Block number that gives each XCD a contiguous run of the n_blocks tiles
-// (neighbouring tiles read neighbouring bricks, and each XCD has its own L2):
-// XCD x receives hardware blocks x, x+8, ...: q+1 of them for x < r, q
-// otherwise.
-__device__ __forceinline__ int xcd_contiguous(int bid, int n_blocks) {
-  const int q = n_blocks >> 3, r = n_blocks & 7;
-  const int xcd = bid & 7, loc = bid >> 3;
-  const int base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-  return base + loc;
-}
 
 // Per-block dL/dF table of the adjoint: kCells cell keys (-1 free) with 8
 // corner sums each, and the bricks the block has marked in io.touched (a
