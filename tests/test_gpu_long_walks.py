@@ -73,18 +73,35 @@ def adversarial_volume(rng, dims, neg_brick_frac):
     return t, w
 
 
-@pytest.mark.parametrize("neg_brick_frac", [0.0, 0.05, 0.3])
-def test_adversarial_volume_long_walks(kf, oracle, neg_brick_frac):
+@pytest.mark.parametrize("neg_brick_frac,camera", [(0.0, "generic"), (0.05, "generic"),
+                                                    (0.3, "generic"), (0.05, "axis"),
+                                                    (0.3, "axis")])
+def test_adversarial_volume_long_walks(kf, oracle, neg_brick_frac, camera):
+    """camera "axis": fx = fy = 128 and the principal point (64, 48) make
+    K^-1 exact, so the rays of pixel column 63 and row 47 (1-indexed 64, 48)
+    have a direction component of exactly +0 at the identity pose (an axis
+    that never steps), and pixel (63, 47) walks along z alone."""
     rng = np.random.default_rng(int(neg_brick_frac * 1000) + 7)
     vol, prm, K, H, W = small_setup(kf, H=96, W=128, dims=(70, 66, 75), start=(-35, -33, -10),
                                     voxel=20.0)
-    K = np.array([110.0, 0, 63.5, 0, 110.0, 47.5, 0, 0, 1], np.float32)
+    if camera == "generic":
+        K = np.array([110.0, 0, 63.5, 0, 110.0, 47.5, 0, 0, 1], np.float32)
+    else:
+        K = np.array([128.0, 0, 64.0, 0, 128.0, 48.0, 0, 0, 1], np.float32)
+        Ki = kf.mat3_inverse(K).ravel()
+        assert Ki[0] * 64 + Ki[2] == 0 and Ki[4] * 48 + Ki[5] == 0
     t, w = adversarial_volume(rng, tuple(vol.dims), neg_brick_frac)
     d = rng.uniform(300.0, 1500.0, (H, W)).astype(np.float32)
     d[rng.random((H, W)) < 0.5] = np.float32(4.8e-5)  # rays from behind the camera
     d[rng.random((H, W)) < 0.05] = 0.0
     T = np.eye(4, dtype=np.float32)
-    say(f"[gpu] long walks: adversarial volume, negative brick fraction {neg_brick_frac}")
+    if camera == "axis":
+        # long walks on the zero-component column and row
+        assert (d[:, 63] == np.float32(4.8e-5)).sum() > 10
+        assert (d[47, :] == np.float32(4.8e-5)).sum() > 10
+        d[47, 63] = np.float32(4.8e-5)
+    say(f"[gpu] long walks: adversarial volume, negative brick fraction {neg_brick_frac}, "
+        f"{camera} camera")
     with kf.KinfuContext(vol, prm, H, W) as c:
         c.upload_volume(t, w)
         c.upload_depth_f32(d)

@@ -63,7 +63,7 @@ def test_depth_gate_bitexact(ctx_room, room_stream, oracle):
                                                     prm.depth_max_mm), "depth")
 
 
-@pytest.mark.parametrize("radius", [1, 2, 3, 5])
+@pytest.mark.parametrize("radius", [1, 2, 3, 5, 7])
 def test_bilateral_bitexact(ctx_room, room_stream, oracle, radius):
     vol, prm, K, H, W, frames, gt = room_stream
     say(f"[gpu] bilateral r={radius}")
