@@ -1,11 +1,14 @@
 """The host/device math shared by the kernels and the oracle, pinned against
 numpy/libm and closed forms (no GPU needed)."""
+import ctypes as C
 import math
+import os
+from fractions import Fraction
 
 import numpy as np
 import pytest
 
-from conftest import ROOT  # noqa: F401  (puts the repo on sys.path)
+from conftest import ROOT
 
 
 def ordered(a):
@@ -172,3 +175,170 @@ def test_dist_gate_threshold_equals_the_sqrt_test(kf, gate):
     cand = cand[~(cand < 0)]  # a float sum of squares is >= +0 or NaN
     for s in cand:
         assert passes(s) == bool(s <= hi), (gate, s)
+
+
+# ---------------------------------------------------------------------------
+# Helpers reached only through the shared header (tests/shim/hd_math_shim.cpp
+# exposes their host build): pinned against exact rational arithmetic,
+# numpy and closed forms.
+
+SHIM = os.path.join(ROOT, "tests", "shim", "libhd_math_shim.so")
+
+
+@pytest.fixture(scope="module")
+def hd():
+    if not os.path.exists(SHIM):
+        pytest.fail(f"{SHIM} is not built (make -C tests/shim)")
+    lib = C.CDLL(SHIM)
+    f, d = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    lib.hd_round_to_int.argtypes = [C.c_float]
+    lib.hd_floor_to_int.argtypes = [C.c_float]
+    lib.hd_mat4_mul.argtypes = [f, f, f]
+    lib.hd_sym_index.argtypes = [C.c_int, C.c_int]
+    lib.hd_icp_increment.argtypes = [d, f]
+    lib.hd_icp_increment_robust.argtypes = [d, f, C.c_double]
+    lib.hd_icp_update_gated.argtypes = [d, f, C.c_double]
+    return lib
+
+
+def _fp(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _edge_floats():
+    halves = [k + 0.5 for k in range(-6, 6)]
+    xs = [0.0, -0.0, 1.0, -1.0, 2.5e-8, -2.5e-8, 1e9, -1e9, 2147483520.0, -2147483648.0,
+          2147483648.0, -2147483904.0, 3e38, -3e38]
+    for h in halves + [639.5, 479.5, -639.5]:
+        xs += [h, float(np.nextafter(np.float32(h), np.float32(np.inf))),
+               float(np.nextafter(np.float32(h), np.float32(-np.inf)))]
+    rng = np.random.default_rng(3)
+    xs += list(rng.uniform(-2000, 2000, 400).astype(np.float32))
+    return [float(np.float32(x)) for x in xs]
+
+
+def test_round_and_floor_to_int_match_exact_arithmetic(hd):
+    """roundf: the nearest integer, halves away from zero; floorf: the
+    largest integer not above; both of the exact float value.  NaN, infinities
+    and values outside int32 give INT32_MIN."""
+    int_min = -2 ** 31
+    for x in _edge_floats():
+        q = Fraction(x)
+        near = math.floor(abs(q) + Fraction(1, 2))
+        want_r = int(near if q >= 0 else -near)
+        want_f = math.floor(q)
+        want_r = want_r if -2 ** 31 <= want_r < 2 ** 31 and -2 ** 31 < x < 2 ** 31 else int_min
+        want_f = want_f if -2 ** 31 <= want_f < 2 ** 31 and -2 ** 31 < x < 2 ** 31 else int_min
+        assert hd.hd_round_to_int(x) == want_r, x
+        assert hd.hd_floor_to_int(x) == want_f, x
+    for x in (float("nan"), float("inf"), -float("inf")):
+        assert hd.hd_round_to_int(x) == int_min and hd.hd_floor_to_int(x) == int_min
+
+
+def test_mat4_mul_within_half_ulp_of_exact_product(hd):
+    """Each element is the FP64 sum of four exact products rounded once to
+    float: the exact rational product, rounded to float, differs from it by
+    at most the FP64 rounding of the sum (far below half a float ulp except
+    at float ties), so the two agree to one ulp and almost always exactly; a
+    transposed operand or a dropped term misses by far more."""
+    rng = np.random.default_rng(5)
+    exact_equal = 0
+    for trial in range(200):
+        A = rng.uniform(-3, 3, 16).astype(np.float32)
+        B = rng.uniform(-3, 3, 16).astype(np.float32)
+        if trial % 4 == 0:
+            A[12:] = [0, 0, 0, 1]
+            B[12:] = [0, 0, 0, 1]
+        C_ = np.zeros(16, np.float32)
+        hd.hd_mat4_mul(_fp(A, C.c_float), _fp(B, C.c_float), _fp(C_, C.c_float))
+        for i in range(4):
+            for j in range(4):
+                s = sum(Fraction(float(A[4 * i + k])) * Fraction(float(B[4 * k + j]))
+                        for k in range(4))
+                want = np.float32(float(s))  # float(s) is s rounded to FP64
+                got = C_[4 * i + j]
+                assert ulp_diff(got, want) <= 1, (trial, i, j)
+                exact_equal += int(got == want)
+    assert exact_equal >= 0.99 * 200 * 16
+    # C may alias A
+    A = rng.uniform(-1, 1, 16).astype(np.float32)
+    I4 = np.eye(4, dtype=np.float32).ravel()
+    A0 = A.copy()
+    hd.hd_mat4_mul(_fp(A, C.c_float), _fp(I4, C.c_float), _fp(A, C.c_float))
+    assert np.array_equal(A, A0)
+
+
+def test_sym_index_enumerates_the_upper_triangle_row_major(hd):
+    order = [(i, j) for i in range(6) for j in range(i, 6)]
+    assert [hd.hd_sym_index(i, j) for (i, j) in order] == list(range(21))
+
+
+def _sums_from_system(A, b, count):
+    s = np.zeros(28)
+    q = 0
+    for i in range(6):
+        for j in range(i, 6):
+            s[q] = A[i, j]
+            q += 1
+    s[21:27] = b
+    s[27] = count
+    return s
+
+
+def test_icp_increment_is_the_small_angle_matrix_of_the_solution(hd):
+    """x = A^-1 b (numpy, FP64) with x = (beta, gamma, alpha, tx, ty, tz);
+    the increment is [[1, x2, -x1, x3], [-x2, 1, x0, x4], [x1, -x0, 1, x5],
+    [0, 0, 0, 1]] (sensor_pose_estimation.cu:406-425): a wrong sign or
+    index of any entry fails."""
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        J = rng.normal(size=(200, 6)) * np.array([3000, 3000, 3000, 1, 1, 1])
+        A = J.T @ J
+        b = rng.normal(size=6) * 1e3
+        x = np.linalg.solve(A, b)
+        inc = np.zeros(16, np.float32)
+        assert hd.hd_icp_increment(_fp(_sums_from_system(A, b, 200), C.c_double),
+                                   _fp(inc, C.c_float)) == 0
+        xf = x.astype(np.float32)
+        want = np.array([[1, xf[2], -xf[1], xf[3]], [-xf[2], 1, xf[0], xf[4]],
+                         [xf[1], -xf[0], 1, xf[5]], [0, 0, 0, 1]], np.float32).ravel()
+        assert ulp_diff(inc, want).max() <= 4
+    # a pivot below FLT_EPSILON: the right-hand side itself is the increment
+    A = np.zeros((6, 6))
+    b = np.array([0.25, -0.5, 0.125, 3.0, -2.0, 1.0])
+    inc = np.zeros(16, np.float32)
+    assert hd.hd_icp_increment(_fp(_sums_from_system(A, b, 1), C.c_double),
+                               _fp(inc, C.c_float)) == 1
+    want = np.array([[1, b[2], -b[1], b[3]], [-b[2], 1, b[0], b[4]], [b[1], -b[0], 1, b[5]],
+                     [0, 0, 0, 1]], np.float32).ravel()
+    assert np.array_equal(inc, want)
+
+
+def test_robust_increment_and_gated_update(hd):
+    """The robust variant gives the identity for a failed solve (status 1)
+    and for a solved system with fewer inliers than the gate (status 2), and
+    the plain increment otherwise; the gated update composes inc * T, and a
+    negative gate selects the plain update."""
+    rng = np.random.default_rng(13)
+    J = rng.normal(size=(100, 6))
+    A, b = J.T @ J, rng.normal(size=6) * 0.01
+    s = _sums_from_system(A, b, 100)
+    plain, rob = np.zeros(16, np.float32), np.zeros(16, np.float32)
+    assert hd.hd_icp_increment(_fp(s, C.c_double), _fp(plain, C.c_float)) == 0
+    assert hd.hd_icp_increment_robust(_fp(s, C.c_double), _fp(rob, C.c_float), 100.0) == 0
+    assert np.array_equal(plain, rob)
+    assert hd.hd_icp_increment_robust(_fp(s, C.c_double), _fp(rob, C.c_float), 101.0) == 2
+    assert np.array_equal(rob, np.eye(4, dtype=np.float32).ravel())
+    z = _sums_from_system(np.zeros((6, 6)), b, 100)
+    assert hd.hd_icp_increment_robust(_fp(z, C.c_double), _fp(rob, C.c_float), 0.0) == 1
+    assert np.array_equal(rob, np.eye(4, dtype=np.float32).ravel())
+    # gated update: T <- inc * T, rounded once per element
+    T = np.array([[0, -1, 0, 100], [1, 0, 0, -50], [0, 0, 1, 2000], [0, 0, 0, 1]],
+                 np.float32).ravel()
+    got = T.copy()
+    assert hd.hd_icp_update_gated(_fp(s, C.c_double), _fp(got, C.c_float), -1.0) == 0
+    want = (plain.reshape(4, 4).astype(np.float64) @ T.reshape(4, 4).astype(np.float64))
+    assert np.array_equal(got, want.astype(np.float32).ravel())
+    got = T.copy()
+    assert hd.hd_icp_update_gated(_fp(s, C.c_double), _fp(got, C.c_float), 101.0) == 2
+    assert np.array_equal(got, T)
