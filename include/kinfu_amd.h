@@ -137,6 +137,13 @@ typedef struct kf_params {
   float pyramid_max_step_mm; /* a coarse depth pixel averages its 2x2 block
                                 only where the four depths differ by at most
                                 this, else it is 0                          */
+  float icp_stop_rot;       /* opt-in early stop (0 = off, the reference's
+                               fixed iteration count): the full-resolution
+                               iterations end after an increment whose
+                               rotation parameters are all below this (rad)
+                               and whose translation is below
+                               icp_stop_trans_mm (no reference counterpart) */
+  float icp_stop_trans_mm;
 } kf_params;
 
 /* Raycast hit handling.  REFERENCE: the reference's per-axis interpolation

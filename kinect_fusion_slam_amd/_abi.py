@@ -53,7 +53,8 @@ class Params(C.Structure):
                 ("raycast_mode", C.c_int32), ("refine_steps", C.c_int32),
                 ("refine_max_residual_mm", C.c_float), ("icp_robust", C.c_int32),
                 ("icp_min_inliers", C.c_int32), ("icp_levels", C.c_int32),
-                ("icp_coarse_iters", C.c_int32 * 2), ("pyramid_max_step_mm", C.c_float)]
+                ("icp_coarse_iters", C.c_int32 * 2), ("pyramid_max_step_mm", C.c_float),
+                ("icp_stop_rot", C.c_float), ("icp_stop_trans_mm", C.c_float)]
 
 
 class SynthScene(C.Structure):

@@ -54,6 +54,15 @@ __global__ void k_set_pose(kf_dev_state* st, kf_pose_arg p) {
   }
 }
 
+__global__ void k_set_icp_stop(kf_dev_state* st, int from, float rot, float trans) {
+  if (threadIdx.x == 0) {
+    st->icp_stop_at = 0;
+    st->icp_stop_from = from;
+    st->icp_stop_rot = rot;
+    st->icp_stop_trans = trans;
+  }
+}
+
 // Order-preserving map of floats to integers (for bisection over floats).
 int64_t f2ord(float f) {
   int32_t b;
@@ -160,6 +169,11 @@ bool params_valid(const kf_params* p, std::string* why) {
     *why = "icp_levels must be in [1, 3], icp_coarse_iters and pyramid_max_step_mm non-negative";
     return false;
   }
+  if (!(p->icp_stop_rot >= 0 && p->icp_stop_rot < INFINITY) ||
+      !(p->icp_stop_trans_mm >= 0 && p->icp_stop_trans_mm < INFINITY)) {
+    *why = "icp_stop_rot and icp_stop_trans_mm must be finite and non-negative";
+    return false;
+  }
   int total = p->icp_iterations;
   for (int l = 1; l < p->icp_levels; ++l) total += p->icp_coarse_iters[l - 1];
   if (total > KF_MAX_ICP_ITERS) {
@@ -208,6 +222,20 @@ void host_spatial_kernel(int r, float sigma, float* w) {
 }
 
 }  // namespace
+
+int kf_icp_stop_from(const kf_params& p) {
+  int total = p.icp_iterations;
+  for (int l = 1; l < p.icp_levels; ++l) total += p.icp_coarse_iters[l - 1];
+  return total - p.icp_iterations + 1;
+}
+
+hipError_t kf_launch_set_icp_stop(kf_dev_state* st, const kf_params& p, hipStream_t s) {
+  const bool on = kf_icp_may_stop(p);
+  k_set_icp_stop<<<1, 64, 0, s>>>(st, on ? kf_icp_stop_from(p) : KF_MAX_ICP_ITERS + 1,
+                                   on ? p.icp_stop_rot : 0.0f, on ? p.icp_stop_trans_mm : 0.0f);
+  return hipGetLastError();
+}
+
 
 void kf_make_cam(const float* K, kf_cam* cam) {
   for (int i = 0; i < 9; ++i) cam->K[i] = K[i];
@@ -347,6 +375,8 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   p.icp_coarse_iters[0] = 5;
   p.icp_coarse_iters[1] = 4;
   p.pyramid_max_step_mm = 30.0f;
+  p.icp_stop_rot = 0.0f;  // the reference's fixed iteration count
+  p.icp_stop_trans_mm = 0.0f;
   switch (which) {
     case KF_PRESET_REFERENCE: {
       // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,
@@ -652,6 +682,8 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       hipMemsetAsync(c->integ_counts, 0, (size_t)4 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4,
                      c->stream) != hipSuccess)
     return fail(KF_ERR_HIP, "hipMemsetAsync failed");
+  if (kf_launch_set_icp_stop(c->state, c->prm, c->stream) != hipSuccess)
+    return fail(KF_ERR_HIP, "kernel launch failed");
   *out = c;
   rc = kf_reset(c);
   if (rc != KF_OK) {
@@ -666,7 +698,10 @@ int kf_set_params(kf_context* c, const kf_params* prm) {
   KF_REQUIRE(c && prm, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
-  return apply_params(c, prm);
+  const int rc = apply_params(c, prm);
+  if (rc != KF_OK) return rc;
+  KF_LAUNCH(kf_launch_set_icp_stop(c->state, c->prm, c->stream));
+  return KF_OK;
 }
 
 int kf_get_params(const kf_context* c, kf_params* prm) {
@@ -962,6 +997,7 @@ int kf_estimate_pose_static(const float* vmap, const float* nmap, const float* m
   kf_make_cam(K, &cam);
   k_set_pose<<<1, 64, 0, c.stream>>>(c.state, a);
   KF_HIP_CHECK(hipGetLastError());
+  KF_LAUNCH(kf_launch_set_icp_stop(c.state, c.prm, c.stream));
   KF_LAUNCH(kf_launch_icp(&c, cam));
   k_pose_out<<<1, 64, 0, c.stream>>>(c.state, T_out);
   KF_HIP_CHECK(hipGetLastError());

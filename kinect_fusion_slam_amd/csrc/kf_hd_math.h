@@ -256,6 +256,25 @@ KF_HD int kf_icp_increment_robust(const double* sums, float* inc, double min_inl
   return status;
 }
 
+// The increment of either update: the robust one when min_inliers >= 0,
+// else the reference's.
+KF_HD int kf_icp_increment_gated(const double* sums, float* inc, double min_inliers) {
+  return min_inliers >= 0.0 ? kf_icp_increment_robust(sums, inc, min_inliers)
+                            : kf_icp_increment(sums, inc);
+}
+
+// Opt-in early stop (no reference counterpart; the reference always runs its
+// 10 iterations, sensor_pose_estimation.cu:339): an increment whose rotation
+// parameters (x0, x1, x2, radians) are all below stop_rot in magnitude and
+// whose translation (x3, x4, x5, mm) is below stop_trans is the last one
+// applied.  inc is kf_icp_increment's matrix, so x0 = inc[6], x1 = inc[8],
+// x2 = inc[1], x3 = inc[3], x4 = inc[7], x5 = inc[11].  A threshold of 0
+// never stops, and neither does a NaN parameter.
+KF_HD bool kf_icp_converged(const float* inc, float stop_rot, float stop_trans) {
+  return fabsf(inc[6]) < stop_rot && fabsf(inc[8]) < stop_rot && fabsf(inc[1]) < stop_rot &&
+         fabsf(inc[3]) < stop_trans && fabsf(inc[7]) < stop_trans && fabsf(inc[11]) < stop_trans;
+}
+
 // One ICP update: T <- inc * T (sensor_pose_estimation.cu:428).  Returns the
 // solver status.
 KF_HD int kf_icp_update(const double* sums, float* T) {

@@ -20,6 +20,10 @@ struct kf_dev_state {
   float F[16];          // Tprev_inv * T (frame_to_frame, sensor_pose_estimation.cu:343)
   float center_dir[4];  // integration front-test direction (projective_tsdf.cu:57-71)
   int32_t icp_iters_done;
+  int32_t icp_stop_at;    // opt-in early stop: the first iteration that did not run (0: none)
+  int32_t icp_stop_from;  //   the first iteration whose kernel may stop
+  float icp_stop_rot;     //   thresholds (kf_icp_converged)
+  float icp_stop_trans;
   unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
   unsigned long long ray_hits;
   double xi_grad[6];             // raycast adjoint: dL/dxi (omega, v)
@@ -248,6 +252,15 @@ hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam);
 // vertex, normal and mask maps from the working depth (instead of
 // kf_launch_vertex) and writes them to the context's maps.
 hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam, bool maps_from_depth = false);
+// Opt-in ICP early stop (kf_params icp_stop_rot / icp_stop_trans_mm, both > 0):
+// whether it is on, and the first iteration whose kernel may stop it (the
+// coarse pyramid levels always run in full).  kf_launch_set_icp_stop stores
+// the thresholds and that iteration in the device state.
+inline bool kf_icp_may_stop(const kf_params& p) {
+  return p.icp_stop_rot > 0 && p.icp_stop_trans_mm > 0;
+}
+int kf_icp_stop_from(const kf_params& p);
+hipError_t kf_launch_set_icp_stop(kf_dev_state* st, const kf_params& p, hipStream_t s);
 // Intrinsics of the next coarser pyramid level (K and Kc: 3x3 row-major).
 void kf_pyramid_intrinsics(const float* K, float* Kc);
 hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam);

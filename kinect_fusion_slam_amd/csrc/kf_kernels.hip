@@ -706,7 +706,10 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 // loads per pixel, raw_vertices.cu:3-51 through vertex_at) and written to
 // vmap_out / nmap_out / vmask_out for the later iterations, so that the
 // pipeline needs no separate vertex kernel.
-template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH>
+// STOP: the opt-in early stop is compiled in (kf_icp_converged, with the
+// thresholds and the first iteration that may stop in the device state), so
+// that the reference's fixed iteration count runs the kernel without it.
+template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH, bool STOP>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const float* __restrict__ depth, kf_p3* __restrict__ vmap_out, kf_p3* __restrict__ nmap_out,
     uint8_t* __restrict__ vmask_out, const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
@@ -761,6 +764,10 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   const int e16 = threadIdx.x & 15;
   const float T0e = iter <= 1 ? stp->Tprev[e16] : stp->iter_T[iter - 2][e16];
   const float Tpe = stp->Tprev_inv[e16];
+  // opt-in early stop: the first iteration that may stop, and whether an
+  // earlier iteration of this frame stopped the ICP
+  const int stop_from = STOP ? stp->icp_stop_from : 0;
+  const int stop_at = STOP ? stp->icp_stop_at : 0;
   asm volatile("" ::: "memory");
   kf_p3 V[PX], Nc[PX];
   // FROM_DEPTH: each pixel's depth and those below and to the right of it
@@ -796,7 +803,9 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       st->Tprev[threadIdx.x] = sh.Tn[threadIdx.x];
       st->Tprev_inv[threadIdx.x] = sh.Tp[threadIdx.x];
     }
+    if (STOP && blockIdx.x == 0 && threadIdx.x == 0) st->icp_stop_at = 0;
   } else {
+    if (STOP && iter > stop_from && stop_at) return;  // uniform: one record for every block
     // finish iteration iter-1, which started from the pose recorded for
     // iter-2 (or from T_prev for iter-1 == 0)
     if (threadIdx.x < 16) {
@@ -806,6 +815,14 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     icp_sum_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(R, nb_in, sh);
     icp_update_pose<ROWS>(sh, min_inliers);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
+    // opt-in early stop after a small increment (every block decides alike
+    // from the same LDS values): iteration iter-1 was the last; the later
+    // iterations and the finishing kernel read icp_stop_at
+    if (STOP && iter >= stop_from &&
+        kf_icp_converged(sh.inc, st->icp_stop_rot, st->icp_stop_trans)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->icp_stop_at = iter;
+      return;
+    }
   }
   if constexpr (FROM_DEPTH) {
     // the frame's vertex, normal and mask (raw_vertices.cu:3-51), kept for
@@ -989,17 +1006,32 @@ constexpr int kIcpFinishThreads = 1024;
 template <bool KAHAN, int PPL>
 __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* __restrict__ st,
                                                              const void* __restrict__ part_in,
-                                                             int nb, int last, float min_inliers) {
+                                                             int nb, int last, float min_inliers,
+                                                             int may_stop) {
   constexpr int ROWS = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
   __shared__ IcpShared<ROWS> sh;
-  if (threadIdx.x < 16) {
-    sh.T0[threadIdx.x] = last == 0 ? st->Tprev[threadIdx.x] : st->iter_T[last - 1][threadIdx.x];
-    sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
+  const int stopped = may_stop ? st->icp_stop_at : 0;
+  if (stopped) {
+    // the ICP stopped early: the final pose is the one recorded for the last
+    // iteration that ran
+    if (threadIdx.x < 16) {
+      sh.Tn[threadIdx.x] = st->iter_T[stopped - 1][threadIdx.x];
+      sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16)
+      sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
+    __syncthreads();
+  } else {
+    if (threadIdx.x < 16) {
+      sh.T0[threadIdx.x] = last == 0 ? st->Tprev[threadIdx.x] : st->iter_T[last - 1][threadIdx.x];
+      sh.Tp[threadIdx.x] = st->Tprev_inv[threadIdx.x];
+    }
+    icp_reduce_partials<ROWS, kIcpFinishThreads, PPL>(
+        static_cast<const IcpPartial<KAHAN>*>(part_in), nb, sh);
+    icp_update_pose<ROWS>(sh, min_inliers);
+    icp_record<ROWS>(st, last, sh);
   }
-  icp_reduce_partials<ROWS, kIcpFinishThreads, PPL>(static_cast<const IcpPartial<KAHAN>*>(part_in),
-                                               nb, sh);
-  icp_update_pose<ROWS>(sh, min_inliers);
-  icp_record<ROWS>(st, last, sh);
   if (threadIdx.x < 16) {
     st->T[threadIdx.x] = sh.Tn[threadIdx.x];
     st->F[threadIdx.x] = sh.F[threadIdx.x];
@@ -3107,15 +3139,28 @@ struct IcpLevel {
 
 template <bool KAHAN, int PX, int PPL>
 static void launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin, void* pout,
-                               uint8_t* m, int it, int nb_in, float min_inl, bool from_depth) {
-  if (from_depth)
-    k_icp_iter<KAHAN, PX, PPL, true><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(
-        L.depth, L.vmap_w, L.nmap_w, L.mask_w, L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W,
-        c->icp_dist2_hi, c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in);
-  else
-    k_icp_iter<KAHAN, PX, PPL, false><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(
-        nullptr, nullptr, nullptr, nullptr, L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W,
-        c->icp_dist2_hi, c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in);
+                               uint8_t* m, int it, int nb_in, float min_inl, bool from_depth,
+                               bool may_stop) {
+  const float* d = from_depth ? L.depth : nullptr;
+  kf_p3* vw = from_depth ? L.vmap_w : nullptr;
+  kf_p3* nw = from_depth ? L.nmap_w : nullptr;
+  uint8_t* mw = from_depth ? L.mask_w : nullptr;
+#define KF_ICP_ITER(FD, ST)                                                                   \
+  k_icp_iter<KAHAN, PX, PPL, FD, ST><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(                \
+      d, vw, nw, mw, L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W, c->icp_dist2_hi,             \
+      c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in)
+  if (may_stop) {
+    if (from_depth)
+      KF_ICP_ITER(true, true);
+    else
+      KF_ICP_ITER(false, true);
+  } else {
+    if (from_depth)
+      KF_ICP_ITER(true, false);
+    else
+      KF_ICP_ITER(false, false);
+  }
+#undef KF_ICP_ITER
 }
 
 // The iterations of every level, coarsest first (lv[n_levels - 1] ... lv[0]),
@@ -3133,6 +3178,9 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
   const float min_inl = c->prm.icp_robust ? (float)c->prm.icp_min_inliers : -1.0f;
   int total = 0;
   for (int l = 0; l < n_levels; ++l) total += lv[l].iters;
+  // opt-in early stop (at full resolution only; its thresholds and first
+  // iteration are in the device state, kf_icp_stop_from)
+  const bool may_stop = kf_icp_may_stop(c->prm);
   int it = 0, nb_prev = lv[n_levels - 1].blocks;
   for (int l = n_levels - 1; l >= 0; --l) {
     const IcpLevel& L = lv[l];
@@ -3145,16 +3193,20 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
       if (it == 0) kf_stage_begin(c, KF_STAGE_ICP_ITER);
       switch (L.px) {
         case 1:
-          launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
+          launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                             may_stop);
           break;
         case 2:
-          launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
+          launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                             may_stop);
           break;
         case 3:
-          launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
+          launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                             may_stop);
           break;
         default:
-          launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd);
+          launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                             may_stop);
           break;
       }
       hipError_t e = hipGetLastError();
@@ -3165,8 +3217,8 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
   }
   if (total > 0) {
     const void* pin = pbase + (size_t)((total - 1) & 1) * half;
-    k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(c->state, pin, nb_prev,
-                                                                      total - 1, min_inl);
+    k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(
+        c->state, pin, nb_prev, total - 1, min_inl, may_stop ? 1 : 0);
   }
   return hipGetLastError();
 }

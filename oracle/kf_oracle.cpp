@@ -1081,7 +1081,8 @@ void oracle_icp(const float* vx, const float* vy, const float* vz, const float* 
                 const float* snz, int H, int W, const float* K, float* T_inout,
                 const float* T_start, int iters, float dist_gate, float angle_gate,
                 int reduce_mode, float* T_iters, double* sums_out, int32_t* status_out,
-                uint8_t* masks, double min_inliers) {
+                uint8_t* masks, double min_inliers, float stop_rot, float stop_trans,
+                int32_t* iters_done) {
   const IcpInputs in{vx, vy, vz, nx, ny, nz, mask, sx, sy, sz, snx, sny, snz, H, W};
   const int64_t n = (int64_t)H * W;
   const int64_t nblk = (n + 63) / 64;
@@ -1089,6 +1090,7 @@ void oracle_icp(const float* vx, const float* vy, const float* vz, const float* 
   kf_m4_inverse(T_inout, Tprev_inv);
   for (int i = 0; i < 16; ++i) Te[i] = T_start ? T_start[i] : T_inout[i];
   std::vector<float> part((size_t)nblk * 28);
+  int done = 0;
   for (int it = 0; it < iters; ++it) {
     float F[16];
     kf_mat4_mul(Tprev_inv, Te, F);
@@ -1128,14 +1130,21 @@ void oracle_icp(const float* vx, const float* vy, const float* vz, const float* 
       }
     }
     // min_inliers < 0: the reference's update; else the robust one
-    const int st = kf_icp_update_gated(sums, Te, min_inliers);
+    // (kf_icp_update_gated, written out to keep the increment)
+    float inc[16];
+    const int st = kf_icp_increment_gated(sums, inc, min_inliers);
+    kf_mat4_mul(inc, Te, Te);
     if (T_iters)
       for (int i = 0; i < 16; ++i) T_iters[it * 16 + i] = Te[i];
     if (sums_out)
       for (int k = 0; k < 28; ++k) sums_out[it * 28 + k] = sums[k];
     if (status_out) status_out[it] = st;
+    done = it + 1;
+    // opt-in early stop (no reference counterpart; 0 thresholds never stop)
+    if (kf_icp_converged(inc, stop_rot, stop_trans)) break;
   }
   for (int i = 0; i < 16; ++i) T_inout[i] = Te[i];
+  if (iters_done) *iters_done = done;
 }
 
 // ---------------------------------------------------------------------------
@@ -1152,7 +1161,7 @@ void oracle_pipeline(const uint16_t* frames, int n_frames, int H, int W, const f
                      float smax, float dmin, float dmax, int bilateral_r, float sigma_s,
                      float sigma_r, int icp_iters, float dist_gate, float angle_gate,
                      int reduce_mode, int fix_bounds, float* tsdf, float* weight,
-                     float* poses_out, double* stage_ms) {
+                     float* poses_out, double* stage_ms, float stop_rot, float stop_trans) {
   const int64_t n = (int64_t)H * W;
   const int64_t nv = (int64_t)dims[0] * dims[1] * dims[2];
   std::fill_n(tsdf, nv, NAN);
@@ -1189,7 +1198,7 @@ void oracle_pipeline(const uint16_t* frames, int n_frames, int H, int W, const f
       oracle_icp(vx.data(), vy.data(), depth.data(), nx.data(), ny.data(), nz.data(),
                  mask.data(), sx.data(), sy.data(), sz.data(), snx.data(), sny.data(),
                  snz.data(), H, W, K, T, nullptr, icp_iters, dist_gate, angle_gate, reduce_mode,
-                 nullptr, nullptr, nullptr, nullptr, -1.0);
+                 nullptr, nullptr, nullptr, nullptr, -1.0, stop_rot, stop_trans, nullptr);
       acc[3] += now() - t3;
     }
     double t4 = now();

@@ -55,11 +55,12 @@ def _load(path=LIB_PATH):
         "oracle_normal_flip": (C.c_int, [C.c_float]),
         "oracle_icp": (None, [_F, _F, _F, _F, _F, _F, _U8, _F, _F, _F, _F, _F, _F, C.c_int,
                               C.c_int, _F, _F, _F, C.c_int, C.c_float, C.c_float, C.c_int, _F,
-                              _D, _I32, _U8, C.c_double]),
+                              _D, _I32, _U8, C.c_double, C.c_float, C.c_float, _I32]),
         "oracle_pipeline": (None, [_U16, C.c_int, C.c_int, C.c_int, _F, _I, _I, C.c_float,
                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                    C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
-                                   C.c_float, C.c_int, C.c_int, _F, _F, _F, _D]),
+                                   C.c_float, C.c_int, C.c_int, _F, _F, _F, _D, C.c_float,
+                                   C.c_float]),
         "oracle_f32_to_f16": (None, [_F, _U16, C.c_int64]),
         "oracle_f16_to_f32": (None, [_U16, _F, C.c_int64]),
         "oracle_max_threads": (C.c_int, []),
@@ -249,10 +250,13 @@ def raycast_tri(depth, K, T, tsdf, vol, trunc, grad_depth=None, grad_normal=None
 
 
 def icp(frame, model, K, T_prev, iters, dist_gate, angle_gate, reduce_mode=0, masks=True,
-        T_start=None, min_inliers=None):
+        T_start=None, min_inliers=None, stop=None):
     """frame: dict vx, vy, vz(depth), nx, ny, nz, mask; model: dict x, y, z, nx, ny, nz.
     min_inliers None: the reference's update (sensor_pose_estimation.cu:389-429);
-    a number: the robust update (a failed solve or fewer inliers: no increment)."""
+    a number: the robust update (a failed solve or fewer inliers: no increment).
+    stop (rot rad, trans mm): the opt-in early stop after an increment below
+    both (kf_icp_converged; no reference counterpart); "iters_done" is the
+    number of iterations run."""
     H, W = frame["vz"].shape
     fr = [_c32(frame[k]) for k in ("vx", "vy", "vz", "nx", "ny", "nz")]
     m = np.ascontiguousarray(frame["mask"], np.uint8)
@@ -262,14 +266,18 @@ def icp(frame, model, K, T_prev, iters, dist_gate, angle_gate, reduce_mode=0, ma
     sums = np.zeros((max(iters, 1), 28), np.float64)
     st = np.zeros(max(iters, 1), np.int32)
     mk = np.zeros((max(iters, 1), H, W), np.uint8)
+    done = np.zeros(1, np.int32)
+    sr, stt = stop if stop is not None else (0.0, 0.0)
     lib.oracle_icp(*(_f(a) for a in fr), m.ctypes.data_as(_U8), *(_f(a) for a in mo), H, W,
                    _f(_c32(K, 9)), _f(T), _f(_c32(T_start, 16)) if T_start is not None else None,
                    iters, dist_gate, angle_gate, reduce_mode, _f(Ts),
                    sums.ctypes.data_as(_D), st.ctypes.data_as(_I32),
                    mk.ctypes.data_as(_U8) if masks else None,
-                   -1.0 if min_inliers is None else float(min_inliers))
-    return {"T": T.reshape(4, 4), "T_iters": Ts[:iters].reshape(-1, 4, 4), "sums": sums[:iters],
-            "status": st[:iters], "masks": mk[:iters] if masks else None}
+                   -1.0 if min_inliers is None else float(min_inliers), float(sr), float(stt),
+                   done.ctypes.data_as(_I32))
+    n = int(done[0])
+    return {"T": T.reshape(4, 4), "T_iters": Ts[:n].reshape(-1, 4, 4), "sums": sums[:n],
+            "status": st[:n], "masks": mk[:n] if masks else None, "iters_done": n}
 
 
 def f32_to_f16_bits(x):
@@ -309,7 +317,7 @@ def pipeline(frames, K, vol, prm, reduce_mode=0, stage_ms=None, timing_build=Fal
                         prm.sdf_max_mm, prm.depth_min_mm, prm.depth_max_mm, r, prm.sigma_s_px,
                         prm.sigma_r_mm, prm.icp_iterations, prm.icp_dist_mm, prm.icp_angle_deg,
                         reduce_mode, prm.raycast_fix_bounds, _f(tsdf), _f(weight), _f(poses),
-                        ms.ctypes.data_as(_D))
+                        ms.ctypes.data_as(_D), prm.icp_stop_rot, prm.icp_stop_trans_mm)
     if stage_ms is not None:
         stage_ms[:] = ms
     return poses.reshape(n, 4, 4), tsdf, weight

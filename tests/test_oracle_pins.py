@@ -607,3 +607,49 @@ def test_icp_robust_inlier_gate(kf, oracle):
     # first iteration is applied only if its count reaches the gate
     part = oracle.icp(fr, model, K, I4, 5, 100.0, 20.0, min_inliers=n0)
     assert part["status"][0] == 0 and np.array_equal(part["T_iters"][0], ref["T_iters"][0])
+
+
+def test_icp_early_stop(kf, oracle):
+    """Opt-in early stop (no reference counterpart): with thresholds, the
+    iterations end right after the first increment whose rotation parameters
+    and translation are all below them, and the pose is that iteration's.
+    The increment of iteration i is T_i T_(i-1)^-1 (FP64 here; the small-angle
+    matrix [[1, x2, -x1, x3], [-x2, 1, x0, x4], [x1, -x0, 1, x5]]), so the
+    stop decision is checked against the recorded poses: a wrong index or
+    sign in the test fails the first or the last of these checks."""
+    scene = kf.preset_scene(kf.PRESET_ROOM)
+    K = intr(131.25, 79.5, 59.5)
+    H, W = 120, 160
+    I4 = np.eye(4, dtype=np.float32)
+    T1 = pose(t=(8.0, -5.0, 6.0), rx=0.004, ry=-0.006, rz=0.003)
+    d0 = kf.render(scene, K, I4, H, W, noise_scale=0.0, hole_frac=0.0).astype(np.float32)
+    d1 = kf.render(scene, K, T1, H, W, noise_scale=0.0, hole_frac=0.0).astype(np.float32)
+    m = _frame_from_depth(kf, oracle, d0, K)
+    model = {"x": m["vx"], "y": m["vy"], "z": d0, "nx": m["nx"], "ny": m["ny"], "nz": m["nz"]}
+    fr = _frame_from_depth(kf, oracle, d1, K)
+    full = oracle.icp(fr, model, K, I4, 10, 100.0, 20.0)
+    assert full["iters_done"] == 10
+    assert oracle.icp(fr, model, K, I4, 10, 100.0, 20.0, stop=(0.0, 0.0))["iters_done"] == 10
+
+    def increment(Ta, Tb):  # x of the increment that took Tb to Ta
+        D = Ta.astype(np.float64) @ np.linalg.inv(Tb.astype(np.float64))
+        return np.array([D[1, 2], D[2, 0], D[0, 1]]), D[:3, 3]
+
+    prev = [I4] + list(full["T_iters"])
+    incs = [increment(prev[i + 1], prev[i]) for i in range(10)]
+    # the motion is recovered in a few iterations, then the increments shrink
+    rot_stop, trans_stop = 5e-6, 5e-3
+    small = [bool(np.abs(r).max() < rot_stop and np.abs(t).max() < trans_stop) for r, t in incs]
+    assert not small[0] and any(small)
+    k = small.index(True)  # the first small increment: iteration k is the last
+    # far from the thresholds, so the FP64 estimate decides as the float one
+    r, t = incs[k]
+    assert np.abs(r).max() < 0.5 * rot_stop and np.abs(t).max() < 0.5 * trans_stop
+    r, t = incs[k - 1]
+    assert np.abs(r).max() > 2 * rot_stop or np.abs(t).max() > 2 * trans_stop
+    got = oracle.icp(fr, model, K, I4, 10, 100.0, 20.0, stop=(rot_stop, trans_stop))
+    assert got["iters_done"] == k + 1
+    assert np.array_equal(got["T_iters"], full["T_iters"][:k + 1])
+    assert np.array_equal(got["T"], full["T_iters"][k])
+    # recovered motion within the small-angle solve's accuracy
+    assert np.abs(got["T"][:3, 3] - T1[:3, 3]).max() < 0.5
