@@ -22,5 +22,8 @@ int hd_icp_increment_robust(const double* sums, float* inc, double min_inliers) 
 int hd_icp_update_gated(const double* sums, float* T, double min_inliers) {
   return kf_icp_update_gated(sums, T, min_inliers);
 }
+int hd_icp_converged(const float* inc, float stop_rot, float stop_trans) {
+  return kf_icp_converged(inc, stop_rot, stop_trans) ? 1 : 0;
+}
 
 }  // extern "C"

@@ -198,6 +198,7 @@ def hd():
     lib.hd_icp_increment.argtypes = [d, f]
     lib.hd_icp_increment_robust.argtypes = [d, f, C.c_double]
     lib.hd_icp_update_gated.argtypes = [d, f, C.c_double]
+    lib.hd_icp_converged.argtypes = [f, C.c_float, C.c_float]
     return lib
 
 
@@ -342,3 +343,39 @@ def test_robust_increment_and_gated_update(hd):
     got = T.copy()
     assert hd.hd_icp_update_gated(_fp(s, C.c_double), _fp(got, C.c_float), 101.0) == 2
     assert np.array_equal(got, T)
+
+
+def test_icp_converged_reads_the_six_increment_parameters(hd):
+    """The opt-in early stop tests |x0..x2| < stop_rot and |x3..x5| <
+    stop_trans, strictly, on the increment matrix's entries x0 = inc[6],
+    x1 = inc[8], x2 = inc[1], x3 = inc[3], x4 = inc[7], x5 = inc[11] (as
+    test_icp_increment_is_the_small_angle_matrix_of_the_solution lays them
+    out); zero thresholds and NaN never stop."""
+    def inc_of(x):
+        return np.array([1, x[2], -x[1], x[3], -x[2], 1, x[0], x[4], x[1], -x[0], 1, x[5],
+                         0, 0, 0, 1], np.float32)
+
+    def conv(x, r=1e-5, t=1e-2):
+        a = inc_of(x)
+        return hd.hd_icp_converged(_fp(a, C.c_float), r, t)
+
+    assert conv([0, 0, 0, 0, 0, 0]) == 1
+    assert conv([0, 0, 0, 0, 0, 0], 0.0, 0.0) == 0
+    for k in range(6):
+        big = [0.0] * 6
+        big[k] = 2e-5 if k < 3 else 2e-2     # above its own threshold
+        assert conv(big) == 0, k
+        near = [0.0] * 6
+        near[k] = 9e-6 if k < 3 else 9e-3    # below it
+        assert conv(near) == 1, k
+        # a rotation value is not compared with the translation threshold,
+        # nor the reverse
+        cross = [0.0] * 6
+        cross[k] = 5e-3 if k < 3 else 5e-6
+        assert conv(cross) == (0 if k < 3 else 1), k
+        edge = [0.0] * 6
+        edge[k] = 1e-5 if k < 3 else 1e-2    # equal: not below
+        assert conv(edge, np.float32(1e-5), np.float32(1e-2)) == 0, k
+        nan = [0.0] * 6
+        nan[k] = float("nan")
+        assert conv(nan) == 0, k
