@@ -20,19 +20,19 @@ struct kf_dev_state {
   float F[16];          // Tprev_inv * T (frame_to_frame, sensor_pose_estimation.cu:343)
   float center_dir[4];  // integration front-test direction (projective_tsdf.cu:57-71)
   int32_t icp_iters_done;
-  int32_t icp_stop_at;    // opt-in early stop: the first iteration that did not run (0: none)
-  int32_t icp_stop_from;  //   the first iteration whose kernel may stop
-  float icp_stop_rot;     //   thresholds (kf_icp_converged)
-  float icp_stop_trans;
   unsigned long long ray_steps;  // raycast work counters (voxels visited, hits)
   unsigned long long ray_hits;
   double xi_grad[6];             // raycast adjoint: dL/dxi (omega, v)
   double refine_msr[KF_MAX_REFINE_STEPS];     // refinement: mean squared residual
   int32_t refine_count[KF_MAX_REFINE_STEPS];  //   pixels used
   int32_t refine_status[KF_MAX_REFINE_STEPS]; //   0 solved, else no update
-  float iter_T[KF_MAX_ICP_ITERS][16];
+  alignas(64) float iter_T[KF_MAX_ICP_ITERS][16];  // one 64-byte line per iteration
   double iter_sums[KF_MAX_ICP_ITERS][KF_ICP_SUMS];
   int32_t iter_status[KF_MAX_ICP_ITERS];
+  int32_t icp_stop_at;    // opt-in early stop: the first iteration that did not run (0: none)
+  int32_t icp_stop_from;  //   the first iteration whose kernel may stop
+  float icp_stop_rot;     //   thresholds (kf_icp_converged)
+  float icp_stop_trans;
 };
 
 // A packed 3-vector of an image map (12 bytes, 4-byte aligned): a wave's 64

@@ -1003,15 +1003,16 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 // Finishes the last iteration: final pose, its inverse and the record.
 constexpr int kIcpFinishThreads = 1024;
 
-template <bool KAHAN, int PPL>
+// STOP: the opt-in early stop may have ended the ICP before the last
+// iteration (compiled only into the kernel of the runs that use it).
+template <bool KAHAN, int PPL, bool STOP>
 __global__ __launch_bounds__(kIcpFinishThreads) void k_icp_finish(kf_dev_state* __restrict__ st,
                                                              const void* __restrict__ part_in,
-                                                             int nb, int last, float min_inliers,
-                                                             int may_stop) {
+                                                             int nb, int last, float min_inliers) {
   constexpr int ROWS = KAHAN ? 2 * KF_ICP_SUMS : KF_ICP_SUMS;
   __shared__ IcpShared<ROWS> sh;
-  const int stopped = may_stop ? st->icp_stop_at : 0;
-  if (stopped) {
+  const int stopped = STOP ? st->icp_stop_at : 0;
+  if (STOP && stopped) {
     // the ICP stopped early: the final pose is the one recorded for the last
     // iteration that ran
     if (threadIdx.x < 16) {
@@ -3217,8 +3218,12 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
   }
   if (total > 0) {
     const void* pin = pbase + (size_t)((total - 1) & 1) * half;
-    k_icp_finish<KAHAN, PPL><<<1, kIcpFinishThreads, 0, c->stream>>>(
-        c->state, pin, nb_prev, total - 1, min_inl, may_stop ? 1 : 0);
+    if (may_stop)
+      k_icp_finish<KAHAN, PPL, true><<<1, kIcpFinishThreads, 0, c->stream>>>(
+          c->state, pin, nb_prev, total - 1, min_inl);
+    else
+      k_icp_finish<KAHAN, PPL, false><<<1, kIcpFinishThreads, 0, c->stream>>>(
+          c->state, pin, nb_prev, total - 1, min_inl);
   }
   return hipGetLastError();
 }
