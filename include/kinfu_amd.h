@@ -144,6 +144,14 @@ typedef struct kf_params {
                                and whose translation is below
                                icp_stop_trans_mm (no reference counterpart) */
   float icp_stop_trans_mm;
+  int32_t raycast_skip;     /* 1: the raycast's second pass takes runs of
+                               steps through voxels that the volume's skip
+                               flags show hold no negative TSDF value in one
+                               round, without loading them (visits, hits and
+                               maps are the same as without it; it pays off
+                               on long walks: on in KF_PRESET_ROOM_HD_F16);
+                               0: every voxel is walked (no reference
+                               counterpart either way)                    */
 } kf_params;
 
 /* Raycast hit handling.  REFERENCE: the reference's per-axis interpolation
@@ -339,9 +347,10 @@ int kf_download_volume_box(kf_context* ctx, const int32_t origin[3], const int32
 int kf_download_volume_raw(kf_context* ctx, void* out, size_t bytes);
 int kf_upload_volume_raw(kf_context* ctx, const void* in, size_t bytes);
 /* Device address of the native volume (for zero-copy consumers).  The call
- * drops a trilinear raycast's saved crossings (see kf_raycast_adjoint):
- * after writing the volume through the address, call it again before the
- * next adjoint. */
+ * drops a trilinear raycast's saved crossings (see kf_raycast_adjoint) and
+ * has the next raycast rebuild its skip flags from the volume: after writing
+ * the volume through the address, call it again before the next raycast or
+ * adjoint. */
 int kf_volume_device_ptr(kf_context* ctx, void** ptr, size_t* bytes);
 
 /* ---- A9-A10: voxel-traversal raycast (voxel_traversal.cu:4-544) ---------
@@ -441,6 +450,9 @@ int kf_reset_stage_times(kf_context* ctx);
  * instantiations, so it is left off when timing. */
 int kf_count_raycast_work(kf_context* ctx, int32_t enable);
 int kf_raycast_work(kf_context* ctx, int64_t* voxels_visited, int64_t* hits);
+/* Of the counted visits, those taken by the second pass's skip rounds
+ * (kf_params raycast_skip; no reference counterpart, a measurement aid). */
+int kf_raycast_skipped(kf_context* ctx, int64_t* voxels_skipped);
 /* Work of the last integration (no reference counterpart; a measurement
  * aid): groups of 2x2x2 bricks and 8^3-voxel bricks that passed its culls
  * (projective_tsdf.cu:5-140 visits every voxel; the culled bricks hold no

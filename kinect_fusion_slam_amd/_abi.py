@@ -54,7 +54,8 @@ class Params(C.Structure):
                 ("refine_max_residual_mm", C.c_float), ("icp_robust", C.c_int32),
                 ("icp_min_inliers", C.c_int32), ("icp_levels", C.c_int32),
                 ("icp_coarse_iters", C.c_int32 * 2), ("pyramid_max_step_mm", C.c_float),
-                ("icp_stop_rot", C.c_float), ("icp_stop_trans_mm", C.c_float)]
+                ("icp_stop_rot", C.c_float), ("icp_stop_trans_mm", C.c_float),
+                ("raycast_skip", C.c_int32)]
 
 
 class SynthScene(C.Structure):
@@ -129,6 +130,7 @@ _SIGNATURES = {
     "kf_refine_history": (C.c_int, [_P, _I32, _D, _I32, _I32]),
     "kf_raycast_adjoint": (C.c_int, [_P, _F, _F, _F, _F, _F, _D]),
     "kf_raycast_work": (C.c_int, [_P, _I64, _I64]),
+    "kf_raycast_skipped": (C.c_int, [_P, _I64]),
     "kf_adjoint_used_saved": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "kf_gate_thresholds": (C.c_int, [C.c_float, _F, _F]),
     "kf_dist_gate_threshold": (C.c_int, [C.c_float, _F]),
@@ -680,3 +682,9 @@ class KinfuContext:
         v, h = C.c_int64(), C.c_int64()
         check(lib.kf_raycast_work(self.h, C.byref(v), C.byref(h)))
         return v.value, h.value
+
+    def raycast_skipped(self):
+        """Counted visits that the second pass's skip rounds took."""
+        v = C.c_int64()
+        check(lib.kf_raycast_skipped(self.h, C.byref(v)))
+        return v.value

@@ -174,6 +174,10 @@ bool params_valid(const kf_params* p, std::string* why) {
     *why = "icp_stop_rot and icp_stop_trans_mm must be finite and non-negative";
     return false;
   }
+  if (p->raycast_skip != 0 && p->raycast_skip != 1) {
+    *why = "raycast_skip must be 0 or 1";
+    return false;
+  }
   int total = p->icp_iterations;
   for (int l = 1; l < p->icp_levels; ++l) total += p->icp_coarse_iters[l - 1];
   if (total > KF_MAX_ICP_ITERS) {
@@ -377,6 +381,7 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
   p.pyramid_max_step_mm = 30.0f;
   p.icp_stop_rot = 0.0f;  // the reference's fixed iteration count
   p.icp_stop_trans_mm = 0.0f;
+  p.raycast_skip = 0;  // skipping pays off on the long walks of large volumes (HD preset)
   switch (which) {
     case KF_PRESET_REFERENCE: {
       // kinfu_pipeline.cpp:14-24, kinfu_pipeline.h:42-62, system_interface.cpp:6-11,
@@ -426,6 +431,7 @@ int kf_preset(int32_t which, kf_volume_desc* vol, kf_params* prm, float K[9], in
       p.sdf_min_mm = -20.0f;
       p.sdf_max_mm = 200.0f;
       p.icp_compensated = 1;
+      p.raycast_skip = 1;  // hole rays walk up to ~2500 voxels
       break;
     }
     default:
@@ -488,7 +494,8 @@ int kf_destroy(kf_context* c) {
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
                   c->adj_partials, c->gn_partials, c->adj_touched, c->adj_map_v,
                   c->adj_map_n, c->adj_map_hit, c->model_aux, c->adj_map_aux,
-                  c->color, c->jpeg_coef, c->jpeg_planes, c->jpeg_q, c->cloud};
+                  c->color, c->jpeg_coef, c->jpeg_planes, c->jpeg_q, c->cloud,
+                  c->skip_flags};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->jpeg_coef_host) (void)hipHostFree(c->jpeg_coef_host);
@@ -654,6 +661,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->state, sizeof(kf_dev_state)},
       {(void**)&c->icp_partials, (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 2 * 4},
       {(void**)&c->gn_partials, kf_refine_partials(c) * 4},
+      {(void**)&c->skip_flags, (size_t)c->n_bricks},
       {&c->volume, c->volume_bytes},
   };
   for (auto& a : allocs) {
@@ -724,6 +732,7 @@ int kf_reset(kf_context* c) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
   c->saved_ok = false;
   KF_LAUNCH(kf_launch_volume_fill(c));
+  KF_LAUNCH(kf_launch_skip_flags_clear(c));
   const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   return kf_set_pose(c, I);
 }
@@ -741,7 +750,7 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
              sizeof(kf_dev_state) +
              (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 8 +
              kf_ray_queue_entries(c->H, c->W) * kf_ray_state_bytes() +
-             kf_integ_group_entries(c->vg) * 4 +
+             kf_integ_group_entries(c->vg) * 4 + (size_t)c->n_bricks +
              (size_t)6 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4;
   if (c->icp_masks) b += (size_t)KF_MAX_ICP_ITERS * n;
   for (const kf_pyr_level& P : c->pyr)
@@ -1159,6 +1168,7 @@ int kf_download_volume_box(kf_context* c, const int32_t origin[3], const int32_t
 int kf_upload_volume(kf_context* c, const float* tsdf, const float* weight) {
   KF_REQUIRE(c && tsdf && weight, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
+  c->skip_flags_stale = true;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   const size_t ns = (size_t)c->n_store;
   if (c->vol.precision == KF_TSDF_F32) {
@@ -1192,6 +1202,7 @@ int kf_download_volume_raw(kf_context* c, void* out, size_t bytes) {
 int kf_upload_volume_raw(kf_context* c, const void* in, size_t bytes) {
   KF_REQUIRE(c && in && bytes == c->volume_bytes, KF_ERR_INVALID, "size mismatch");
   c->saved_ok = false;
+  c->skip_flags_stale = true;
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   KF_HIP_CHECK(hipMemcpy(c->volume, in, bytes, hipMemcpyHostToDevice));
   return KF_OK;
@@ -1200,6 +1211,7 @@ int kf_upload_volume_raw(kf_context* c, const void* in, size_t bytes) {
 int kf_volume_device_ptr(kf_context* c, void** ptr, size_t* bytes) {
   KF_REQUIRE(c && ptr, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
+  c->skip_flags_stale = true;  // the next raycast rebuilds the skip flags
   *ptr = c->volume;
   if (bytes) *bytes = c->volume_bytes;
   return KF_OK;
@@ -1769,6 +1781,7 @@ int kf_reset_stage_times(kf_context* c) {
 int kf_count_raycast_work(kf_context* c, int32_t enable) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
   KF_HIP_CHECK(hipMemsetAsync(&c->state->ray_steps, 0, 2 * sizeof(unsigned long long), c->stream));
+  KF_HIP_CHECK(hipMemsetAsync(&c->state->ray_skipped, 0, sizeof(unsigned long long), c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   c->count_rays = enable != 0;
   return KF_OK;
@@ -1803,6 +1816,16 @@ int kf_raycast_work(kf_context* c, int64_t* voxels_visited, int64_t* hits) {
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   *voxels_visited = (int64_t)v[0];
   *hits = (int64_t)v[1];
+  return KF_OK;
+}
+
+int kf_raycast_skipped(kf_context* c, int64_t* voxels_skipped) {
+  KF_REQUIRE(c && voxels_skipped, KF_ERR_INVALID, "null argument");
+  unsigned long long v = 0;
+  KF_HIP_CHECK(hipMemcpyAsync(&v, &c->state->ray_skipped, sizeof(v), hipMemcpyDeviceToHost,
+                              c->stream));
+  KF_HIP_CHECK(hipStreamSynchronize(c->stream));
+  *voxels_skipped = (int64_t)v;
   return KF_OK;
 }
 

@@ -33,6 +33,7 @@ struct kf_dev_state {
   int32_t icp_stop_from;  //   the first iteration whose kernel may stop
   float icp_stop_rot;     //   thresholds (kf_icp_converged)
   float icp_stop_trans;
+  unsigned long long ray_skipped;  // raycast work counter: steps taken by skip rounds
 };
 
 // A packed 3-vector of an image map (12 bytes, 4-byte aligned): a wave's 64
@@ -169,6 +170,12 @@ struct kf_context {
   int64_t n_voxels = 0;   // Nx*Ny*Nz
   int64_t n_store = 0;    // bricks * 512
   int n_bricks = 0;
+  // Second-pass skip flags, one byte per brick (set where the brick or a
+  // neighbour holds a negative TSDF value; a superset between resets).
+  // skip_flags_stale: the volume was handed out by address since the flags
+  // were last rebuilt, so the next raycast rebuilds them first.
+  uint8_t* skip_flags = nullptr;
+  bool skip_flags_stale = false;
 
   // ICP
   kf_dev_state* state = nullptr;
@@ -273,6 +280,9 @@ size_t kf_adjoint_partials(const kf_context* c);  // doubles
 size_t kf_refine_partials(const kf_context* c);   // floats
 hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps);
 hipError_t kf_launch_volume_fill(kf_context* c);
+// Skip flags: cleared (a reset volume), or rebuilt from the whole volume.
+hipError_t kf_launch_skip_flags_clear(kf_context* c);
+hipError_t kf_launch_skip_flags_rebuild(kf_context* c);
 // dequantisation + inverse DCT of the coefficient blocks in jpeg_coef, then
 // upsampling and colour conversion into color
 hipError_t kf_launch_jpeg_decode(kf_context* c, const kf_jpeg_dev& f);

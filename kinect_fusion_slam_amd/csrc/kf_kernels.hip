@@ -1312,8 +1312,9 @@ __global__ __launch_bounds__(256) void k_integ_table(const float* __restrict__ d
 // sdf_max], as in the reference.
 
 // Updates one brick with the calling wave: lane (lx, ly) walks its z column.
-template <int PREC, bool FIRST>
-__device__ __forceinline__ void integrate_brick(void* __restrict__ tsdf_, void* __restrict__ weight_,
+// Returns whether this lane stored a negative TSDF value (NEG; else false).
+template <int PREC, bool FIRST, bool NEG>
+__device__ __forceinline__ bool integrate_brick(void* __restrict__ tsdf_, void* __restrict__ weight_,
                                                 const float2* __restrict__ tab, const float* K,
                                                 const kf_vol& vg, int H, int W, float trunc,
                                                 float smin, float smax, const float* Ti,
@@ -1322,6 +1323,7 @@ __device__ __forceinline__ void integrate_brick(void* __restrict__ tsdf_, void* 
   const float d = vg.d;
   const float hd = d / 2.f;
   const float Wp1 = (float)(W + 1), Hp1 = (float)(H + 1);
+  bool wneg = false;
   do {
     const int bz = brick % vg.bz;
     const int q = brick / vg.bz;
@@ -1411,31 +1413,90 @@ __device__ __forceinline__ void integrate_brick(void* __restrict__ tsdf_, void* 
         if constexpr (FIRST) {
           T[v] = f;
           Wt[v] = 1.0f;
+          if constexpr (NEG) wneg = wneg || f < 0.f;
         } else {
           float Fo = F0[lz], Wo = W0[lz];
           if (Fo != Fo) Fo = 0.0f;
           if (Wo != Wo) Wo = 0.0f;
-          T[v] = (Wo * Fo + 1.0f * f) / (Wo + 1.0f);
+          const float Fn = (Wo * Fo + 1.0f * f) / (Wo + 1.0f);
+          T[v] = Fn;
           Wt[v] = Wo + 1.0f;
+          if constexpr (NEG) wneg = wneg || Fn < 0.f;
         }
       } else {
         uint16_t* T = reinterpret_cast<uint16_t*>(tsdf_);
         uint16_t* Wt = reinterpret_cast<uint16_t*>(weight_);
         if constexpr (FIRST) {
-          T[v] = __half_as_ushort(__float2half_rn(f));
+          const __half h = __float2half_rn(f);
+          T[v] = __half_as_ushort(h);
           Wt[v] = 1;
+          if constexpr (NEG) wneg = wneg || __half2float(h) < 0.f;
         } else {
           float Fo = F0[lz];
           if (Fo != Fo) Fo = 0.0f;
           const uint32_t w = Wc[lz];
           const float Wo = (float)w;
           const float Fn = (Wo * Fo + 1.0f * f) / (Wo + 1.0f);
-          T[v] = __half_as_ushort(__float2half_rn(Fn));
+          const __half h = __float2half_rn(Fn);
+          T[v] = __half_as_ushort(h);
           Wt[v] = (uint16_t)(w == 65535u ? 65535u : w + 1u);
+          if constexpr (NEG) wneg = wneg || __half2float(h) < 0.f;
         }
       }
     }
   } while (0);
+  return wneg;
+}
+
+// Skip flags of the second raycast pass, one byte per brick: a brick's flag
+// is set when it or one of its 26 neighbours holds (or once held) a negative
+// TSDF value.  Flags are only ever set between volume resets, so they stay a
+// superset of that property as integration changes the volume.  Lane q < 27
+// of the calling wave sets the flag of neighbour (q / 9 - 1, q / 3 % 3 - 1,
+// q % 3 - 1) of brick (bx, by, bz).
+__device__ __forceinline__ void set_skip_flags(uint8_t* __restrict__ flags, const kf_vol& vg,
+                                               int bx, int by, int bz) {
+  const int q = threadIdx.x & 63;
+  const int x = bx + q / 9 - 1, y = by + (q / 3) % 3 - 1, z = bz + q % 3 - 1;
+  if (q < 27 && x >= 0 && x < vg.bx && y >= 0 && y < vg.by && z >= 0 && z < vg.bz)
+    flags[(x * vg.by + y) * vg.bz + z] = 1;
+}
+
+// Rebuilds the skip flags from the whole volume (after a volume is uploaded
+// or written through its device address): one brick per wave, lane l reads
+// voxels 8l .. 8l + 7 of the brick.  The flags must be all zero before.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_skip_flags(const void* __restrict__ tsdf,
+                                                    uint8_t* __restrict__ flags, kf_vol vg,
+                                                    int n_bricks) {
+  const int brick = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (brick >= n_bricks) return;
+  const int lane = threadIdx.x & 63;
+  const int base = (brick << 9) | (lane << 3);
+  bool neg = false;
+  if constexpr (PREC == KF_TSDF_F32) {
+    const float4* T4 = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(tsdf) + base);
+    const float4 a = T4[0], b = T4[1];
+    neg = a.x < 0.f || a.y < 0.f || a.z < 0.f || a.w < 0.f || b.x < 0.f || b.y < 0.f ||
+          b.z < 0.f || b.w < 0.f;
+  } else {
+    const uint4 a = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(tsdf) + base);
+    const uint32_t ta[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t h = (ta[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+      neg = neg || __half2float(__ushort_as_half((uint16_t)h)) < 0.f;
+    }
+  }
+  if (__ballot(neg)) {
+    const int bz = brick % vg.bz, q = brick / vg.bz;
+    set_skip_flags(flags, vg, q / vg.by, q % vg.by, bz);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t* __restrict__ p, int n, uint8_t v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // Brick cull: one listed group per wave, lane = (brick of the group,
@@ -1547,12 +1608,13 @@ __global__ __launch_bounds__(256) void k_integ_cull(kf_cam cam, kf_vol vg, int H
   }
 }
 
-// Brick update: one listed brick per wave.
-template <int PREC, bool FIRST>
+// Brick update: one listed brick per wave; with FLAGS, a brick that received
+// a negative value sets the skip flags around it.
+template <int PREC, bool FIRST, bool FLAGS>
 __global__ __launch_bounds__(256) void k_integrate(
     void* __restrict__ tsdf_, void* __restrict__ weight_, const float2* __restrict__ tab,
     kf_cam cam, kf_vol vg, int H, int W, float trunc, float smin, float smax,
-    const kf_dev_state* __restrict__ st, GroupList gl) {
+    const kf_dev_state* __restrict__ st, GroupList gl, uint8_t* __restrict__ skip_flags) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float Ti[12], t[3], cd[3];
   load_uniform<12>(st->Tinv, Ti);
@@ -1567,8 +1629,12 @@ __global__ __launch_bounds__(256) void k_integrate(
   const uint32_t stride = gridDim.x * 4 / KF_RAY_PARTS;
   for (uint32_t e = (uint32_t)wid / KF_RAY_PARTS; e < n_bricks; e += stride) {
     const int brick = __builtin_amdgcn_readfirstlane(gl.blist[part * gl.bcap + e]);
-    integrate_brick<PREC, FIRST>(tsdf_, weight_, tab, cam.K, vg, H, W, trunc, smin, smax, Ti, t,
-                                 cd, brick);
+    const bool neg = integrate_brick<PREC, FIRST, FLAGS>(tsdf_, weight_, tab, cam.K, vg, H, W,
+                                                         trunc, smin, smax, Ti, t, cd, brick);
+    if (FLAGS && __ballot(neg)) {
+      const int bz = brick % vg.bz, q = brick / vg.bz;
+      set_skip_flags(skip_flags, vg, q / vg.by, q % vg.by, bz);
+    }
   }
 }
 
@@ -1752,6 +1818,7 @@ struct RayIO {
   uint32_t* qcount;    // this raycast's partition append counters
   uint32_t* qnext;     // the next raycast's counters (cleared by the second pass)
   int qcap;            // rays per queue partition
+  const uint8_t* skip_flags;  // per-brick skip flags of the second pass (null: no skipping)
 };
 
 // Raycast modes: 0 = the reference's hit handling (voxel_traversal.cu:222-
@@ -2382,14 +2449,48 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Empty-space skipping of the second pass.  No step can stop the walk
+// before it reaches a voxel with a negative TSDF value: a break or a
+// crossing needs f < 0 or prev < 0, and while neither holds a visit only
+// sets prev = f.  A query reads the skip flags at kSkipWindows points
+// kSkipSpacing voxels apart along the ray: sample k is the continuous voxel
+// coordinate (the walk's voxel at the query) + 0.5 + 8 k (dx, dy, dz), the
+// ray's position at parameter tq + 8 k d (tq: the query's tent).  Window k
+// holds the steps whose boundary distance lies within 4 d of tq + 8 k d.
+// The voxel such a step enters lies within 4 voxels per axis of the sample,
+// plus one for the number of boundaries the walk crosses on that axis in
+// between (they are td apart), so its index is within 5 of the sample's
+// rounded-down coordinate, inside the 8 voxels a set flag covers around its
+// brick.  A window whose sample reads a clear flag therefore holds no
+// negative voxel, and a skip round takes every step whose boundary distance
+// lies below the first flagged window ahead: with the sequences of one
+// round, it sets the state the merge would reach after those steps, and
+// loads no TSDF value (the next full round loads the last one as its prev).
+constexpr int kSkipSpacing = 8;
+constexpr int kSkipWindows = 64;
+
+// This lane's query sample: the skip flag of the brick that holds it
+// (clamped into the volume, which moves it towards every voxel inside).
+__device__ __forceinline__ uint8_t skip_flag_at(const uint8_t* __restrict__ flags, const kf_vol& vg,
+                                                const DdaState& c, float dx, float dy, float dz) {
+  const float o = (float)(kSkipSpacing * lane_id());
+  const float fx = floorf(((float)(c.ix - 1) + 0.5f) + o * dx);
+  const float fy = floorf(((float)(c.iy - 1) + 0.5f) + o * dy);
+  const float fz = floorf(((float)(c.iz - 1) + 0.5f) + o * dz);
+  const int x = (int)fminf(fmaxf(fx, 0.0f), (float)(vg.nx - 1));
+  const int y = (int)fminf(fmaxf(fy, 0.0f), (float)(vg.ny - 1));
+  const int z = (int)fminf(fmaxf(fz, 0.0f), (float)(vg.nz - 1));
+  return flags[((x >> 3) * vg.by + (y >> 3)) * vg.bz + (z >> 3)];
+}
+
 // Walks ray r (the same state in every lane) to its end.  Returns the lane
 // that holds the result (ov, on, hit and, for the adjoint and refinement
 // modes, the gradient terms); the other lanes keep ov = on = 0, hit = -1.
-template <int PREC, bool COUNT, int MODE>
+template <int PREC, bool COUNT, int MODE, bool SKIP>
 __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, const RayIO& io,
                                          const float* T, WaveWalkLds& L, float4& ov,
-                                         float4& on, int& hit, int& visited, double* xi,
-                                         const Sink& sink, float* gn) {
+                                         float4& on, int& hit, int& visited, int& skipped,
+                                         double* xi, const Sink& sink, float* gn) {
   const int lane = lane_id();
   const kf_vol& vg = io.vg;
   const int Nx = vg.nx, Ny = vg.ny, Nz = vg.nz;
@@ -2415,11 +2516,24 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     return 0;
   }
   const float itx = 1.0f / p.tdx, ity = 1.0f / p.tdy, itz = 1.0f / p.tdz;
-  // Each round either ends the walk or advances it by kWaveSteps steps along
-  // the fixed step directions, so the walk leaves the volume after at most
-  // (Nx + Ny + Nz) / kWaveSteps + 1 rounds.
+  // Skip rounds (SKIP, with io.skip_flags set): the windows of the last flag
+  // query (bit k: window k flagged), its ray parameter tq, the parameter from
+  // which the walk queries again, and the store index of the voxel whose TSDF
+  // is the next visit's prev after a skip round (-1: r.prev holds it).
+  const bool skip_on = SKIP && io.skip_flags != nullptr;
+  const float s_mm = (float)kSkipSpacing * vg.d;
+  uint64_t qmask = ~0ull;
+  float tq = 0.0f, qagain = -INFINITY;
+  int pidx = -1;
+  // Each round either ends the walk or advances it by kWaveSteps steps (a
+  // skip round: at least one step) along the fixed step directions, so the
+  // walk leaves the volume after at most Nx + Ny + Nz rounds.
   for (;;) {
     const DdaState c0 = r.cur;
+    // the query's flag loads go out before the sequences and are used after
+    const bool query = skip_on && !(r.tent < qagain);
+    uint8_t qflag = 1;
+    if (query) qflag = skip_flag_at(io.skip_flags, vg, c0, r.dx, r.dy, r.dz);
     // 1. per-axis boundary distances (lanes 0-2; the others write row 3, so
     //    that no lane-dependent branch precedes the cross-lane steps below)
     {
@@ -2442,11 +2556,95 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
       L.seq[min(lane, 3)][kWaveSteps] = t;
     }
     wave_lds_sync();
+    const float ex = L.seq[0][lane], ey = L.seq[1][lane], ez = L.seq[2][lane];
+    if (query) {
+      qmask = __ballot(qflag != 0);
+      tq = r.tent;
+      qagain = uni(tq + (float)(kSkipWindows - 16) * s_mm);
+    }
+    // skip round: every step whose boundary distance lies below tr, where tr
+    // is the first flagged window ahead, the round's sequence ends (t_a(64))
+    // and the distance of each axis's first step out of the traversal.  It
+    // visits the current voxel and those its steps enter but the last (the
+    // next current voxel), all in clear windows; the current voxel must lie
+    // in the traversal (a full round ends a walk whose current voxel does
+    // not), and prev, the last visited voxel's TSDF, must not be negative
+    // (after a skip round it is not)
+    if (skip_on && (c0.ix < Nx - 1 && c0.ix > 0 && c0.iy < Ny - 1 && c0.iy > 0 &&
+                    c0.iz < Nz - 1 && c0.iz > 0) &&
+        (pidx >= 0 || !(r.prev < 0.f))) {
+      // windows that end before tent are ignored (rounded towards keeping
+      // them); k0 <= kSkipWindows - 16 since the walk queries again there
+      const int k0 = max(0, (int)floorf((r.tent - tq) / s_mm) - 1);
+      const uint64_t ahead = qmask >> k0;
+      const float tneg = ahead ? tq + ((float)(k0 + __ffsll((unsigned long long)ahead) - 1) - 0.5f) * s_mm
+                               : tq + ((float)kSkipWindows - 0.5f) * s_mm;
+      if (fminf(c0.tmx, fminf(c0.tmy, c0.tmz)) < tneg) {
+        float tr = fminf(tneg, fminf(L.seq[0][kWaveSteps],
+                                     fminf(L.seq[1][kWaveSteps], L.seq[2][kWaveSteps])));
+        // steps along an axis that keep the walk inside (0, N - 1) (>= 0: the
+        // current voxel is inside)
+        const int jx = p.sx > 0 ? Nx - 2 - c0.ix : c0.ix - 1;
+        const int jy = p.sy > 0 ? Ny - 2 - c0.iy : c0.iy - 1;
+        const int jz = p.sz > 0 ? Nz - 2 - c0.iz : c0.iz - 1;
+        if (jx <= kWaveSteps) tr = fminf(tr, L.seq[0][jx]);
+        if (jy <= kWaveSteps) tr = fminf(tr, L.seq[1][jy]);
+        if (jz <= kWaveSteps) tr = fminf(tr, L.seq[2][jz]);
+        const int nx = __popcll(__ballot(ex < tr));
+        const int ny = __popcll(__ballot(ey < tr));
+        const int nz = __popcll(__ballot(ez < tr));
+        if (nx + ny + nz > 0) {
+          // the state after those steps: the other indices at each axis's
+          // last step (ranks with the merge's ties), the next boundary
+          // distances and the last step's distance
+          // distances of each axis's last step (-inf: none); the walk's last
+          // step is the largest of them, ties going to x, then y
+          const float vx = nx > 0 ? uni(L.seq[0][nx - 1]) : -INFINITY;
+          const float vy = ny > 0 ? uni(L.seq[1][ny - 1]) : -INFINITY;
+          const float vz = nz > 0 ? uni(L.seq[2][nz - 1]) : -INFINITY;
+          DdaState& c = r.cur;
+          if (nx > 0) {
+            c.XY = c0.iy + p.sy * __popcll(__ballot(ey <= vx));
+            c.XZ = c0.iz + p.sz * __popcll(__ballot(ez <= vx));
+          }
+          if (ny > 0) {
+            c.YX = c0.ix + p.sx * __popcll(__ballot(ex < vy));
+            c.YZ = c0.iz + p.sz * __popcll(__ballot(ez <= vy));
+          }
+          if (nz > 0) {
+            c.ZX = c0.ix + p.sx * __popcll(__ballot(ex < vz));
+            c.ZY = c0.iy + p.sy * __popcll(__ballot(ey < vz));
+          }
+          const bool lx = vx >= vy && vx >= vz;
+          const bool ly = !lx && vy >= vz;
+          const bool lz = !lx && !ly;
+          c.ix = c0.ix + p.sx * nx;
+          c.iy = c0.iy + p.sy * ny;
+          c.iz = c0.iz + p.sz * nz;
+          c.tmx = uni(L.seq[0][nx]);
+          c.tmy = uni(L.seq[1][ny]);
+          c.tmz = uni(L.seq[2][nz]);
+          r.tent = fmaxf(vx, fmaxf(vy, vz));
+          // the last voxel the skipped steps visited: one step back along the
+          // axis of the last step (its TSDF is the next visit's prev)
+          pidx = kf_store_index(vg, c.ix - (lx ? p.sx : 0) - 1, c.iy - (ly ? p.sy : 0) - 1,
+                                c.iz - (lz ? p.sz : 0) - 1);
+          if constexpr (COUNT) {
+            visited += lane == 0 ? nx + ny + nz : 0;
+            skipped += lane == 0 ? nx + ny + nz : 0;
+          }
+          wave_lds_sync();  // every lane has read this round's sequences
+          if constexpr (MODE == kModeRefine) {
+            if (r.tent > io.walk_tmax) return 0;  // no crossing ahead can be used
+          }
+          continue;
+        }
+      }
+    }
     // 2. ranks in the merge; ties: y and z before x, z before y.  Each rank
     //    is #{j : s[j] <= v} (or < v) over one sequence: a guess from s0 + j
     //    td, checked against the two stored values around it (all 12 loaded
     //    together), and corrected by a walk where the check fails.
-    const float ex = L.seq[0][lane], ey = L.seq[1][lane], ez = L.seq[2][lane];
     int y_x = seq_guess<true>(c0.tmy, ity, ex);
     int z_x = seq_guess<true>(c0.tmz, itz, ex);
     int x_y = seq_guess<false>(c0.tmx, itx, ey);
@@ -2489,14 +2687,16 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
       L.key[rz] = ez;
     }
     wave_lds_sync();
-    // 3. lane k visits V_k
+    // 3. lane k visits V_k (and after a skip round, the TSDF of V_0's
+    //    predecessor is loaded with the visits)
+    const float p0 = SKIP && pidx >= 0 ? F(pidx) : r.prev;
     const uint32_t ck = lane == 0 ? 0u : L.cnt[lane - 1];
     const int cx = ck & 0xff, cy = (ck >> 8) & 0xff, cz = ck >> 16;
     const int vx = c0.ix + p.sx * cx, vy = c0.iy + p.sy * cy, vz = c0.iz + p.sz * cz;
     const bool alive = vx < Nx - 1 && vx > 0 && vy < Ny - 1 && vy > 0 && vz < Nz - 1 && vz > 0;
     const float f = alive ? F(kf_store_index(vg, vx - 1, vy - 1, vz - 1)) : 0.0f;
     float prev = __shfl_up(f, 1);
-    if (lane == 0) prev = r.prev;
+    if (lane == 0) prev = p0;
     const bool brk =
         (f != f && prev < 0.f) || (f < 0.f && prev != prev) || (f > 0.f && prev < 0.f);
     const bool cross = !brk && f < 0.f && prev > 0.f;
@@ -2564,6 +2764,7 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     c.tmz = uni(L.seq[2][nz]);
     r.tent = uni(L.key[kWaveSteps - 1]);
     r.prev = uni(__shfl(f, kWaveSteps - 1));
+    pidx = -1;
     wave_lds_sync();  // every lane has read this round's records
     if constexpr (MODE == kModeRefine) {
       if (r.tent > io.walk_tmax) return 0;  // no crossing ahead can be used
@@ -2573,13 +2774,15 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
 
 // Optional work counters (COUNT): voxels visited by the walks and rays that
 // hit, summed into the device state for the algorithmic-traffic figure.
-__device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool hit) {
-  uint64_t v = (uint64_t)visited;
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+__device__ __forceinline__ void add_counts(kf_dev_state* st, int visited, bool hit,
+                                           int skipped = 0) {
+  uint64_t v = (uint64_t)visited, sk = (uint64_t)skipped;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o), sk += __shfl_xor(sk, o);
   const uint64_t h = __popcll(__ballot(hit));
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&st->ray_steps, (unsigned long long)v);
     atomicAdd(&st->ray_hits, (unsigned long long)h);
+    if (sk) atomicAdd(&st->ray_skipped, (unsigned long long)sk);
   }
 }
 
@@ -2797,7 +3000,7 @@ __device__ __forceinline__ void uniform_ray(RayState& r) {
 // which cluster in some partitions, spread over all waves.  Lane l holds
 // partition l's count and its inclusive prefix sum; ray q lies in the
 // partition p = #{l : incl_l <= q}, at offset q - (incl_p - count_p).
-template <int PREC, bool COUNT, int MODE>
+template <int PREC, bool COUNT, int MODE, bool SKIP>
 __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
     kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n, float2* __restrict__ out_aux,
@@ -2840,16 +3043,16 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
     RayState r = queue[(size_t)part * io.qcap + off];
     uniform_ray(r);
     float4 ov = make_float4(0.f, 0.f, 0.f, 0.f), on = make_float4(0.f, 0.f, 0.f, 0.f);
-    int hit = -1, visited = 0;
+    int hit = -1, visited = 0, skipped = 0;
     const Sink sink{io.gF, false, nullptr, nullptr, io.touched};
     float px[MODE == kModeRefine ? kGnPix : 1] = {};  // refinement: the ray's pixel record
-    const int owner =
-        wave_walk<PREC, COUNT, MODE>(F, r, io, T, lds[wave], ov, on, hit, visited, xi, sink, px);
+    const int owner = wave_walk<PREC, COUNT, MODE, SKIP>(F, r, io, T, lds[wave], ov, on, hit,
+                                                         visited, skipped, xi, sink, px);
     if constexpr (MODE == kModeRefine) gn_add_pixel(px, gn);
     if constexpr (MODE != kModeRefine) {
       if (lane == owner) store_maps<MODE>(out_v, out_n, out_aux, out_hit, r.pix, ov, on, hit);
     }
-    if constexpr (COUNT) add_counts(st, visited, hit >= 0);
+    if constexpr (COUNT) add_counts(st, visited, hit >= 0, skipped);
   }
   if constexpr (MODE == kModeRefine)
     block_gn(gn, io.gn_part + kGnSums * (io.part_base + blockIdx.x));
@@ -3396,25 +3599,33 @@ hipError_t kf_launch_integ_table(kf_context* c, const kf_cam& cam) {
 hipError_t kf_launch_integrate(kf_context* c, const kf_cam& cam, int first) {
   const kf_params& p = c->prm;
   const GroupList gl = make_group_list(c, cam);
+  // the skip flags are kept while raycast_skip is on; otherwise the first
+  // raycast that skips rebuilds them
+  uint8_t* flags = p.raycast_skip ? c->skip_flags : nullptr;
+  if (!flags) c->skip_flags_stale = true;
   c->integ_slot ^= 1;
   k_integ_cull<<<kIntegCullBlocks, 256, 0, c->stream>>>(cam, c->vg, c->H, c->W, c->state, gl,
                                                          c->integ_tiles, (c->W + 15) / 16);
-#define KF_INTEG(P, FI)                                                                   \
-  k_integrate<P, FI><<<kIntegBlocks, 256, 0, c->stream>>>(c->tsdf, c->weight, c->integ_tab, \
-                                                          cam, c->vg, c->H, c->W,           \
-                                                          p.truncation_mm, p.sdf_min_mm,    \
-                                                          p.sdf_max_mm, c->state, gl)
-  if (c->vol.precision == KF_TSDF_F32) {
-    if (first)
-      KF_INTEG(KF_TSDF_F32, true);
-    else
-      KF_INTEG(KF_TSDF_F32, false);
-  } else {
-    if (first)
-      KF_INTEG(KF_TSDF_F16, true);
-    else
-      KF_INTEG(KF_TSDF_F16, false);
-  }
+#define KF_INTEG(P, FI, FL)                                                               \
+  k_integrate<P, FI, FL><<<kIntegBlocks, 256, 0, c->stream>>>(                              \
+      c->tsdf, c->weight, c->integ_tab, cam, c->vg, c->H, c->W, p.truncation_mm,            \
+      p.sdf_min_mm, p.sdf_max_mm, c->state, gl, flags)
+#define KF_INTEG_P(P)              \
+  do {                             \
+    if (first && flags)            \
+      KF_INTEG(P, true, true);     \
+    else if (first)                \
+      KF_INTEG(P, true, false);    \
+    else if (flags)                \
+      KF_INTEG(P, false, true);    \
+    else                           \
+      KF_INTEG(P, false, false);   \
+  } while (0)
+  if (c->vol.precision == KF_TSDF_F32)
+    KF_INTEG_P(KF_TSDF_F32);
+  else
+    KF_INTEG_P(KF_TSDF_F16);
+#undef KF_INTEG_P
 #undef KF_INTEG
   return hipGetLastError();
 }
@@ -3452,6 +3663,7 @@ static RayIO make_ray_io(const kf_context* c, const kf_cam& cam) {
   io.walk_tmax = INFINITY;
   io.refine_batches = 1;
   io.pass1_batches = pass1_batches(c);
+  io.skip_flags = nullptr;
   return io;
 }
 
@@ -3513,8 +3725,12 @@ static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb
         c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, q, tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
-  k_raycast_rest<PREC, COUNT, MODE><<<kRestBlocks, 256, 0, c->stream>>>(
-      c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, q);
+  if (io.skip_flags)
+    k_raycast_rest<PREC, COUNT, MODE, true><<<kRestBlocks, 256, 0, c->stream>>>(
+        c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, q);
+  else
+    k_raycast_rest<PREC, COUNT, MODE, false><<<kRestBlocks, 256, 0, c->stream>>>(
+        c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, q);
 }
 
 template <int PREC>
@@ -3547,6 +3763,13 @@ static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode
   io.qnext = c->ray_counts + (size_t)(c->ray_slot ^ 1) * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE;
   io.qcap = kf_ray_part_capacity(nb);
   c->ray_slot ^= 1;
+  if (c->prm.raycast_skip && c->skip_flags) {
+    if (c->skip_flags_stale) {
+      const hipError_t e = kf_launch_skip_flags_rebuild(c);
+      if (e != hipSuccess) return e;
+    }
+    io.skip_flags = c->skip_flags;
+  }
   if (c->vol.precision == KF_TSDF_F32)
     launch_raycast_p<KF_TSDF_F32>(c, io, tiles_x, nb, count && c->count_rays, mode, m);
   else
@@ -3617,5 +3840,28 @@ hipError_t kf_launch_volume_fill(kf_context* c) {
   const int64_t n = c->n_store;
   k_volume_fill<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->tsdf, c->weight,
                                                                      c->vol.precision, n);
+  return hipGetLastError();
+}
+
+hipError_t kf_launch_skip_flags_clear(kf_context* c) {
+  if (c->skip_flags) {
+    k_fill_u8<<<(unsigned)((c->n_bricks + 255) / 256), 256, 0, c->stream>>>(c->skip_flags,
+                                                                             c->n_bricks, 0);
+    c->skip_flags_stale = false;
+  }
+  return hipGetLastError();
+}
+
+hipError_t kf_launch_skip_flags_rebuild(kf_context* c) {
+  if (!c->skip_flags) return hipSuccess;
+  const hipError_t e = kf_launch_skip_flags_clear(c);
+  if (e != hipSuccess) return e;
+  const unsigned blocks = (unsigned)((c->n_bricks + 3) / 4);
+  if (c->vol.precision == KF_TSDF_F32)
+    k_skip_flags<KF_TSDF_F32><<<blocks, 256, 0, c->stream>>>(c->tsdf, c->skip_flags, c->vg,
+                                                             c->n_bricks);
+  else
+    k_skip_flags<KF_TSDF_F16><<<blocks, 256, 0, c->stream>>>(c->tsdf, c->skip_flags, c->vg,
+                                                             c->n_bricks);
   return hipGetLastError();
 }
