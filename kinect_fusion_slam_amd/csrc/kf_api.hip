@@ -488,7 +488,7 @@ int kf_destroy(kf_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   timing_drop(c);
   void* bufs[] = {c->ray_queue, c->ray_counts, c->integ_groups, c->integ_counts, c->depth,   c->depth_tmp, c->depth_u16, c->vmap,         c->nmap,
-                  c->mask,    c->model_v,   c->model_n,   c->hit,          c->integ_tab,
+                  c->mask,    c->model_v,   c->model_n,   c->model_nt, c->hit, c->integ_tab,
                   c->integ_tiles,
                   c->spatial, c->range_lut, c->volume,    c->state,     c->icp_partials, c->icp_masks,
                   c->frames,  c->poses,     c->grad_tsdf, c->adj_depth,    c->adj_normal,
@@ -502,7 +502,7 @@ int kf_destroy(kf_context* c) {
   if (c->jpeg_q_host) (void)hipHostFree(c->jpeg_q_host);
   if (c->jpeg_copied) (void)hipEventDestroy(c->jpeg_copied);
   for (kf_pyr_level& P : c->pyr) {
-    void* pb[] = {P.depth, P.vmap, P.nmap, P.mask, P.mv, P.mn, P.ok};
+    void* pb[] = {P.depth, P.vmap, P.nmap, P.mask, P.mv, P.mn, P.mt, P.ok};
     for (void* b : pb)
       if (b) (void)hipFree(b);
   }
@@ -529,7 +529,7 @@ static int alloc_pyramid(kf_context* c, int levels) {
       size_t bytes;
     } a[] = {{(void**)&P.depth, n * 4}, {(void**)&P.vmap, n * 12}, {(void**)&P.nmap, n * 12},
              {(void**)&P.mask, n},      {(void**)&P.mv, n * 12},   {(void**)&P.mn, n * 12},
-             {(void**)&P.ok, n}};
+             {(void**)&P.mt, n * 4},    {(void**)&P.ok, n}};
     for (auto& b : a)
       if (hipMalloc(b.p, b.bytes) != hipSuccess) {
         *b.p = nullptr;
@@ -648,6 +648,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       {(void**)&c->depth_u16, n * 2},   {(void**)&c->vmap, n * 12},
       {(void**)&c->nmap, n * 12},       {(void**)&c->mask, n},
       {(void**)&c->model_v, n * 12},    {(void**)&c->model_n, n * 12},
+      {(void**)&c->model_nt, n * 4},
       {(void**)&c->model_aux, n * 8},
       {(void**)&c->hit, n * 4},         {(void**)&c->integ_tab, n * 8},
       {(void**)&c->color, n * 3},
@@ -690,7 +691,8 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
       hipMemsetAsync(c->integ_counts, 0, (size_t)4 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4,
                      c->stream) != hipSuccess)
     return fail(KF_ERR_HIP, "hipMemsetAsync failed");
-  if (kf_launch_set_icp_stop(c->state, c->prm, c->stream) != hipSuccess)
+  if (kf_launch_set_icp_stop(c->state, c->prm, c->stream) != hipSuccess ||
+      kf_launch_fill_f32(c->model_nt, (int64_t)n, INFINITY, c->stream) != hipSuccess)
     return fail(KF_ERR_HIP, "kernel launch failed");
   *out = c;
   rc = kf_reset(c);
@@ -1425,13 +1427,16 @@ int kf_upload_model_maps(kf_context* c, const float* x, const float* y, const fl
   c->saved_ok = false;
   const size_t n = (size_t)c->H * c->W;
   std::vector<kf_p3> V(n), N(n);
+  std::vector<float> Nt(n);
   for (size_t i = 0; i < n; ++i) {
     V[i] = kf_p3{x[i], y[i], z[i]};
     N[i] = kf_p3{nx[i], ny[i], nz[i]};
+    Nt[i] = kf_inv_norm3(nx[i], ny[i], nz[i]);
   }
   c->maps_trilinear = false;
   KF_HIP_CHECK(hipMemcpyAsync(c->model_v, V.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipMemcpyAsync(c->model_n, N.data(), n * 12, hipMemcpyHostToDevice, c->stream));
+  KF_HIP_CHECK(hipMemcpyAsync(c->model_nt, Nt.data(), n * 4, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   return KF_OK;
 }

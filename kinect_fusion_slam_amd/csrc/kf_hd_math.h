@@ -88,6 +88,11 @@ KF_HD float kf_expf(float x) {
 // whose last-bit behaviour is not pinned by anything in the reference; these
 // FP64 versions are the convention on both sides of the comparison.
 
+// The factor the reference's normalize (cuda_functions.cuh:2-10) scales a
+// 3-vector by: 1 / |v| with the float sum x^2 + y^2 + z^2 left to right and
+// a correctly rounded square root and division (+inf for the zero vector).
+KF_HD float kf_inv_norm3(float x, float y, float z) { return 1 / sqrtf(x * x + y * y + z * z); }
+
 // Element (i, j) of A * B for 4x4 row-major float matrices, FP64 products
 // summed left to right and rounded once.
 KF_HD float kf_mat4_mul_elem(const float* A, const float* B, int i, int j) {

@@ -111,6 +111,7 @@ struct kf_pyr_level {
   uint8_t* mask = nullptr;
   kf_p3* mv = nullptr;
   kf_p3* mn = nullptr;
+  float* mt = nullptr;  // kf_inv_norm3 of mn
   uint8_t* ok = nullptr;
 };
 
@@ -147,6 +148,8 @@ struct kf_context {
   uint8_t* mask = nullptr;      // vertex mask (reference layout)
   kf_p3* model_v = nullptr;     // raycast vertices {x, y, z}
   kf_p3* model_n = nullptr;     // raycast normals {nx, ny, nz}
+  float* model_nt = nullptr;    // kf_inv_norm3 of each model normal (the factor the
+                                // ICP's normal gate normalises it by; +inf for 0)
   float2* model_aux = nullptr;  // trilinear raycasts only: {camera depth, crossing te}
   bool maps_trilinear = false;  // model_aux belongs to the maps (last raycast trilinear)
   int32_t* hit = nullptr;       // raycast hit voxel index
@@ -280,6 +283,8 @@ size_t kf_adjoint_partials(const kf_context* c);  // doubles
 size_t kf_refine_partials(const kf_context* c);   // floats
 hipError_t kf_launch_refine(kf_context* c, const kf_cam& cam, int steps);
 hipError_t kf_launch_volume_fill(kf_context* c);
+// n floats at p set to v.
+hipError_t kf_launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 // Skip flags: cleared (a reset volume), or rebuilt from the whole volume.
 hipError_t kf_launch_skip_flags_clear(kf_context* c);
 hipError_t kf_launch_skip_flags_rebuild(kf_context* c);

@@ -45,7 +45,7 @@ __device__ __forceinline__ void load_uniform(const float* __restrict__ src, floa
 
 // cuda_functions.cuh:2-10
 __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
-  const float t = 1 / sqrtf(x * x + y * y + z * z);
+  const float t = kf_inv_norm3(x, y, z);
   x = x * t;
   y = y * t;
   z = z * t;
@@ -529,12 +529,13 @@ __global__ __launch_bounds__(256) void k_depth_down(const float* __restrict__ d_
 // Model maps: where all four pixels hold a surface (hit index >= 0 at level
 // 0, ok set at coarser levels), the mean vertex ((v00 + v01) + (v10 + v11)) *
 // 0.25 and the normalised sum of the normals (normalize3); elsewhere zeros
-// and ok = 0, as a raycast pixel without a hit.
+// and ok = 0, as a raycast pixel without a hit.  t_out: kf_inv_norm3 of the
+// normal written.
 __global__ __launch_bounds__(256) void k_model_down(
     const kf_p3* __restrict__ v_in, const kf_p3* __restrict__ n_in,
     const int32_t* __restrict__ hit_in, const uint8_t* __restrict__ ok_in, int Wi,
-    kf_p3* __restrict__ v_out, kf_p3* __restrict__ n_out, uint8_t* __restrict__ ok_out, int Ho,
-    int Wo) {
+    kf_p3* __restrict__ v_out, kf_p3* __restrict__ n_out, float* __restrict__ t_out,
+    uint8_t* __restrict__ ok_out, int Ho, int Wo) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= Ho * Wo) return;
   const int y = i / Wo, x = i - y * Wo;
@@ -556,6 +557,7 @@ __global__ __launch_bounds__(256) void k_model_down(
   }
   v_out[i] = V;
   n_out[i] = N;
+  t_out[i] = kf_inv_norm3(N.x, N.y, N.z);
   ok_out[i] = ok ? 1 : 0;
 }
 
@@ -709,11 +711,15 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 // STOP: the opt-in early stop is compiled in (kf_icp_converged, with the
 // thresholds and the first iteration that may stop in the device state), so
 // that the reference's fixed iteration count runs the kernel without it.
-template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH, bool STOP>
+// NT: the normal gate normalises the model normal by mt[j] (kf_inv_norm3 of
+// mn[j], kept with the model maps: it does not change between iterations)
+// instead of normalize3, the same float operations.
+template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH, bool STOP, bool NT>
 __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const float* __restrict__ depth, kf_p3* __restrict__ vmap_out, kf_p3* __restrict__ nmap_out,
     uint8_t* __restrict__ vmask_out, const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
-    const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, kf_cam cam, int H, int W,
+    const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, const float* __restrict__ mt,
+    kf_cam cam, int H, int W,
     float dist2_hi, float cos_lo, float min_inliers, kf_dev_state* __restrict__ st,
     const void* __restrict__ part_in_, void* __restrict__ part_out_,
     uint8_t* __restrict__ mask_out, int iter, int bpx, int nb_in) {
@@ -879,15 +885,19 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   // (vertex, then normal after the distance gate) become 2 PX dependent
   // memory round trips.
   kf_p3 S[PX], Sn[PX];
+  float St[NT ? PX : 1];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
     S[k] = mv[max(jx[k], 0)];
     Sn[k] = mn[max(jx[k], 0)];
+    if constexpr (NT) St[k] = mt[max(jx[k], 0)];
   }
 #pragma unroll
-  for (int k = 0; k < PX; ++k)
+  for (int k = 0; k < PX; ++k) {
     asm volatile("" ::"v"(S[k].x), "v"(S[k].y), "v"(S[k].z), "v"(Sn[k].x), "v"(Sn[k].y),
                  "v"(Sn[k].z));
+    if constexpr (NT) asm volatile("" ::"v"(St[k]));
+  }
   // phase 3: gates, Jacobian rows, accumulation
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
@@ -913,7 +923,14 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
         normalize3(nnx, nny, nnz);
         const kf_p3 Snk = Sn[k];
         float ax = Snk.x, ay = Snk.y, az = Snk.z;
-        normalize3(ax, ay, az);
+        if constexpr (NT) {
+          const float tn = St[k];
+          ax = ax * tn;
+          ay = ay * tn;
+          az = az * tn;
+        } else {
+          normalize3(ax, ay, az);
+        }
         const float c = (nnx * ax + nny * ay + nnz * az);
         if (c >= cos_lo && c <= 1.0f) {
           ok = true;
@@ -2912,11 +2929,13 @@ __device__ __forceinline__ void table_flush(const int* key, const float* val, in
 template <int MODE>
 __device__ __forceinline__ void store_maps(kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n,
                                            float2* __restrict__ out_aux,
-                                           int32_t* __restrict__ out_hit, int p, const float4& ov,
+                                           int32_t* __restrict__ out_hit,
+                                           float* __restrict__ out_nt, int p, const float4& ov,
                                            const float4& on, int hit) {
   out_v[p] = kf_p3{ov.x, ov.y, ov.z};
   out_n[p] = kf_p3{on.x, on.y, on.z};
   out_hit[p] = hit;
+  if (out_nt) out_nt[p] = kf_inv_norm3(on.x, on.y, on.z);
   if constexpr (MODE == kModeTrilinear) out_aux[p] = make_float2(ov.w, on.w);
 }
 
@@ -2924,7 +2943,8 @@ template <int PREC, bool COUNT, int MODE, bool MULTI>
 __global__ __launch_bounds__(256) void k_raycast(
     const void* __restrict__ volume, RayIO io, int H, kf_dev_state* __restrict__ st,
     kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n, float2* __restrict__ out_aux,
-    int32_t* __restrict__ out_hit, RayState* __restrict__ queue, int tiles_x, int n_blocks) {
+    int32_t* __restrict__ out_hit, float* __restrict__ out_nt, RayState* __restrict__ queue,
+    int tiles_x, int n_blocks) {
   const int bid = xcd_contiguous(blockIdx.x, n_blocks);
   const int W = io.W;
   const int tile = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -2976,7 +2996,7 @@ __global__ __launch_bounds__(256) void k_raycast(
       queue[(size_t)part * io.qcap + base + __popcll(mask & ((1ull << ln) - 1ull))] = r;
   }
   if (!active || MODE == kModeRefine) return;
-  store_maps<MODE>(out_v, out_n, out_aux, out_hit, p, ov, on, hit);
+  store_maps<MODE>(out_v, out_n, out_aux, out_hit, out_nt, p, ov, on, hit);
 }
 
 __device__ __forceinline__ void uniform_ray(RayState& r) {
@@ -3004,7 +3024,8 @@ template <int PREC, bool COUNT, int MODE, bool SKIP>
 __global__ __launch_bounds__(256) void k_raycast_rest(
     const void* __restrict__ volume, RayIO io, kf_dev_state* __restrict__ st,
     kf_p3* __restrict__ out_v, kf_p3* __restrict__ out_n, float2* __restrict__ out_aux,
-    int32_t* __restrict__ out_hit, const RayState* __restrict__ queue) {
+    int32_t* __restrict__ out_hit, float* __restrict__ out_nt,
+    const RayState* __restrict__ queue) {
   __shared__ WaveWalkLds lds[4];
   static_assert(KF_RAY_PARTS == 64, "one queue partition per lane");
   const int ln = threadIdx.x & 63;
@@ -3050,7 +3071,8 @@ __global__ __launch_bounds__(256) void k_raycast_rest(
                                                          visited, skipped, xi, sink, px);
     if constexpr (MODE == kModeRefine) gn_add_pixel(px, gn);
     if constexpr (MODE != kModeRefine) {
-      if (lane == owner) store_maps<MODE>(out_v, out_n, out_aux, out_hit, r.pix, ov, on, hit);
+      if (lane == owner)
+        store_maps<MODE>(out_v, out_n, out_aux, out_hit, out_nt, r.pix, ov, on, hit);
     }
     if constexpr (COUNT) add_counts(st, visited, hit >= 0, skipped);
   }
@@ -3336,6 +3358,7 @@ struct IcpLevel {
   int H, W, blocks, bpx, px, iters;
   kf_cam cam;
   const kf_p3 *vmap, *nmap, *mv, *mn;
+  const float* mt;  // kf_inv_norm3 of mn (null: the kernel normalises)
   const float* depth;
   kf_p3 *vmap_w, *nmap_w;
   uint8_t* mask_w;
@@ -3349,20 +3372,26 @@ static void launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin
   kf_p3* vw = from_depth ? L.vmap_w : nullptr;
   kf_p3* nw = from_depth ? L.nmap_w : nullptr;
   uint8_t* mw = from_depth ? L.mask_w : nullptr;
-#define KF_ICP_ITER(FD, ST)                                                                   \
-  k_icp_iter<KAHAN, PX, PPL, FD, ST><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(                \
-      d, vw, nw, mw, L.vmap, L.nmap, L.mv, L.mn, L.cam, L.H, L.W, c->icp_dist2_hi,             \
+#define KF_ICP_ITER(FD, ST, NT)                                                               \
+  k_icp_iter<KAHAN, PX, PPL, FD, ST, NT><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(            \
+      d, vw, nw, mw, L.vmap, L.nmap, L.mv, L.mn, L.mt, L.cam, L.H, L.W, c->icp_dist2_hi,       \
       c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in)
-  if (may_stop) {
-    if (from_depth)
-      KF_ICP_ITER(true, true);
+  if (!L.mt) {
+    // caller's model maps (the allocation-free entry point): no depth maps
+    if (may_stop)
+      KF_ICP_ITER(false, true, false);
     else
-      KF_ICP_ITER(false, true);
+      KF_ICP_ITER(false, false, false);
+  } else if (may_stop) {
+    if (from_depth)
+      KF_ICP_ITER(true, true, true);
+    else
+      KF_ICP_ITER(false, true, true);
   } else {
     if (from_depth)
-      KF_ICP_ITER(true, false);
+      KF_ICP_ITER(true, false, true);
     else
-      KF_ICP_ITER(false, false);
+      KF_ICP_ITER(false, false, true);
   }
 #undef KF_ICP_ITER
 }
@@ -3466,10 +3495,10 @@ static hipError_t build_pyramid(kf_context* c, const kf_cam& cam0, IcpLevel* lv)
     k_vertex<<<grid, 256, 0, c->stream>>>(P.depth, cam, P.H, P.W, P.vmap, P.nmap, P.mask);
     k_model_down<<<grid, 256, 0, c->stream>>>(v_in, n_in, l == 1 ? c->hit : nullptr,
                                               l == 1 ? nullptr : c->pyr[l - 1].ok, Wi, P.mv,
-                                              P.mn, P.ok, P.H, P.W);
+                                              P.mn, P.mt, P.ok, P.H, P.W);
     lv[l] = IcpLevel{P.H,   P.W,    P.blocks, P.bpx, P.px,    c->prm.icp_coarse_iters[l - 1],
-                     cam,   P.vmap, P.nmap,   P.mv,  P.mn,    nullptr,
-                     nullptr, nullptr, nullptr};
+                     cam,   P.vmap, P.nmap,   P.mv,  P.mn,    P.mt,
+                     nullptr, nullptr, nullptr, nullptr};
   }
   return hipGetLastError();
 }
@@ -3478,8 +3507,8 @@ hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam, bool maps_from_depth)
   IcpLevel lv[KF_MAX_ICP_LEVELS];
   lv[0] = IcpLevel{c->H,    c->W,    c->icp_blocks, c->icp_bpx,
                    c->icp_px, c->prm.icp_iterations, cam, c->vmap,
-                   c->nmap, c->model_v, c->model_n, maps_from_depth ? c->depth : nullptr,
-                   c->vmap, c->nmap, c->mask};
+                   c->nmap, c->model_v, c->model_n, c->model_nt,
+                   maps_from_depth ? c->depth : nullptr, c->vmap, c->nmap, c->mask};
   const int n_levels = c->prm.icp_levels > 1 ? c->prm.icp_levels : 1;
   if (n_levels > 1) {
     if (!c->pyr[n_levels - 1].depth) return hipErrorInvalidValue;
@@ -3711,6 +3740,7 @@ struct RayMaps {
   kf_p3* n;
   float2* aux;
   int32_t* hit;
+  float* nt;  // kf_inv_norm3 of each normal (null: not kept)
 };
 
 template <int PREC, bool COUNT, int MODE>
@@ -3719,18 +3749,18 @@ static void launch_raycast_t(kf_context* c, const RayIO& io, int tiles_x, int nb
   RayState* q = reinterpret_cast<RayState*>(c->ray_queue);
   if (MODE != kModeRefine && io.pass1_batches > kPass1Batches)
     k_raycast<PREC, COUNT, MODE, true><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, q, tiles_x, nb);
+        c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, m.nt, q, tiles_x, nb);
   else
     k_raycast<PREC, COUNT, MODE, false><<<nb, 256, 0, c->stream>>>(
-        c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, q, tiles_x, nb);
+        c->tsdf, io, c->H, c->state, m.v, m.n, m.aux, m.hit, m.nt, q, tiles_x, nb);
   RayIO io2 = io;
   io2.part_base = nb;
   if (io.skip_flags)
     k_raycast_rest<PREC, COUNT, MODE, true><<<kRestBlocks, 256, 0, c->stream>>>(
-        c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, q);
+        c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, m.nt, q);
   else
     k_raycast_rest<PREC, COUNT, MODE, false><<<kRestBlocks, 256, 0, c->stream>>>(
-        c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, q);
+        c->tsdf, io2, c->state, m.v, m.n, m.aux, m.hit, m.nt, q);
 }
 
 template <int PREC>
@@ -3750,7 +3780,8 @@ static void launch_raycast_p(kf_context* c, const RayIO& io, int tiles_x, int nb
 // switch (ignored by the modes that do not count).
 static hipError_t launch_raycast_mode(kf_context* c, const kf_cam& cam, int mode,
                                       const RayMaps* maps = nullptr, bool count = true) {
-  const RayMaps m = maps ? *maps : RayMaps{c->model_v, c->model_n, c->model_aux, c->hit};
+  const RayMaps m =
+      maps ? *maps : RayMaps{c->model_v, c->model_n, c->model_aux, c->hit, c->model_nt};
   if (!maps && mode != kModeRefine) c->maps_trilinear = mode == kModeTrilinear;
   const int tiles_x = (c->W + 15) / 16, tiles_y = (c->H + 15) / 16;
   const int nb = tiles_x * tiles_y;
@@ -3812,9 +3843,9 @@ hipError_t kf_launch_raycast_adjoint(kf_context* c, const kf_cam& cam, bool save
   // Without saved crossings for these inputs, the trilinear raycast first
   // walks every ray into the adjoint's own maps (the context's model maps
   // stay as they are); the adjoint then starts from those crossings.
-  RayMaps m{c->model_v, c->model_n, c->model_aux, c->hit};
+  RayMaps m{c->model_v, c->model_n, c->model_aux, c->hit, c->model_nt};
   if (!saved) {
-    m = RayMaps{c->adj_map_v, c->adj_map_n, c->adj_map_aux, c->adj_map_hit};
+    m = RayMaps{c->adj_map_v, c->adj_map_n, c->adj_map_aux, c->adj_map_hit, nullptr};
     hipError_t e = launch_raycast_mode(c, cam, kModeTrilinear, &m, false);
     if (e != hipSuccess) return e;
   }
@@ -3834,6 +3865,16 @@ size_t kf_ray_state_bytes() { return sizeof(RayState); }
 
 size_t kf_ray_queue_entries(int H, int W) {
   return (size_t)KF_RAY_PARTS * kf_ray_part_capacity(((W + 15) / 16) * ((H + 15) / 16));
+}
+
+__global__ __launch_bounds__(256) void k_fill_f32(float* __restrict__ p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t kf_launch_fill_f32(float* p, int64_t n, float v, hipStream_t s) {
+  k_fill_f32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
 }
 
 hipError_t kf_launch_volume_fill(kf_context* c) {

@@ -25,5 +25,6 @@ int hd_icp_update_gated(const double* sums, float* T, double min_inliers) {
 int hd_icp_converged(const float* inc, float stop_rot, float stop_trans) {
   return kf_icp_converged(inc, stop_rot, stop_trans) ? 1 : 0;
 }
+float hd_inv_norm3(float x, float y, float z) { return kf_inv_norm3(x, y, z); }
 
 }  // extern "C"

@@ -199,6 +199,8 @@ def hd():
     lib.hd_icp_increment_robust.argtypes = [d, f, C.c_double]
     lib.hd_icp_update_gated.argtypes = [d, f, C.c_double]
     lib.hd_icp_converged.argtypes = [f, C.c_float, C.c_float]
+    lib.hd_inv_norm3.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.hd_inv_norm3.restype = C.c_float
     return lib
 
 
@@ -379,3 +381,35 @@ def test_icp_converged_reads_the_six_increment_parameters(hd):
         nan = [0.0] * 6
         nan[k] = float("nan")
         assert conv(nan) == 0, k
+
+
+def test_inv_norm3_closed_forms_and_float_steps(hd):
+    """kf_inv_norm3 (the factor normalize scales by, cuda_functions.cuh:2-10):
+    closed forms (axis vectors 1; (3, 4, 0) and its power-of-two multiples
+    2^-k / 5 correctly rounded; the zero vector +inf; NaN in, NaN out), and on
+    random vectors equal to the float steps (x x + y y) + z z, square root,
+    quotient, each correctly rounded, and within 4 ulp of the exact 1 / |v|
+    (the five roundings add at most 3.5 * 2^-24 relative)."""
+    f32 = np.float32
+    for v in ((1, 0, 0), (0, -1, 0), (0, 0, 1), (-1, 0, 0)):
+        assert hd.hd_inv_norm3(*v) == 1.0
+    for k in range(-20, 21):
+        s = 2.0 ** k
+        want = f32(Fraction(1, 5) / Fraction(s))
+        for v in ((3 * s, 4 * s, 0.0), (0.0, -4 * s, 3 * s), (4 * s, 0.0, -3 * s)):
+            assert hd.hd_inv_norm3(*v) == want, (k, v)
+    assert hd.hd_inv_norm3(0.0, 0.0, 0.0) == float("inf")
+    assert hd.hd_inv_norm3(-0.0, 0.0, -0.0) == float("inf")
+    assert math.isnan(hd.hd_inv_norm3(float("nan"), 1.0, 0.0))
+    rng = np.random.default_rng(11)
+    vs = rng.normal(size=(2000, 3)).astype(np.float32)
+    vs[:500] *= np.float32(1e3)
+    vs[500:1000] /= np.linalg.norm(vs[500:1000], axis=1, keepdims=True).astype(np.float32)
+    for x, y, z in vs:
+        got = hd.hd_inv_norm3(float(x), float(y), float(z))
+        s = (f32(x) * f32(x) + f32(y) * f32(y)) + f32(z) * f32(z)
+        assert got == f32(1) / np.sqrt(s), (x, y, z)
+        exact = 1.0 / math.sqrt(float(Fraction(float(x)) ** 2 + Fraction(float(y)) ** 2 +
+                                      Fraction(float(z)) ** 2))
+        ulp = float(np.spacing(f32(exact)))
+        assert abs(got - exact) <= 4 * ulp, (x, y, z)
