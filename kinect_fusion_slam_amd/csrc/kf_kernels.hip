@@ -675,16 +675,28 @@ __device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh, float min_i
       sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? sh.rowsum[2 * k] + sh.rowsum[2 * k + 1] : sh.rowsum[k];
     }
     icp_wave_sync();
-    if (threadIdx.x == 0)
-      sh.status = min_inliers >= 0.0f
-                      ? kf_icp_increment_robust(sh.sums, sh.inc, (double)min_inliers)
-                      : kf_icp_increment(sh.sums, sh.inc);
+    if (threadIdx.x == 0) {
+      // the 28 sums in registers before the solve starts: left to the
+      // compiler, each LDS read would be waited for inside the FP64 chain
+      double s[KF_ICP_SUMS];
+#pragma unroll
+      for (int k = 0; k < KF_ICP_SUMS; ++k) s[k] = sh.sums[k];
+#pragma unroll
+      for (int k = 0; k < KF_ICP_SUMS; ++k) asm volatile("" ::"v"(s[k]));
+      sh.status = min_inliers >= 0.0f ? kf_icp_increment_robust(s, sh.inc, (double)min_inliers)
+                                      : kf_icp_increment(s, sh.inc);
+    }
     icp_wave_sync();
-    if (threadIdx.x < 16)
-      sh.Tn[threadIdx.x] = kf_mat4_mul_elem(sh.inc, sh.T0, threadIdx.x >> 2, threadIdx.x & 3);
-    icp_wave_sync();
-    if (threadIdx.x < 16)
-      sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
+    if (threadIdx.x < 16) {
+      // lane (i, j): column j of T_new = inc * T0 (each lane the same values),
+      // then F(i, j) = (Tp * T_new)(i, j), with no LDS round trip between
+      const int i = threadIdx.x >> 2, j = threadIdx.x & 3;
+      float col[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) col[k] = kf_mat4_mul_elem(sh.inc, sh.T0, k, j);
+      sh.Tn[threadIdx.x] = i == 0 ? col[0] : (i == 1 ? col[1] : (i == 2 ? col[2] : col[3]));
+      sh.F[threadIdx.x] = kf_mat4_row_col(sh.Tp, i, col);
+    }
   }
   __syncthreads();
 }
