@@ -94,18 +94,13 @@ KF_HD float kf_expf(float x) {
 KF_HD float kf_inv_norm3(float x, float y, float z) { return 1 / sqrtf(x * x + y * y + z * z); }
 
 // Element (i, j) of A * B for 4x4 row-major float matrices, FP64 products
-// summed left to right and rounded once; kf_mat4_row_col takes column j of
-// B as bcol[0..3].
-KF_HD float kf_mat4_row_col(const float* A, int i, const float* bcol) {
-  double s = (double)A[i * 4 + 0] * (double)bcol[0];
-  s = s + (double)A[i * 4 + 1] * (double)bcol[1];
-  s = s + (double)A[i * 4 + 2] * (double)bcol[2];
-  s = s + (double)A[i * 4 + 3] * (double)bcol[3];
-  return (float)s;
-}
+// summed left to right and rounded once.
 KF_HD float kf_mat4_mul_elem(const float* A, const float* B, int i, int j) {
-  const float bcol[4] = {B[0 * 4 + j], B[1 * 4 + j], B[2 * 4 + j], B[3 * 4 + j]};
-  return kf_mat4_row_col(A, i, bcol);
+  double s = (double)A[i * 4 + 0] * (double)B[0 * 4 + j];
+  s = s + (double)A[i * 4 + 1] * (double)B[1 * 4 + j];
+  s = s + (double)A[i * 4 + 2] * (double)B[2 * 4 + j];
+  s = s + (double)A[i * 4 + 3] * (double)B[3 * 4 + j];
+  return (float)s;
 }
 
 // C = A * B, 4x4 row-major; C may alias A or B.

@@ -675,28 +675,16 @@ __device__ __forceinline__ void icp_update_pose(IcpShared<ROWS>& sh, float min_i
       sh.sums[k] = ROWS == 2 * KF_ICP_SUMS ? sh.rowsum[2 * k] + sh.rowsum[2 * k + 1] : sh.rowsum[k];
     }
     icp_wave_sync();
-    if (threadIdx.x == 0) {
-      // the 28 sums in registers before the solve starts: left to the
-      // compiler, each LDS read would be waited for inside the FP64 chain
-      double s[KF_ICP_SUMS];
-#pragma unroll
-      for (int k = 0; k < KF_ICP_SUMS; ++k) s[k] = sh.sums[k];
-#pragma unroll
-      for (int k = 0; k < KF_ICP_SUMS; ++k) asm volatile("" ::"v"(s[k]));
-      sh.status = min_inliers >= 0.0f ? kf_icp_increment_robust(s, sh.inc, (double)min_inliers)
-                                      : kf_icp_increment(s, sh.inc);
-    }
+    if (threadIdx.x == 0)
+      sh.status = min_inliers >= 0.0f
+                      ? kf_icp_increment_robust(sh.sums, sh.inc, (double)min_inliers)
+                      : kf_icp_increment(sh.sums, sh.inc);
     icp_wave_sync();
-    if (threadIdx.x < 16) {
-      // lane (i, j): column j of T_new = inc * T0 (each lane the same values),
-      // then F(i, j) = (Tp * T_new)(i, j), with no LDS round trip between
-      const int i = threadIdx.x >> 2, j = threadIdx.x & 3;
-      float col[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) col[k] = kf_mat4_mul_elem(sh.inc, sh.T0, k, j);
-      sh.Tn[threadIdx.x] = i == 0 ? col[0] : (i == 1 ? col[1] : (i == 2 ? col[2] : col[3]));
-      sh.F[threadIdx.x] = kf_mat4_row_col(sh.Tp, i, col);
-    }
+    if (threadIdx.x < 16)
+      sh.Tn[threadIdx.x] = kf_mat4_mul_elem(sh.inc, sh.T0, threadIdx.x >> 2, threadIdx.x & 3);
+    icp_wave_sync();
+    if (threadIdx.x < 16)
+      sh.F[threadIdx.x] = kf_mat4_mul_elem(sh.Tp, sh.Tn, threadIdx.x >> 2, threadIdx.x & 3);
   }
   __syncthreads();
 }
@@ -2554,13 +2542,22 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
   uint64_t qmask = ~0ull;
   float tq = 0.0f, qagain = -INFINITY;
   int pidx = -1;
+  // The parameter at which the walk entered its current voxel, which the
+  // query's windows are aligned to.  r.tent holds it in the trilinear modes;
+  // the reference mode's first pass does not keep it, so there it is the
+  // last boundary crossed on any axis, max_a (t_a - td_a) (within an ulp;
+  // an axis that has not stepped gives a value before the walk's start).
+  // Rounds keep r.tent exact from then on.
+  float tpos = r.tent;
+  if constexpr (MODE == kModeReference)
+    tpos = fmaxf(tpos, fmaxf(r.cur.tmx - p.tdx, fmaxf(r.cur.tmy - p.tdy, r.cur.tmz - p.tdz)));
   // Each round either ends the walk or advances it by kWaveSteps steps (a
   // skip round: at least one step) along the fixed step directions, so the
   // walk leaves the volume after at most Nx + Ny + Nz rounds.
   for (;;) {
     const DdaState c0 = r.cur;
     // the query's flag loads go out before the sequences and are used after
-    const bool query = skip_on && !(r.tent < qagain);
+    const bool query = skip_on && !(tpos < qagain);
     uint8_t qflag = 1;
     if (query) qflag = skip_flag_at(io.skip_flags, vg, c0, r.dx, r.dy, r.dz);
     // 1. per-axis boundary distances (lanes 0-2; the others write row 3, so
@@ -2588,7 +2585,7 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     const float ex = L.seq[0][lane], ey = L.seq[1][lane], ez = L.seq[2][lane];
     if (query) {
       qmask = __ballot(qflag != 0);
-      tq = r.tent;
+      tq = tpos;
       qagain = uni(tq + (float)(kSkipWindows - 16) * s_mm);
     }
     // skip round: every step whose boundary distance lies below tr, where tr
@@ -2602,9 +2599,9 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     if (skip_on && (c0.ix < Nx - 1 && c0.ix > 0 && c0.iy < Ny - 1 && c0.iy > 0 &&
                     c0.iz < Nz - 1 && c0.iz > 0) &&
         (pidx >= 0 || !(r.prev < 0.f))) {
-      // windows that end before tent are ignored (rounded towards keeping
+      // windows that end before tpos are ignored (rounded towards keeping
       // them); k0 <= kSkipWindows - 16 since the walk queries again there
-      const int k0 = max(0, (int)floorf((r.tent - tq) / s_mm) - 1);
+      const int k0 = max(0, (int)floorf((tpos - tq) / s_mm) - 1);
       const uint64_t ahead = qmask >> k0;
       const float tneg = ahead ? tq + ((float)(k0 + __ffsll((unsigned long long)ahead) - 1) - 0.5f) * s_mm
                                : tq + ((float)kSkipWindows - 0.5f) * s_mm;
@@ -2654,6 +2651,7 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
           c.tmy = uni(L.seq[1][ny]);
           c.tmz = uni(L.seq[2][nz]);
           r.tent = fmaxf(vx, fmaxf(vy, vz));
+          tpos = r.tent;
           // the last voxel the skipped steps visited: one step back along the
           // axis of the last step (its TSDF is the next visit's prev)
           pidx = kf_store_index(vg, c.ix - (lx ? p.sx : 0) - 1, c.iy - (ly ? p.sy : 0) - 1,
@@ -2794,6 +2792,7 @@ __device__ __forceinline__ int wave_walk(const VolRead<PREC>& F, RayState& r, co
     r.tent = uni(L.key[kWaveSteps - 1]);
     r.prev = uni(__shfl(f, kWaveSteps - 1));
     pidx = -1;
+    tpos = r.tent;
     wave_lds_sync();  // every lane has read this round's records
     if constexpr (MODE == kModeRefine) {
       if (r.tent > io.walk_tmax) return 0;  // no crossing ahead can be used

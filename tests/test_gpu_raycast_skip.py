@@ -70,14 +70,7 @@ def test_room_stream_skip_on_equals_off(kf, precision, mode):
     assert on[5] > 0.02 * on[4][0], "skip rounds were not taken"
 
 
-@pytest.mark.parametrize("precision", ["f32", "f16"])
-@pytest.mark.parametrize("neg_brick_frac", [0.0, 0.01, 0.05])
-def test_adversarial_volume_skip_equals_oracle(kf, oracle, precision, neg_brick_frac):
-    """Isolated negative voxels in a fraction of the bricks, free space of NaN,
-    positive values, +0 and -0, and rays that start behind the camera: the
-    skip rounds must stop before every negative voxel a walk meets.  The
-    uploaded volume's flags are rebuilt before the raycast.  F16: the
-    volume is rounded to half first, and the oracle walks the rounded one."""
+def adversarial_case(kf, precision, neg_brick_frac):
     rng = np.random.default_rng(int(neg_brick_frac * 1000) + 101)
     vol, prm, K, H, W = small_setup(kf, H=96, W=128, dims=(90, 84, 120), start=(-45, -42, -10),
                                     voxel=20.0)
@@ -92,8 +85,12 @@ def test_adversarial_volume_skip_equals_oracle(kf, oracle, precision, neg_brick_
     a = 0.3
     T = np.array([[np.cos(a), 0, np.sin(a), 40.0], [0, 1, 0, -30.0],
                   [-np.sin(a), 0, np.cos(a), 15.0], [0, 0, 0, 1]], np.float32)
-    say(f"[gpu] raycast skip: adversarial {precision} volume, negative bricks {neg_brick_frac}")
-    prm.raycast_skip = 1
+    return vol, prm, K, H, W, t, w, d, T
+
+
+def raycast_counted(kf, vol, prm, H, W, t, w, d, K, T, skip, mode):
+    prm.raycast_skip = skip
+    prm.raycast_mode = mode
     with kf.KinfuContext(vol, prm, H, W) as c:
         c.upload_volume(t, w)
         c.upload_depth_f32(d)
@@ -102,13 +99,42 @@ def test_adversarial_volume_skip_equals_oracle(kf, oracle, precision, neg_brick_
         visited, hits = c.raycast_work()
         skipped = c.raycast_skipped()
         got = c.model_maps()
-    want = oracle.raycast(d, K, T, t, vol, prm.truncation_mm, 0)
-    for k in MAP_KEYS:
+        if mode == 1:
+            got["depth"] = c.rendered_depth()
+    return got, (visited, hits), skipped
+
+
+@pytest.mark.parametrize("precision,neg_brick_frac,mode",
+                         [("f32", 0.0, 0), ("f32", 0.01, 0), ("f32", 0.05, 0),
+                          ("f16", 0.0, 0), ("f16", 0.01, 0), ("f16", 0.05, 0),
+                          ("f32", 0.0, 1), ("f32", 0.05, 1), ("f16", 0.01, 1)])
+def test_adversarial_volume_skip_equals_oracle(kf, oracle, precision, neg_brick_frac, mode):
+    """Isolated negative voxels in a fraction of the bricks, free space of NaN,
+    positive values, +0 and -0, and rays that start behind the camera: the
+    skip rounds must stop before every negative voxel a walk meets.  The
+    uploaded volume's flags are rebuilt before the raycast.  F16: the
+    volume is rounded to half first, and the oracle walks the rounded one.
+    Mode 1 (trilinear hit handling, the same walk): against the oracle's
+    trilinear raycast, and the visit counts against the walk without skip
+    rounds."""
+    vol, prm, K, H, W, t, w, d, T = adversarial_case(kf, precision, neg_brick_frac)
+    say(f"[gpu] raycast skip: adversarial {precision} volume, negative bricks {neg_brick_frac}, "
+        f"mode {mode}")
+    got, work, skipped = raycast_counted(kf, vol, prm, H, W, t, w, d, K, T, 1, mode)
+    if mode == 0:
+        want = oracle.raycast(d, K, T, t, vol, prm.truncation_mm, 0)
+        assert work == (want["steps"], want["hits"])
+        keys = MAP_KEYS
+    else:
+        want = oracle.raycast_tri(d, K, T, t, vol, prm.truncation_mm)
+        _, work_off, _ = raycast_counted(kf, vol, prm, H, W, t, w, d, K, T, 0, mode)
+        assert work == work_off
+        keys = MAP_KEYS + ("depth",)
+    for k in keys:
         assert_same(got[k], want[k], k)
-    assert (visited, hits) == (want["steps"], want["hits"])
-    say(f"[gpu] raycast skip: {skipped} of {visited} visits skipped")
+    say(f"[gpu] raycast skip: {skipped} of {work[0]} visits skipped")
     if neg_brick_frac <= 0.01:
-        assert skipped > visited // 4, "skip rounds were not taken"
+        assert skipped > work[0] // 4, "skip rounds were not taken"
 
 
 def test_flags_rebuilt_after_device_address(kf):
@@ -160,3 +186,23 @@ def test_skip_switched_on_after_integration_without_flags(kf):
         assert_same(got[0][k], want[0][k], k)
     assert got[1] == want[1]
     assert got[2] > 0
+
+
+def test_refinement_with_skip_rounds_equals_without(kf):
+    """The pose refinement's raycasts (mode 3, walks bounded by the residual
+    gate) and the pipeline's trilinear raycast with skip rounds on and off:
+    the same poses after every frame."""
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
+    n = 8
+    frames, _ = kf.stream(kf.preset_scene(kf.PRESET_ROOM), K, H, W, 0, n, threads=16)
+    poses = {}
+    for skip in (1, 0):
+        say(f"[gpu] raycast skip: {n} frames with 2 refinement steps, skip {skip}")
+        prm.raycast_skip = skip
+        prm.raycast_mode = kf.KF_RAYCAST_TRILINEAR
+        prm.refine_steps = 2
+        with kf.KinfuContext(vol, prm, H, W) as c:
+            c.stream_upload(frames)
+            c.pipeline_run(K, 0, n)
+            poses[skip] = c.poses(n)
+    assert_same(poses[1], poses[0], "poses")
