@@ -190,9 +190,16 @@ bool params_valid(const kf_params* p, std::string* why) {
 // ICP grid: one block per compute unit, the image split evenly over all of
 // them (a CU with two blocks would set the time of every iteration, and each
 // block's two waves per SIMD hide each other's VALU latency).  Up to 4 pixels
-// per thread; larger images keep 4 and take more blocks (KF_ICP_MAX_BLOCKS
-// blocks cover 2^20 pixels).  The grid fixes the order of the ICP sums.
-void icp_grid(size_t n, int n_cu, int* blocks, int* bpx, int* px) {
+// per thread of a KF_ICP_BLOCK-thread block.  Larger images take blocks of
+// 2 KF_ICP_BLOCK threads (3 or 4 pixels each, still one block per compute
+// unit): every block reads every block partial of the iteration before, so
+// the block count squared sets those reads (at 1280x720, 450 blocks of 512
+// threads read 45 MB per iteration, 256 of 1024 threads 15 MB).  Beyond
+// that, 4 pixels per thread of KF_ICP_BLOCK threads and more blocks
+// (KF_ICP_MAX_BLOCKS blocks cover 2^20 pixels).  The grid fixes the order of
+// the ICP sums.
+void icp_grid(size_t n, int n_cu, int* blocks, int* bpx, int* px, int* threads) {
+  *threads = KF_ICP_BLOCK;
   *px = 4;
   *bpx = KF_ICP_BLOCK * 4;
   *blocks = (int)((n + *bpx - 1) / *bpx);
@@ -203,6 +210,11 @@ void icp_grid(size_t n, int n_cu, int* blocks, int* bpx, int* px) {
     *blocks = (int)((n + b - 1) / b);
     *bpx = b;
     *px = (b + KF_ICP_BLOCK - 1) / KF_ICP_BLOCK;
+  } else if (b <= 8 * KF_ICP_BLOCK) {
+    *threads = 2 * KF_ICP_BLOCK;
+    *blocks = (int)((n + b - 1) / b);
+    *bpx = b;
+    *px = (b + 2 * KF_ICP_BLOCK - 1) / (2 * KF_ICP_BLOCK);  // 3 or 4
   }
 }
 
@@ -523,7 +535,7 @@ static int alloc_pyramid(kf_context* c, int levels) {
     const size_t n = (size_t)H * W;
     P.H = H;
     P.W = W;
-    icp_grid(n, device_cus(c->device), &P.blocks, &P.bpx, &P.px);
+    icp_grid(n, device_cus(c->device), &P.blocks, &P.bpx, &P.px, &P.threads);
     struct {
       void** p;
       size_t bytes;
@@ -625,7 +637,7 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
 
   const size_t n = (size_t)height * width;
   // ICP grid before the device is known (the block-count check)
-  icp_grid(n, KF_ICP_MAX_BLOCKS, &c->icp_blocks, &c->icp_bpx, &c->icp_px);
+  icp_grid(n, KF_ICP_MAX_BLOCKS, &c->icp_blocks, &c->icp_bpx, &c->icp_px, &c->icp_threads);
   c->volume_bytes = (size_t)c->n_store * (vol->precision == KF_TSDF_F32 ? 8 : 4);
   if (c->icp_blocks > KF_ICP_MAX_BLOCKS) {
     delete c;
@@ -637,7 +649,8 @@ int kf_create(const kf_volume_desc* vol, int32_t height, int32_t width, const kf
     return kf_fail(code, m);
   };
   if (hipGetDevice(&c->device) != hipSuccess) return fail(KF_ERR_HIP, "no HIP device");
-  icp_grid(n, device_cus(c->device), &c->icp_blocks, &c->icp_bpx, &c->icp_px);
+  icp_grid(n, device_cus(c->device), &c->icp_blocks, &c->icp_bpx, &c->icp_px,
+           &c->icp_threads);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(KF_ERR_HIP, "hipStreamCreate failed");
   struct Alloc {
@@ -990,7 +1003,8 @@ int kf_estimate_pose_static(const float* vmap, const float* nmap, const float* m
   c.prm = *prm;
   c.prm.record_icp = 0;
   c.stream = static_cast<hipStream_t>(stream);
-  icp_grid((size_t)height * width, device_cus(device), &c.icp_blocks, &c.icp_bpx, &c.icp_px);
+  icp_grid((size_t)height * width, device_cus(device), &c.icp_blocks, &c.icp_bpx, &c.icp_px,
+           &c.icp_threads);
   KF_REQUIRE(c.icp_blocks <= KF_ICP_MAX_BLOCKS, KF_ERR_INVALID,
              "image too large for the static ICP scratch (at most 2^20 pixels)");
   c.vmap = reinterpret_cast<kf_p3*>(const_cast<float*>(vmap));

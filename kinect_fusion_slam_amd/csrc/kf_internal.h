@@ -104,7 +104,7 @@ __host__ __device__ inline int kf_store_index(const kf_vol& v, int x, int y, int
 // image and ICP grid, the frame's depth, vertex, normal and mask maps, and
 // the model maps with their surface flags.
 struct kf_pyr_level {
-  int H = 0, W = 0, blocks = 0, bpx = 0, px = 0;
+  int H = 0, W = 0, blocks = 0, bpx = 0, px = 0, threads = KF_ICP_BLOCK;
   float* depth = nullptr;
   kf_p3* vmap = nullptr;
   kf_p3* nmap = nullptr;
@@ -187,6 +187,7 @@ struct kf_context {
   int icp_blocks = 0;
   int icp_bpx = 0;                 // pixels per block
   int icp_px = 4;                  // pixels per thread, at most
+  int icp_threads = KF_ICP_BLOCK;  // threads per block (icp_grid)
   uint8_t* icp_masks = nullptr;    // [KF_MAX_ICP_ITERS][H*W] when record_icp
   float icp_cos_lo = 0.f;          // ICP normal gate as a cosine threshold
   float icp_dist2_hi = 0.f;        // ICP distance gate as a squared-distance threshold

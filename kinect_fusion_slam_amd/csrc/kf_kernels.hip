@@ -714,8 +714,12 @@ __device__ __forceinline__ void icp_record(kf_dev_state* st, int it, const IcpSh
 // NT: the normal gate normalises the model normal by mt[j] (kf_inv_norm3 of
 // mn[j], kept with the model maps: it does not change between iterations)
 // instead of normalize3, the same float operations.
-template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH, bool STOP, bool NT>
-__global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
+// BLK: threads per block, KF_ICP_BLOCK, or 2 KF_ICP_BLOCK for images above
+// one block of 4 KF_ICP_BLOCK pixels per compute unit (icp_grid): every
+// block reads every block partial of the iteration before, so halving the
+// block count quarters those reads.
+template <bool KAHAN, int PX, int PPL, bool FROM_DEPTH, bool STOP, bool NT, int BLK>
+__global__ __launch_bounds__(BLK) void k_icp_iter(
     const float* __restrict__ depth, kf_p3* __restrict__ vmap_out, kf_p3* __restrict__ nmap_out,
     uint8_t* __restrict__ vmask_out, const kf_p3* __restrict__ vmap, const kf_p3* __restrict__ nmap,
     const kf_p3* __restrict__ mv, const kf_p3* __restrict__ mn, const float* __restrict__ mt,
@@ -730,19 +734,19 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   const PT* __restrict__ part_in = static_cast<const PT*>(part_in_);
   PT* __restrict__ part_out = static_cast<PT*>(part_out_);
   __shared__ IcpShared<ROWS> sh;
-  __shared__ PT red_s[KF_ICP_BLOCK / 64][NS];
-  __shared__ PT red_c[KAHAN ? KF_ICP_BLOCK / 64 : 1][NS];
+  __shared__ PT red_s[BLK / 64][NS];
+  __shared__ PT red_c[KAHAN ? BLK / 64 : 1][NS];
   const int nb = gridDim.x;
 
   // chunk c takes pixels [c bpx, (c + 1) bpx) of the image, thread t the
-  // pixels t, t + KF_ICP_BLOCK, ... below bpx (bpx <= PX KF_ICP_BLOCK); each
+  // pixels t, t + BLK, ... below bpx (bpx <= PX BLK); each
   // XCD takes a contiguous band of chunks, so the model rows its blocks
   // gather are shared in its L2 instead of fetched by two XCDs at every
   // chunk boundary.  The partials stay indexed by block.
   const int n = H * W;
   const int chunk = xcd_contiguous(blockIdx.x, nb);
   const int base = chunk * bpx + threadIdx.x;
-  const int lim = min(bpx, n - chunk * bpx) - (int)threadIdx.x;  // k valid: k * KF_ICP_BLOCK < lim
+  const int lim = min(bpx, n - chunk * bpx) - (int)threadIdx.x;  // k valid: k * BLK < lim
   // phase 1: the previous iteration's block partials and the two poses it
   // needs, then this thread's vertex and normal loads, all issued before
   // the previous iteration is finished (the maps do not depend on the pose).
@@ -765,8 +769,8 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   const KF_GLOBAL kf_p3* vm = (const KF_GLOBAL kf_p3*)vm_;
   const KF_GLOBAL kf_p3* nm = (const KF_GLOBAL kf_p3*)nm_;
   const KF_GLOBAL float* dp = (const KF_GLOBAL float*)dp_;
-  IcpPartialRegs<ROWS, KF_ICP_BLOCK, PPL, PT> R;
-  icp_load_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(pin, nb_in, R);
+  IcpPartialRegs<ROWS, BLK, PPL, PT> R;
+  icp_load_partials<ROWS, BLK, PPL, PT>(pin, nb_in, R);
   const int e16 = threadIdx.x & 15;
   const float T0e = iter <= 1 ? stp->Tprev[e16] : stp->iter_T[iter - 2][e16];
   const float Tpe = stp->Tprev_inv[e16];
@@ -781,7 +785,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   float Di[PD], Dr[PD], Dc[PD];
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
-    const int x = min(base + k * KF_ICP_BLOCK, n - 1);
+    const int x = min(base + k * BLK, n - 1);
     if constexpr (FROM_DEPTH) {
       const int py = x / W, px = x - py * W;
       Di[k] = dp[x];
@@ -818,7 +822,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
       sh.T0[threadIdx.x] = T0e;
       sh.Tp[threadIdx.x] = Tpe;
     }
-    icp_sum_partials<ROWS, KF_ICP_BLOCK, PPL, PT>(R, nb_in, sh);
+    icp_sum_partials<ROWS, BLK, PPL, PT>(R, nb_in, sh);
     icp_update_pose<ROWS>(sh, min_inliers);
     if (blockIdx.x == 0) icp_record<ROWS>(st, iter - 1, sh);
     // opt-in early stop after a small increment (every block decides alike
@@ -835,11 +839,11 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     // the later iterations
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
-      const int x = min(base + k * KF_ICP_BLOCK, n - 1);
+      const int x = min(base + k * BLK, n - 1);
       const int py = x / W, px = x - py * W;
       uint8_t m;
       vertex_at(px, py, H, W, cam.Kinv, Di[k], Dr[k], Dc[k], V[k], Nc[k], m);
-      if (k * KF_ICP_BLOCK < lim) {
+      if (k * BLK < lim) {
         vmap_out[x] = V[k];
         nmap_out[x] = Nc[k];
         vmask_out[x] = m;
@@ -857,7 +861,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
 
 #pragma unroll
   for (int k = 0; k < PX; ++k)
-    if (k * KF_ICP_BLOCK >= lim) V[k] = Nc[k] = kf_p3{0.f, 0.f, 0.f};
+    if (k * BLK >= lim) V[k] = Nc[k] = kf_p3{0.f, 0.f, 0.f};
   // phase 2: projective association, then the model gathers, issued together.
   // Vz != 0 is the reference's "Vz != 0 and mask" (the vertex map stores 0
   // where the mask is 0).
@@ -901,7 +905,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
   // phase 3: gates, Jacobian rows, accumulation
 #pragma unroll
   for (int k = 0; k < PX; ++k) {
-    const int x = base + k * KF_ICP_BLOCK;
+    const int x = base + k * BLK;
     bool ok = false;
     float a[6], b = 0.f;
     if (jx[k] >= 0) {
@@ -944,7 +948,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
         }
       }
     }
-    if (mask_out && k * KF_ICP_BLOCK < lim) mask_out[x] = ok ? 1 : 0;
+    if (mask_out && k * BLK < lim) mask_out[x] = ok ? 1 : 0;
     if (ok) {
       int q = 0;
 #pragma unroll
@@ -1002,7 +1006,7 @@ __global__ __launch_bounds__(KF_ICP_BLOCK) void k_icp_iter(
     const int k = threadIdx.x;
     PT s = red_s[0][k], c = KAHAN ? red_c[0][k] : (PT)0;
 #pragma unroll
-    for (int w = 1; w < KF_ICP_BLOCK / 64; ++w) {
+    for (int w = 1; w < BLK / 64; ++w) {
       if constexpr (KAHAN)
         two_sum_add(s, c, red_s[w][k], red_c[w][k]);
       else
@@ -3366,7 +3370,7 @@ hipError_t kf_launch_vertex(kf_context* c, const kf_cam& cam) {
 // depth set: the first iteration computes the frame's maps from it and
 // writes them to vmap_w / nmap_w / mask_w (the pipeline's level 0).
 struct IcpLevel {
-  int H, W, blocks, bpx, px, iters;
+  int H, W, blocks, bpx, px, threads, iters;
   kf_cam cam;
   const kf_p3 *vmap, *nmap, *mv, *mn;
   const float* mt;  // kf_inv_norm3 of mn (null: the kernel normalises)
@@ -3376,17 +3380,42 @@ struct IcpLevel {
 };
 
 template <bool KAHAN, int PX, int PPL>
-static void launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin, void* pout,
-                               uint8_t* m, int it, int nb_in, float min_inl, bool from_depth,
-                               bool may_stop) {
+static hipError_t launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin,
+                                     void* pout, uint8_t* m, int it, int nb_in, float min_inl,
+                                     bool from_depth, bool may_stop) {
   const float* d = from_depth ? L.depth : nullptr;
   kf_p3* vw = from_depth ? L.vmap_w : nullptr;
   kf_p3* nw = from_depth ? L.nmap_w : nullptr;
   uint8_t* mw = from_depth ? L.mask_w : nullptr;
-#define KF_ICP_ITER(FD, ST, NT)                                                               \
-  k_icp_iter<KAHAN, PX, PPL, FD, ST, NT><<<L.blocks, KF_ICP_BLOCK, 0, c->stream>>>(            \
+#define KF_ICP_ITER_B(FD, ST, NT, B)                                                          \
+  k_icp_iter<KAHAN, PX, PPL, FD, ST, NT, B><<<L.blocks, B, 0, c->stream>>>(                    \
       d, vw, nw, mw, L.vmap, L.nmap, L.mv, L.mn, L.mt, L.cam, L.H, L.W, c->icp_dist2_hi,       \
       c->icp_cos_lo, min_inl, c->state, pin, pout, m, it, L.bpx, nb_in)
+#define KF_ICP_ITER(FD, ST, NT) KF_ICP_ITER_B(FD, ST, NT, KF_ICP_BLOCK)
+  if (L.threads == 2 * KF_ICP_BLOCK) {
+    // the large-image grid (icp_grid): 3 or 4 pixels per thread, at most one
+    // block per compute unit (so 4 partials per lane cover the grid)
+    if constexpr ((PX == 3 || PX == 4) && PPL == 4) {
+      if (!L.mt) {
+        if (may_stop)
+          KF_ICP_ITER_B(false, true, false, 2 * KF_ICP_BLOCK);
+        else
+          KF_ICP_ITER_B(false, false, false, 2 * KF_ICP_BLOCK);
+      } else if (may_stop) {
+        if (from_depth)
+          KF_ICP_ITER_B(true, true, true, 2 * KF_ICP_BLOCK);
+        else
+          KF_ICP_ITER_B(false, true, true, 2 * KF_ICP_BLOCK);
+      } else {
+        if (from_depth)
+          KF_ICP_ITER_B(true, false, true, 2 * KF_ICP_BLOCK);
+        else
+          KF_ICP_ITER_B(false, false, true, 2 * KF_ICP_BLOCK);
+      }
+      return hipGetLastError();
+    }
+    return hipErrorInvalidConfiguration;  // icp_grid gives no other such grid
+  }
   if (!L.mt) {
     // caller's model maps (the allocation-free entry point): no depth maps
     if (may_stop)
@@ -3405,6 +3434,8 @@ static void launch_icp_iter_px(kf_context* c, const IcpLevel& L, const void* pin
       KF_ICP_ITER(false, false, true);
   }
 #undef KF_ICP_ITER
+#undef KF_ICP_ITER_B
+  return hipGetLastError();
 }
 
 // The iterations of every level, coarsest first (lv[n_levels - 1] ... lv[0]),
@@ -3435,25 +3466,25 @@ static hipError_t launch_icp_levels(kf_context* c, const IcpLevel* lv, int n_lev
       const void* pin = pbase + (size_t)((it + 1) & 1) * half;
       void* pout = pbase + (size_t)(it & 1) * half;
       if (it == 0) kf_stage_begin(c, KF_STAGE_ICP_ITER);
+      hipError_t e;
       switch (L.px) {
         case 1:
-          launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
-                                             may_stop);
+          e = launch_icp_iter_px<KAHAN, 1, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                                may_stop);
           break;
         case 2:
-          launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
-                                             may_stop);
+          e = launch_icp_iter_px<KAHAN, 2, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                                may_stop);
           break;
         case 3:
-          launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
-                                             may_stop);
+          e = launch_icp_iter_px<KAHAN, 3, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                                may_stop);
           break;
         default:
-          launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
-                                             may_stop);
+          e = launch_icp_iter_px<KAHAN, 4, PPL>(c, L, pin, pout, m, it, nb_prev, min_inl, fd,
+                                                may_stop);
           break;
       }
-      hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       if (it == total - 1) kf_stage_end(c, KF_STAGE_ICP_ITER);
       nb_prev = L.blocks;
@@ -3507,7 +3538,7 @@ static hipError_t build_pyramid(kf_context* c, const kf_cam& cam0, IcpLevel* lv)
     k_model_down<<<grid, 256, 0, c->stream>>>(v_in, n_in, l == 1 ? c->hit : nullptr,
                                               l == 1 ? nullptr : c->pyr[l - 1].ok, Wi, P.mv,
                                               P.mn, P.mt, P.ok, P.H, P.W);
-    lv[l] = IcpLevel{P.H,   P.W,    P.blocks, P.bpx, P.px,    c->prm.icp_coarse_iters[l - 1],
+    lv[l] = IcpLevel{P.H,   P.W,    P.blocks, P.bpx, P.px, P.threads, c->prm.icp_coarse_iters[l - 1],
                      cam,   P.vmap, P.nmap,   P.mv,  P.mn,    P.mt,
                      nullptr, nullptr, nullptr, nullptr};
   }
@@ -3517,7 +3548,7 @@ static hipError_t build_pyramid(kf_context* c, const kf_cam& cam0, IcpLevel* lv)
 hipError_t kf_launch_icp(kf_context* c, const kf_cam& cam, bool maps_from_depth) {
   IcpLevel lv[KF_MAX_ICP_LEVELS];
   lv[0] = IcpLevel{c->H,    c->W,    c->icp_blocks, c->icp_bpx,
-                   c->icp_px, c->prm.icp_iterations, cam, c->vmap,
+                   c->icp_px, c->icp_threads, c->prm.icp_iterations, cam, c->vmap,
                    c->nmap, c->model_v, c->model_n, c->model_nt,
                    maps_from_depth ? c->depth : nullptr, c->vmap, c->nmap, c->mask};
   const int n_levels = c->prm.icp_levels > 1 ? c->prm.icp_levels : 1;
