@@ -761,7 +761,10 @@ int kf_synchronize(kf_context* c) {
 int kf_device_bytes(const kf_context* c, size_t* bytes) {
   KF_REQUIRE(c && bytes, KF_ERR_INVALID, "null argument");
   const size_t n = (size_t)c->H * c->W;
-  size_t b = c->volume_bytes + n * (4 + 4 + 2 + 12 + 12 + 1 + 12 + 12 + 8 + 4 + 8) +
+  // per pixel: depth, filtered depth, u16 staging, vertex and normal maps,
+  // mask, model vertex, normal and normal factor, trilinear plane, hit index,
+  // integration table, colour
+  size_t b = c->volume_bytes + n * (4 + 4 + 2 + 12 + 12 + 1 + 12 + 12 + 4 + 8 + 4 + 8 + 3) +
              sizeof(kf_dev_state) +
              (size_t)2 * c->icp_blocks * KF_ICP_SUMS * 8 +
              kf_ray_queue_entries(c->H, c->W) * kf_ray_state_bytes() +
@@ -769,7 +772,7 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
              (size_t)6 * KF_RAY_PARTS * KF_RAY_COUNT_STRIDE * 4;
   if (c->icp_masks) b += (size_t)KF_MAX_ICP_ITERS * n;
   for (const kf_pyr_level& P : c->pyr)
-    if (P.depth) b += (size_t)P.H * P.W * (4 + 12 + 12 + 1 + 12 + 12 + 1);
+    if (P.depth) b += (size_t)P.H * P.W * (4 + 12 + 12 + 1 + 12 + 12 + 4 + 1);
   if (c->frames) b += (size_t)c->n_frames * n * 2 + (size_t)c->n_frames * 64;
   *bytes = b;
   return KF_OK;
