@@ -2,9 +2,9 @@
 
 forward(tsdf, xi) renders the camera depth map at T = exp(xi^) T0 through
 the C ABI (kf_raycast in KF_RAYCAST_TRILINEAR mode); backward calls
-kf_raycast_adjoint, which walks every ray again (the inputs are uploaded
-again first, which drops the forward's saved crossings) and returns dL/dTSDF and
-dL/dxi.  dL/dxi is the left-perturbation gradient at T, i.e. the gradient
+kf_raycast_adjoint, which (the inputs are uploaded again first, which drops
+the forward's saved crossings) runs the trilinear raycast into its own maps,
+then the adjoint from those crossings, and returns dL/dTSDF and dL/dxi.  dL/dxi is the left-perturbation gradient at T, i.e. the gradient
 with respect to xi at xi = 0.  torch only carries the arrays here; every
 computation runs in libkinfu_amd.so.
 """
