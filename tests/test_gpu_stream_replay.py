@@ -24,7 +24,16 @@ frame are bit-exact as well.
   sensor_pose_estimation.cu:48, 69): 30 frames of the room stream rendered
   with that K, checkpoints after frames 10, 20 and 30.  At 300 mm / 10 mm the
   raycast's first pass runs ceil((1.3 * 30 + 2) / 8) = 6 batches of 8 steps,
-  its multi-batch instantiation.
+  its multi-batch instantiation;
+* the 5 mm precision workload (720x720x600 @5 mm = 311M voxels, FP16 TSDF +
+  u16 weight, trunc 15 mm, band [-15, 150], compensated ICP sums; the stream
+  with half the sensor noise): all 300 frames, checkpoints after frames 150
+  and 300.  Bit-exact here means the same half and u16 bits;
+* the 1280x720 workload (1024x1024x768 @4 mm = 805M voxels, FP16 + u16,
+  fx = fy = 1000, trunc 20 mm, 12 spheres and 8 boxes, raycast skip rounds
+  on): all 200 frames, checkpoints after frames 100 and 200.  The oracle has
+  no skip rounds, so this also checks that they leave the maps unchanged
+  over a whole stream.
 """
 import time
 
@@ -74,20 +83,32 @@ def run_gpu(kf, vol, prm, K, H, W, frames, checkpoints):
     return poses, snaps
 
 
-def replay(oracle, vol, prm, K, frames, poses, snaps, what):
-    """Integrates every frame at the GPU's pose and compares each checkpoint."""
-    t, w = oracle.new_volume(vol.dims, "f32")
+def replay(oracle, vol, prm, K, frames, poses, snaps, what, kind="f32"):
+    """Integrates every frame at the GPU's pose and compares each checkpoint.
+
+    kind "f16": the oracle keeps half bits and u16 counts; the GPU's
+    reference-layout download widens half to float exactly and gives NaN for
+    a weight of 0, so the oracle's volume is widened the same way."""
+    t, w = oracle.new_volume(vol.dims, kind)
     t0 = time.perf_counter()
     last = max(snaps)
     for k in range(last):
         d = gated(oracle, frames[k], prm)
         oracle.integrate(d, K, poses[k], vol, prm, k == 0, t, w)
+        if (k + 1) % 50 == 0 and k + 1 not in snaps:
+            say(f"[gpu] {what}: oracle replay at frame {k + 1} "
+                f"({time.perf_counter() - t0:.1f} s)")
         if k + 1 in snaps:
             gt_t, gt_w, maps = snaps[k + 1]
             say(f"[gpu] {what}: checkpoint after frame {k + 1} "
                 f"({time.perf_counter() - t0:.1f} s of oracle replay)")
-            assert_same(gt_w, w, f"{what} weight after frame {k + 1}")
-            assert_same(gt_t, t, f"{what} tsdf after frame {k + 1}")
+            if kind == "f16":
+                assert_same(gt_w, np.where(w == 0, np.float32(np.nan), w.astype(np.float32)),
+                            f"{what} u16 weight after frame {k + 1}")
+                assert_same(gt_t, oracle.f16_bits_to_f32(t), f"{what} half tsdf after frame {k + 1}")
+            else:
+                assert_same(gt_w, w, f"{what} weight after frame {k + 1}")
+                assert_same(gt_t, t, f"{what} tsdf after frame {k + 1}")
             want = oracle.raycast(d, K, poses[k], t, vol, prm.truncation_mm,
                                   prm.raycast_fix_bounds)
             assert_same(maps["hit"], want["hit"], f"{what} hit index at frame {k}")
@@ -124,3 +145,24 @@ def test_reference_preset_30_frames_volume_and_maps_bitexact(kf, oracle):
     t, w = replay(oracle, vol, prm, K, frames, poses, snaps, "reference preset")
     # the wide band updates voxels up to 300 mm in front of the surface
     assert np.isfinite(w).sum() > 200_000
+
+
+def observed(w):
+    return int((w > 0).sum()) if w.dtype == np.uint16 else int(np.isfinite(w).sum())
+
+
+@pytest.mark.parametrize("preset,n,checkpoints,noise", [
+    ("PRESET_ROOM_5MM_F16", 300, (150, 300), 0.5),
+    ("PRESET_ROOM_HD_F16", 200, (100, 200), 1.0),
+], ids=["5mm_f16_300", "hd_f16_200"])
+def test_f16_workload_stream_volume_and_maps_bitexact(kf, oracle, preset, n, checkpoints, noise):
+    code = getattr(kf, preset)
+    vol, prm, K, H, W = kf.preset(code)
+    assert vol.precision == kf.KF_TSDF_F16
+    frames, gt = kf.stream(kf.preset_scene(code), K, H, W, 0, n, noise_scale=noise, threads=16)
+    say(f"[gpu] {preset}: {n} frames {W}x{H} into {list(vol.dims)} @{vol.voxel_mm}mm (F16)")
+    poses, snaps = run_gpu(kf, vol, prm, K, H, W, frames, checkpoints)
+    err = np.linalg.norm(poses[:, :3, 3] - gt[:, :3, 3], axis=1)
+    assert err.max() < 60.0, f"tracking lost (max {err.max():.1f} mm)"
+    t, w = replay(oracle, vol, prm, K, frames, poses, snaps, preset, kind="f16")
+    assert np.isfinite(snaps[checkpoints[0]][1]).sum() < observed(w)
