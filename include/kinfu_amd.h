@@ -194,7 +194,12 @@ int kf_device_bytes(const kf_context* ctx, size_t* bytes);
 
 /* ---- A1: depth input (system_utility.cpp:58-68, 77-86) -------------------
  * Converts uint16 mm to float mm and zeroes values outside
- * [depth_min_mm, depth_max_mm] (inclusive both ends, cv::inRange). */
+ * [depth_min_mm, depth_max_mm] (inclusive both ends, cv::inRange).
+ * mem_kind KF_MEM_HOST: the frame is copied into the context and the range
+ * gate is enqueued by the next call that reads the depth; with the bilateral
+ * filter on, kf_pipeline_initial / _incoming(KF_FRAME_CURRENT) gate and
+ * filter it in one kernel.  KF_MEM_DEVICE: gated at once (the caller's
+ * buffer is not kept). */
 int kf_upload_depth_u16(kf_context* ctx, const uint16_t* depth, int32_t mem_kind);
 /* Loads an already-gated float depth map unchanged (stage tests, callers
  * holding float depth as the reference's cv::Mat CV_32F does). */

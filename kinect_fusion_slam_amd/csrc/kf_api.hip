@@ -780,18 +780,44 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
 
 // ---- A1 -------------------------------------------------------------------
 
+// Runs the range gate of a host-uploaded frame (depth_u16 -> depth) if it
+// is still pending.
+static int flush_depth_gate(kf_context* c) {
+  if (!c->depth_gate_pending) return KF_OK;
+  c->depth_gate_pending = false;
+  kf_stage_begin(c, KF_STAGE_DEPTH);
+  KF_LAUNCH(kf_launch_depth_u16(c, c->depth_u16));
+  kf_stage_end(c, KF_STAGE_DEPTH);
+  return KF_OK;
+}
+
+// The working depth must be loaded; a pending range gate runs first.
+static int need_depth(kf_context* c) {
+  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  return flush_depth_gate(c);
+}
+#define KF_NEED_DEPTH(c)            \
+  do {                              \
+    const int rc_ = need_depth(c);  \
+    if (rc_ != KF_OK) return rc_;   \
+  } while (0)
+
 int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) {
   KF_REQUIRE(c && depth, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
-  kf_stage_begin(c, KF_STAGE_DEPTH);
-  const uint16_t* src = depth;
   if (mem_kind == KF_MEM_HOST) {
+    // The copy is ours: the range gate waits for its first reader, so that
+    // kf_pipeline_incoming(KF_FRAME_CURRENT) can gate and filter in one pass.
     KF_HIP_CHECK(hipMemcpyAsync(c->depth_u16, depth, (size_t)c->H * c->W * 2,
                                 hipMemcpyHostToDevice, c->stream));
-    src = c->depth_u16;
+    c->depth_gate_pending = true;
+  } else {
+    // the caller's device buffer may change after this call: gate it now
+    c->depth_gate_pending = false;
+    kf_stage_begin(c, KF_STAGE_DEPTH);
+    KF_LAUNCH(kf_launch_depth_u16(c, depth));
+    kf_stage_end(c, KF_STAGE_DEPTH);
   }
-  KF_LAUNCH(kf_launch_depth_u16(c, src));
-  kf_stage_end(c, KF_STAGE_DEPTH);
   c->have_depth = true;
   return KF_OK;
 }
@@ -799,6 +825,7 @@ int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) 
 int kf_upload_depth_f32(kf_context* c, const float* depth_mm, int32_t mem_kind) {
   KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
+  c->depth_gate_pending = false;
   KF_HIP_CHECK(hipMemcpyAsync(c->depth, depth_mm, (size_t)c->H * c->W * 4,
                               mem_kind == KF_MEM_HOST ? hipMemcpyHostToDevice
                                                       : hipMemcpyDeviceToDevice,
@@ -822,6 +849,8 @@ int kf_read_depth_file(const char* path, int32_t height, int32_t width, uint16_t
 
 int kf_download_depth(kf_context* c, float* depth_mm) {
   KF_REQUIRE(c && depth_mm, KF_ERR_INVALID, "null argument");
+  const int rc = flush_depth_gate(c);
+  if (rc != KF_OK) return rc;
   KF_HIP_CHECK(hipMemcpyAsync(depth_mm, c->depth, (size_t)c->H * c->W * 4, hipMemcpyDeviceToHost,
                               c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -833,7 +862,7 @@ int kf_download_depth(kf_context* c, float* depth_mm) {
 int kf_bilateral(kf_context* c) {
   KF_REQUIRE(c, KF_ERR_INVALID, "null context");
   c->saved_ok = false;
-  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  KF_NEED_DEPTH(c);
   kf_stage_begin(c, KF_STAGE_BILATERAL);
   KF_LAUNCH(kf_launch_bilateral(c));
   kf_stage_end(c, KF_STAGE_BILATERAL);
@@ -845,7 +874,7 @@ int kf_bilateral(kf_context* c) {
 int kf_vertex_normals(kf_context* c, const float K[9]) {
   KF_REQUIRE(c && K, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
-  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  KF_NEED_DEPTH(c);
   kf_cam cam;
   kf_make_cam(K, &cam);
   kf_stage_begin(c, KF_STAGE_VERTEX);
@@ -864,7 +893,11 @@ int kf_download_vertex_maps(kf_context* c, float* vx, float* vy, float* vz, floa
   KF_HIP_CHECK(hipMemcpyAsync(N.data(), c->nmap, n * 12, hipMemcpyDeviceToHost, c->stream));
   if (mask)
     KF_HIP_CHECK(hipMemcpyAsync(mask, c->mask, n, hipMemcpyDeviceToHost, c->stream));
-  if (vz) KF_HIP_CHECK(hipMemcpyAsync(vz, c->depth, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (vz) {
+    const int rc = flush_depth_gate(c);
+    if (rc != KF_OK) return rc;
+    KF_HIP_CHECK(hipMemcpyAsync(vz, c->depth, n * 4, hipMemcpyDeviceToHost, c->stream));
+  }
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < n; ++i) {
     if (vx) vx[i] = V[i].x;
@@ -890,6 +923,7 @@ int kf_upload_vertex_maps(kf_context* c, const float* vx, const float* vy, const
   KF_HIP_CHECK(hipMemcpyAsync(c->vmap, V.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipMemcpyAsync(c->nmap, N.data(), n * 12, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipMemcpyAsync(c->mask, mask, n, hipMemcpyHostToDevice, c->stream));
+  c->depth_gate_pending = false;
   KF_HIP_CHECK(hipMemcpyAsync(c->depth, vz, n * 4, hipMemcpyHostToDevice, c->stream));
   KF_HIP_CHECK(hipStreamSynchronize(c->stream));
   c->have_depth = true;
@@ -1090,7 +1124,7 @@ static int integrate_at_current_pose(kf_context* c, const kf_cam& cam, int first
 int kf_integrate(kf_context* c, const float K[9], const float T[16], int32_t first_frame) {
   KF_REQUIRE(c && K && T, KF_ERR_INVALID, "null argument");
   c->saved_ok = false;
-  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  KF_NEED_DEPTH(c);
   int rc = kf_set_pose(c, T);
   if (rc != KF_OK) return rc;
   kf_cam cam;
@@ -1247,7 +1281,7 @@ static int raycast_at_current_pose(kf_context* c, const kf_cam& cam) {
 
 int kf_raycast(kf_context* c, const float K[9], const float T[16]) {
   KF_REQUIRE(c && K && T, KF_ERR_INVALID, "null argument");
-  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  KF_NEED_DEPTH(c);
   int rc = kf_set_pose(c, T);
   if (rc != KF_OK) return rc;
   kf_cam cam;
@@ -1287,7 +1321,7 @@ int kf_refine_pose(kf_context* c, const float K[9], int32_t steps, float T_inout
   KF_REQUIRE(c && K && steps >= 0 && steps <= KF_MAX_REFINE_STEPS, KF_ERR_INVALID,
              "bad argument");
   c->saved_ok = false;
-  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  KF_NEED_DEPTH(c);
   if (T_inout) {
     int rc = kf_set_pose(c, T_inout);
     if (rc != KF_OK) return rc;
@@ -1393,7 +1427,7 @@ extern "C" {
 int kf_raycast_adjoint(kf_context* c, const float K[9], const float T[16], const float* dl_ddepth,
                        const float* dl_dnormal, float* dl_dtsdf, double dl_dxi[6]) {
   KF_REQUIRE(c && K && T && dl_ddepth, KF_ERR_INVALID, "null argument");
-  KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+  KF_NEED_DEPTH(c);
   const size_t n = (size_t)c->H * c->W;
   int rc = adjoint_buffers(c);
   if (rc != KF_OK) return rc;
@@ -1666,8 +1700,19 @@ int kf_stream_upload_u16(kf_context* c, const uint16_t* frames, int32_t n_frames
 
 static int load_stream_frame(kf_context* c, int32_t frame) {
   if (frame == KF_FRAME_CURRENT) {
-    // the depth the caller uploaded (already range-gated)
+    // the depth the caller uploaded
     KF_REQUIRE(c->have_depth, KF_ERR_STATE, "no depth frame loaded");
+    if (c->depth_gate_pending && c->prm.bilateral) {
+      // a uint16 frame from the host: range gate and filter in one kernel,
+      // as for a frame of the resident stream
+      c->depth_gate_pending = false;
+      kf_stage_begin(c, KF_STAGE_BILATERAL);
+      KF_LAUNCH(kf_launch_depth_bilateral(c, c->depth_u16));
+      kf_stage_end(c, KF_STAGE_BILATERAL);
+      return KF_OK;
+    }
+    const int rc = flush_depth_gate(c);
+    if (rc != KF_OK) return rc;
     if (c->prm.bilateral) {
       kf_stage_begin(c, KF_STAGE_BILATERAL);
       KF_LAUNCH(kf_launch_bilateral(c));
@@ -1675,6 +1720,7 @@ static int load_stream_frame(kf_context* c, int32_t frame) {
     }
     return KF_OK;
   }
+  c->depth_gate_pending = false;
   KF_REQUIRE(c->frames && frame >= 0 && frame < c->n_frames, KF_ERR_INVALID,
              "frame outside the uploaded stream");
   const uint16_t* src = c->frames + (size_t)frame * c->H * c->W;

@@ -203,6 +203,10 @@ struct kf_context {
   double stage_ms[KF_STAGE_COUNT] = {};
   int64_t stage_count[KF_STAGE_COUNT] = {};
   bool have_depth = false;
+  // depth_u16 holds a frame uploaded from the host whose range gate into
+  // depth has not run yet: the host-fed pipeline gates and filters it in
+  // one kernel; any other reader of depth runs the gate first.
+  bool depth_gate_pending = false;
   bool count_rays = false;  // raycast work counters on
   int ray_slot = 0;         // ray_counts slot of the next raycast
   // raycast adjoint buffers (allocated on first use)
