@@ -513,6 +513,14 @@ int kf_destroy(kf_context* c) {
   if (c->jpeg_coef_host) (void)hipHostFree(c->jpeg_coef_host);
   if (c->jpeg_q_host) (void)hipHostFree(c->jpeg_q_host);
   if (c->jpeg_copied) (void)hipEventDestroy(c->jpeg_copied);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  for (int i = 0; i < 2; ++i) {
+    if (c->up_host[i]) (void)hipHostFree(c->up_host[i]);
+    if (c->up_copied[i]) (void)hipEventDestroy(c->up_copied[i]);
+    if (c->up_consumed[i]) (void)hipEventDestroy(c->up_consumed[i]);
+  }
+  if (c->up_dev[1]) (void)hipFree(c->up_dev[1]);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   for (kf_pyr_level& P : c->pyr) {
     void* pb[] = {P.depth, P.vmap, P.nmap, P.mask, P.mv, P.mn, P.mt, P.ok};
     for (void* b : pb)
@@ -774,20 +782,68 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
   for (const kf_pyr_level& P : c->pyr)
     if (P.depth) b += (size_t)P.H * P.W * (4 + 12 + 12 + 1 + 12 + 12 + 4 + 1);
   if (c->frames) b += (size_t)c->n_frames * n * 2 + (size_t)c->n_frames * 64;
+  if (c->up_dev[1]) b += n * 2;
   *bytes = b;
   return KF_OK;
 }
 
 // ---- A1 -------------------------------------------------------------------
 
-// Runs the range gate of a host-uploaded frame (depth_u16 -> depth) if it
-// is still pending.
+// Runs the range gate of a host-uploaded frame (up_dev[up_cur] -> depth) if
+// it is still pending.
 static int flush_depth_gate(kf_context* c) {
   if (!c->depth_gate_pending) return KF_OK;
   c->depth_gate_pending = false;
   kf_stage_begin(c, KF_STAGE_DEPTH);
-  KF_LAUNCH(kf_launch_depth_u16(c, c->depth_u16));
+  KF_LAUNCH(kf_launch_depth_u16(c, c->up_dev[c->up_cur]));
   kf_stage_end(c, KF_STAGE_DEPTH);
+  KF_HIP_CHECK(hipEventRecord(c->up_consumed[c->up_cur], c->stream));
+  return KF_OK;
+}
+
+// The two upload slots, their pinned buffers, events and copy stream, all
+// or none (created on the first host upload).
+static int upload_slots(kf_context* c) {
+  if (c->copy_stream) return KF_OK;
+  const size_t bytes = (size_t)c->H * c->W * 2;
+  bool ok = hipMalloc((void**)&c->up_dev[1], bytes) == hipSuccess;
+  if (!ok) c->up_dev[1] = nullptr;
+  for (int i = 0; i < 2 && ok; ++i) {
+    ok = hipHostMalloc((void**)&c->up_host[i], bytes, hipHostMallocDefault) == hipSuccess;
+    if (!ok) c->up_host[i] = nullptr;
+  }
+  for (int i = 0; i < 2 && ok; ++i) {
+    ok = hipEventCreateWithFlags(&c->up_copied[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) c->up_copied[i] = nullptr;
+    if (ok) ok = hipEventCreateWithFlags(&c->up_consumed[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) c->up_consumed[i] = nullptr;
+  }
+  hipStream_t s = nullptr;
+  if (ok) ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+  // every event starts recorded, so the first waits on it return at once
+  for (int i = 0; i < 2 && ok; ++i)
+    ok = hipEventRecord(c->up_copied[i], s) == hipSuccess &&
+         hipEventRecord(c->up_consumed[i], c->stream) == hipSuccess;
+  if (!ok) {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2; ++i) {
+      if (c->up_host[i]) (void)hipHostFree(c->up_host[i]);
+      if (c->up_copied[i]) (void)hipEventDestroy(c->up_copied[i]);
+      if (c->up_consumed[i]) (void)hipEventDestroy(c->up_consumed[i]);
+      c->up_host[i] = nullptr;
+      c->up_copied[i] = c->up_consumed[i] = nullptr;
+    }
+    if (c->up_dev[1]) (void)hipFree(c->up_dev[1]);
+    c->up_dev[1] = nullptr;
+    (void)hipGetLastError();
+    return kf_fail(KF_ERR_OOM, "cannot allocate the host upload slots");
+  }
+  c->up_dev[0] = c->depth_u16;
+  c->copy_stream = s;
   return KF_OK;
 }
 
@@ -808,8 +864,21 @@ int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) 
   if (mem_kind == KF_MEM_HOST) {
     // The copy is ours: the range gate waits for its first reader, so that
     // kf_pipeline_incoming(KF_FRAME_CURRENT) can gate and filter in one pass.
-    KF_HIP_CHECK(hipMemcpyAsync(c->depth_u16, depth, (size_t)c->H * c->W * 2,
-                                hipMemcpyHostToDevice, c->stream));
+    int rc = upload_slots(c);
+    if (rc != KF_OK) return rc;
+    const size_t bytes = (size_t)c->H * c->W * 2;
+    const int i = c->up_next;
+    // the slot's pinned buffer is free once its previous copy is done
+    KF_HIP_CHECK(hipEventSynchronize(c->up_copied[i]));
+    memcpy(c->up_host[i], depth, bytes);
+    // its device frame is free once the previous frame in it has been read
+    KF_HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->up_consumed[i], 0));
+    KF_HIP_CHECK(hipMemcpyAsync(c->up_dev[i], c->up_host[i], bytes, hipMemcpyHostToDevice,
+                                c->copy_stream));
+    KF_HIP_CHECK(hipEventRecord(c->up_copied[i], c->copy_stream));
+    KF_HIP_CHECK(hipStreamWaitEvent(c->stream, c->up_copied[i], 0));
+    c->up_cur = i;
+    c->up_next = i ^ 1;
     c->depth_gate_pending = true;
   } else {
     // the caller's device buffer may change after this call: gate it now
@@ -1707,8 +1776,9 @@ static int load_stream_frame(kf_context* c, int32_t frame) {
       // as for a frame of the resident stream
       c->depth_gate_pending = false;
       kf_stage_begin(c, KF_STAGE_BILATERAL);
-      KF_LAUNCH(kf_launch_depth_bilateral(c, c->depth_u16));
+      KF_LAUNCH(kf_launch_depth_bilateral(c, c->up_dev[c->up_cur]));
       kf_stage_end(c, KF_STAGE_BILATERAL);
+      KF_HIP_CHECK(hipEventRecord(c->up_consumed[c->up_cur], c->stream));
       return KF_OK;
     }
     const int rc = flush_depth_gate(c);

@@ -142,7 +142,7 @@ struct kf_context {
   // images (H*W)
   float* depth = nullptr;       // working depth, float mm
   float* depth_tmp = nullptr;   // bilateral output
-  uint16_t* depth_u16 = nullptr;// staging for host uint16 uploads
+  uint16_t* depth_u16 = nullptr;// device frame of uint16 uploads (slot 0)
   kf_p3* vmap = nullptr;        // {Vx, Vy, Vz}, Vz = depth where the mask is 1, else 0
   kf_p3* nmap = nullptr;        // {Nx, Ny, Nz}
   uint8_t* mask = nullptr;      // vertex mask (reference layout)
@@ -207,6 +207,19 @@ struct kf_context {
   // depth has not run yet: the host-fed pipeline gates and filters it in
   // one kernel; any other reader of depth runs the gate first.
   bool depth_gate_pending = false;
+  // Uploads of uint16 frames from host memory use two slots in turn: the
+  // caller's frame is copied into a pinned buffer, then onto the device by
+  // copy_stream, so that the copy runs beside the previous frame's kernels.
+  // up_copied[i]: slot i's copy done (recorded on copy_stream);
+  // up_consumed[i]: the last reader of slot i's device frame done (recorded
+  // on stream).  Created on the first host upload.
+  hipStream_t copy_stream = nullptr;
+  uint16_t* up_host[2] = {nullptr, nullptr};
+  uint16_t* up_dev[2] = {nullptr, nullptr};  // up_dev[0] = depth_u16
+  hipEvent_t up_copied[2] = {nullptr, nullptr};
+  hipEvent_t up_consumed[2] = {nullptr, nullptr};
+  int up_next = 0;  // slot of the next host upload
+  int up_cur = 0;   // slot of the frame whose gate is pending
   bool count_rays = false;  // raycast work counters on
   int ray_slot = 0;         // ray_counts slot of the next raycast
   // raycast adjoint buffers (allocated on first use)
