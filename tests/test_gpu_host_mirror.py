@@ -72,10 +72,10 @@ def test_host_mirror_from_dat_files_equals_pipeline_run(kf, tmp_path):
 
 def test_host_fed_frames_equal_the_resident_stream(kf):
     """kf_pipeline_initial / kf_pipeline_incoming with KF_FRAME_CURRENT take
-    the depth last uploaded from host memory (range gate at upload, then the
-    bilateral filter on the float depth) instead of a resident stream frame
-    (gate fused into the filter): the same values, so the pose after every
-    frame and the final volume are bit-exact (room preset, bilateral 5x5)."""
+    the depth last uploaded from host memory (copied through the two upload
+    slots in turn, then gated and filtered by the fused kernel) instead of a
+    resident stream frame: the pose after every frame and the final volume
+    are bit-exact (room preset, bilateral 5x5)."""
     vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
     n = 5
     frames, gt = kf.stream(kf.preset_scene(kf.PRESET_ROOM), K, H, W, 0, n, threads=16)
@@ -99,3 +99,30 @@ def test_host_fed_frames_equal_the_resident_stream(kf):
     assert_same(np.array(fed), want, "poses")
     assert_same(w_fed, w, "weight")
     assert_same(t_fed, t, "tsdf")
+
+
+def test_host_upload_slots(kf, oracle):
+    """kf_upload_depth_u16 from host memory copies each frame through one of
+    two pinned buffers into one of two device slots, in turn, and its range
+    gate waits for the first reader of the depth.  Three uploads with no
+    reader between: the last one is the depth.  Frames read back right after
+    each upload (both slots in turn) equal the oracle's gate.  A float upload
+    after a host upload replaces the frame whose gate is pending."""
+    vol, prm, K, H, W = kf.preset(kf.PRESET_ROOM)
+    frames, gt = kf.stream(kf.preset_scene(kf.PRESET_ROOM), K, H, W, 0, 6, threads=16)
+
+    def gate(f):
+        return oracle.range_gate(frames[f], prm.depth_min_mm, prm.depth_max_mm)
+
+    say("[gpu] host upload slots")
+    with kf.KinfuContext(vol, prm, H, W) as ctx:
+        for f in (0, 1, 2):
+            ctx.upload_depth_u16(frames[f])
+        assert_same(ctx.depth(), gate(2), "last of three uploads")
+        for f in range(6):
+            ctx.upload_depth_u16(frames[f])
+            assert_same(ctx.depth(), gate(f), f"upload {f}")
+        ctx.upload_depth_u16(frames[0])
+        ctx.upload_depth_f32(gate(3))
+        assert_same(ctx.depth(), gate(3), "float upload after a host upload")
+        assert (gate(3) > 0).mean() > 0.5
