@@ -789,11 +789,20 @@ int kf_device_bytes(const kf_context* c, size_t* bytes) {
 
 // ---- A1 -------------------------------------------------------------------
 
+// Before the first reader of a host upload is enqueued: its copy to the
+// device is done.
+static int wait_upload(kf_context* c) {
+  KF_HIP_CHECK(hipEventSynchronize(c->up_copied[c->up_cur]));
+  return KF_OK;
+}
+
 // Runs the range gate of a host-uploaded frame (up_dev[up_cur] -> depth) if
 // it is still pending.
 static int flush_depth_gate(kf_context* c) {
   if (!c->depth_gate_pending) return KF_OK;
   c->depth_gate_pending = false;
+  const int rc = wait_upload(c);
+  if (rc != KF_OK) return rc;
   kf_stage_begin(c, KF_STAGE_DEPTH);
   KF_LAUNCH(kf_launch_depth_u16(c, c->up_dev[c->up_cur]));
   kf_stage_end(c, KF_STAGE_DEPTH);
@@ -868,15 +877,17 @@ int kf_upload_depth_u16(kf_context* c, const uint16_t* depth, int32_t mem_kind) 
     if (rc != KF_OK) return rc;
     const size_t bytes = (size_t)c->H * c->W * 2;
     const int i = c->up_next;
-    // the slot's pinned buffer is free once its previous copy is done
+    // The two streams are ordered from the host (a stream waiting on another
+    // stream's event costs the host more than the copy itself): the slot's
+    // pinned buffer is free once its previous copy is done, and its device
+    // frame once the previous frame in it has been read; the reader waits
+    // for this copy (wait_upload) before it is enqueued.
     KF_HIP_CHECK(hipEventSynchronize(c->up_copied[i]));
     memcpy(c->up_host[i], depth, bytes);
-    // its device frame is free once the previous frame in it has been read
-    KF_HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->up_consumed[i], 0));
+    KF_HIP_CHECK(hipEventSynchronize(c->up_consumed[i]));
     KF_HIP_CHECK(hipMemcpyAsync(c->up_dev[i], c->up_host[i], bytes, hipMemcpyHostToDevice,
                                 c->copy_stream));
     KF_HIP_CHECK(hipEventRecord(c->up_copied[i], c->copy_stream));
-    KF_HIP_CHECK(hipStreamWaitEvent(c->stream, c->up_copied[i], 0));
     c->up_cur = i;
     c->up_next = i ^ 1;
     c->depth_gate_pending = true;
@@ -1775,6 +1786,8 @@ static int load_stream_frame(kf_context* c, int32_t frame) {
       // a uint16 frame from the host: range gate and filter in one kernel,
       // as for a frame of the resident stream
       c->depth_gate_pending = false;
+      const int rc = wait_upload(c);
+      if (rc != KF_OK) return rc;
       kf_stage_begin(c, KF_STAGE_BILATERAL);
       KF_LAUNCH(kf_launch_depth_bilateral(c, c->up_dev[c->up_cur]));
       kf_stage_end(c, KF_STAGE_BILATERAL);

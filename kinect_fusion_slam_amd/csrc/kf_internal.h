@@ -212,7 +212,8 @@ struct kf_context {
   // copy_stream, so that the copy runs beside the previous frame's kernels.
   // up_copied[i]: slot i's copy done (recorded on copy_stream);
   // up_consumed[i]: the last reader of slot i's device frame done (recorded
-  // on stream).  Created on the first host upload.
+  // on stream).  The host waits for these events; neither stream waits on
+  // the other's.  Created on the first host upload.
   hipStream_t copy_stream = nullptr;
   uint16_t* up_host[2] = {nullptr, nullptr};
   uint16_t* up_dev[2] = {nullptr, nullptr};  // up_dev[0] = depth_u16
