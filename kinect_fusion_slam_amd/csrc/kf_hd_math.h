@@ -186,34 +186,39 @@ KF_HD int kf_m3_inverse(const float* in, float* out) {
 // return value; when a pivot is not positive the right-hand side is left in
 // place and used as the increment.  Here: LDL^T without pivoting in FP64;
 // a pivot below FLT_EPSILON (or NaN) reports failure and the caller then uses
-// b itself, which reproduces that behaviour.
+// b itself, which reproduces that behaviour.  Each update of a running sum
+// is one fused multiply-add with a stored product LD[j][k] = L[j][k] d_k
+// (the column's sum before its division by the pivot d_k), so that the dependent
+// chain of the solve, which every ICP iteration waits for, is one operation
+// per term.  fma is correctly rounded on the host and on the device, so
+// both compute the same bits.
 #define KF_LDLT_EPS 1.1920928955078125e-07
 
 KF_HD int kf_ldlt6_solve(const double* A, const double* b, double* x) {
-  double L[6][6];
-  double d[6], rd[6];  // pivots and their reciprocals (6 divisions in total)
+  double L[6][6], LD[6][6];
+  double rd[6];  // reciprocals of the pivots (6 divisions in total)
   KF_UNROLL for (int i = 0; i < 6; ++i) {
     KF_UNROLL for (int j = 0; j < i; ++j) {
       double s = A[i * 6 + j];
-      KF_UNROLL for (int k = 0; k < j; ++k) s = s - L[i][k] * L[j][k] * d[k];
+      KF_UNROLL for (int k = 0; k < j; ++k) s = fma(-L[i][k], LD[j][k], s);
+      LD[i][j] = s;
       L[i][j] = s * rd[j];
     }
     double s = A[i * 6 + i];
-    KF_UNROLL for (int k = 0; k < i; ++k) s = s - L[i][k] * L[i][k] * d[k];
+    KF_UNROLL for (int k = 0; k < i; ++k) s = fma(-L[i][k], LD[i][k], s);
     if (!(s >= KF_LDLT_EPS)) return 1;
-    d[i] = s;
     rd[i] = 1.0 / s;
   }
   double y[6];
   KF_UNROLL for (int i = 0; i < 6; ++i) {
     double s = b[i];
-    KF_UNROLL for (int k = 0; k < i; ++k) s = s - L[i][k] * y[k];
+    KF_UNROLL for (int k = 0; k < i; ++k) s = fma(-L[i][k], y[k], s);
     y[i] = s;
   }
   KF_UNROLL for (int i = 0; i < 6; ++i) y[i] = y[i] * rd[i];
   KF_UNROLL for (int i = 5; i >= 0; --i) {
     double s = y[i];
-    KF_UNROLL for (int k = i + 1; k < 6; ++k) s = s - L[k][i] * x[k];
+    KF_UNROLL for (int k = i + 1; k < 6; ++k) s = fma(-L[k][i], x[k], s);
     x[i] = s;
   }
   return 0;
